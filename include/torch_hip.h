@@ -1,0 +1,138 @@
+/* C ABI of libtorchhip.so: TORCH's Cartesian finite-volume step on MI355X.
+ *
+ * The reference (BFG508/TORCH, python/) has no native interface; its drop-in
+ * boundary for this path is the CPU flux-worker backend
+ * `Execution(workers).map(fn, batches)` (python/cfdSolver/torchExecution.py:17-68)
+ * that `hllc_flux_grid` (torchFlux.py:66-74), `_hydro` (torchSolver3D.py:182-189),
+ * `muscl_hancock_step_2d` (torchSolver2D.py:388-410) and the 1D HLLC
+ * (torchSolver.py:542-560) hand their face batches to, under the step driver
+ * `Simulation.step` (torchSession.py:374-522). Each entry point below names
+ * the reference interface it replaces. All arrays crossing the boundary are
+ * host float64 in the reference layout: C order (dim+2, nz, ny, nx), x fastest.
+ *
+ * Every call returns a status (TORCH_OK, or one of the codes below); the
+ * message of the last failure on the calling thread is available from
+ * torch_last_error. No C++ exception crosses this boundary. A context
+ * serialises its calls with one mutex; each state owns its device buffers.
+ */
+#ifndef TORCH_HIP_H
+#define TORCH_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. Python maps 2, 5, 6, 7, 8 to RuntimeError, 3 to ValueError. */
+#define TORCH_OK 0
+#define TORCH_NONPHYSICAL 2       /* "...hydrodynamic update produced a nonphysical state" */
+#define TORCH_INVALID 3           /* invalid argument or chemistry input outside its domain */
+#define TORCH_HIP_ERROR 4         /* device/runtime failure */
+#define TORCH_STALLED 5           /* "Integration stalled before requested final time" */
+#define TORCH_NO_STABLE_STEP 6    /* "Could not find stable reactive step" */
+#define TORCH_SPECIES_RANGE 7     /* "3D scalar transport left the physical interval" */
+#define TORCH_REACTION_STATE 8    /* "Reaction produced a nonphysical state" */
+
+/* Enumerations shared with the host mirror (torch_amd/_abi.py). */
+enum torch_limiter { TORCH_VANLEER = 0, TORCH_ALBADA = 1, TORCH_MINBEE = 2, TORCH_SUPERBEE = 3, TORCH_UNOS = 4 };
+enum torch_boundary { TORCH_OPEN = 0, TORCH_PERIODIC = 1, TORCH_REFLECTIVE = 2, TORCH_INFLOW = 3 };
+enum torch_precision { TORCH_FP64 = 0, TORCH_FP32 = 1 };
+
+typedef struct torch_ctx torch_ctx;
+typedef struct torch_state torch_state;
+
+/* Session configuration: the numeric subset of Simulation.config
+ * (torchSession.py:66-92) for the Cartesian kernels. */
+typedef struct torch_config {
+    int dim;                   /* 1, 2 or 3 */
+    int n[3];                  /* counts x, y, z (unused axes 1) */
+    double lengths[3];         /* lengths x, y, z */
+    double gamma;              /* inert gamma */
+    int limiter;               /* enum torch_limiter */
+    int bc[3];                 /* enum torch_boundary per axis x, y, z */
+    double inlet_primitive[4]; /* 2D Inflow [rho, u, v, p] */
+    int reactive;              /* 0/1 */
+    int kinetic_model;         /* 0 arrhenius, 1 znd */
+    double q_heat, activation_energy, pre_exponential;
+    int controller;            /* 1D: 0 "Sound Speed Controlled", 1 "w=0" */
+    int variant;               /* 1D: 0 "Variante 1 - Original", 1 "Variante_2" */
+    int wavespeed;             /* 1D: 0 "Pressure Based", 1 "Enthalpy Based" */
+    double cfl;
+    int precision;             /* enum torch_precision */
+    int block_x, block_y, zchunk; /* 3D tile; 0 selects the tuned default */
+    double spacing[3];         /* cell sizes x, y, z; 0 derives lengths / n */
+    int gamma_mode;            /* 0 scalar gamma; 1 gamma(lambda) per cell (reactive);
+                                  2 per-cell gamma array uploaded in place of lambda */
+} torch_config;
+
+/* Per-step results (the values Simulation.step folds into its integrals). */
+typedef struct torch_step_info {
+    double dt;
+    int fallback;        /* whole-step first-order fallback was used */
+    int retries;         /* reactive dt halvings */
+    double outward[5];   /* dt*vol/h * sum(F_last - F_first), per field */
+    double source[5];    /* reactive source integral increment */
+    double wave_max[3];  /* CFL maxima of the committed state */
+} torch_step_info;
+
+/* Execution(workers) construction (torchExecution.py:17-24): one device
+ * context with its own HIP stream. */
+int torch_ctx_create(int device, torch_ctx** out);
+int torch_ctx_destroy(torch_ctx* ctx);
+/* Message of the calling thread's last failure. */
+int torch_last_error(char* buffer, size_t size);
+
+/* Device-resident session state (Simulation.state/lam/beta,
+ * torchSession.py:145-158), double-buffered so a failed step leaves the
+ * committed data intact (:374-377, 518-522). */
+int torch_state_create(torch_ctx* ctx, const torch_config* config, torch_state** out);
+int torch_state_destroy(torch_state* state);
+int torch_state_upload(torch_state* state, const double* U, const double* lam, const double* beta);
+int torch_state_download(torch_state* state, double* U, double* lam, double* beta);
+
+/* Simulation._dt on the committed state (torchSession.py:333-360). */
+int torch_state_dt(torch_state* state, double* dt_limit);
+
+/* Simulation.step (torchSession.py:374-522): CFL dt clipped to t_end - time,
+ * reactive R(dt/2) with <= 60 halvings, hydro with fallback, species
+ * transport, R(dt/2), balances; commits only on success. */
+int torch_state_step(torch_state* state, double time, double t_end, torch_step_info* info);
+
+/* One hydro update of the committed state with a given dt, committing the
+ * result: _hydro / muscl_hancock_step_3d (torchSolver3D.py:122-222),
+ * muscl_hancock_step_2d (torchSolver2D.py:274-466), the 1D session hydro
+ * (torchSession.py:413-437). When want_fluxes is set the face fluxes are kept
+ * for torch_state_fluxes. */
+int torch_state_hydro(torch_state* state, double dt, int want_fluxes, torch_step_info* info);
+/* Download the face fluxes of the last torch_state_hydro, reference layout:
+ * x faces (dim+2, nz, ny, nx+1), y faces (dim+2, nz, ny+1, nx), z faces
+ * (dim+2, nz+1, ny, nx); unused pointers may be NULL. */
+int torch_state_fluxes(torch_state* state, double* fx, double* fy, double* fz);
+
+/* One chemistry half-step of the committed state, committing the result:
+ * reaction_step_3d / reaction_step_2d / new_lambda_n_e_calc
+ * (torchSolver3D.py:225-243, torchSolver2D.py:474-509, torchSolver.py:708-789). */
+int torch_state_react(torch_state* state, double dt);
+
+/* Conserved totals sum(U)*volume per field (Simulation.integral,
+ * torchSession.py:285-295), reduced on the device in double. */
+int torch_state_totals(torch_state* state, double* out);
+
+/* Measurement hook, no counterpart in the reference: when enabled, HIP events
+ * bracket the hydro kernel launch of every step on the state's stream. Returns
+ * the accumulated kernel milliseconds and launch count since the last call,
+ * then resets them. */
+int torch_state_profile(torch_state* state, int enable, double* kernel_ms, long* launches);
+
+/* hllc_flux_grid on host face arrays (torchFlux.py:6-77): the flux-worker
+ * batch itself. left/right are (nv, m) in Cartesian component order, gl/gr
+ * have m entries, out is (nv, m). */
+int torch_hllc_faces(torch_ctx* ctx, int precision, int nv, int direction, long m, const double* left,
+                     const double* right, const double* gl, const double* gr, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TORCH_HIP_H */

@@ -1,0 +1,253 @@
+"""ctypes binding of libtorchhip.so (declared in include/torch_hip.h).
+
+The library is loaded from this package directory. When it is missing or
+cannot be loaded, every entry point raises: there is no CPU fallback.
+"""
+
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBRARY = os.path.join(HERE, "libtorchhip.so")
+
+OK, NONPHYSICAL, INVALID, HIP_ERROR, STALLED, NO_STABLE_STEP, SPECIES_RANGE, REACTION_STATE = (
+    0, 2, 3, 4, 5, 6, 7, 8)
+
+LIMITERS = ("VanLeer", "Albada", "MINBEE", "Superbee", "Unos")
+BOUNDARY_CODES = {"open": 0, "transmissive": 0, "outflow": 0, "periodic": 1, "wrap": 1,
+                  "reflective": 2, "wall": 2, "slip": 2, "inflow": 3}
+CONTROLLERS = ("Sound Speed Controlled", "w=0")
+VARIANTS = ("Variante 1 - Original", "Variante_2")
+WAVESPEEDS = ("Pressure Based", "Enthalpy Based")
+PRECISIONS = {"fp64": 0, "float64": 0, "fp32": 1, "float32": 1}
+
+EXPORTS = (
+    "torch_ctx_create", "torch_ctx_destroy", "torch_last_error", "torch_state_create",
+    "torch_state_destroy", "torch_state_upload", "torch_state_download", "torch_state_dt",
+    "torch_state_step", "torch_state_hydro", "torch_state_fluxes", "torch_state_react",
+    "torch_state_totals", "torch_state_profile", "torch_hllc_faces",
+)
+
+
+class Config(ctypes.Structure):
+    _fields_ = [
+        ("dim", ctypes.c_int), ("n", ctypes.c_int * 3), ("lengths", ctypes.c_double * 3),
+        ("gamma", ctypes.c_double), ("limiter", ctypes.c_int), ("bc", ctypes.c_int * 3),
+        ("inlet_primitive", ctypes.c_double * 4), ("reactive", ctypes.c_int),
+        ("kinetic_model", ctypes.c_int), ("q_heat", ctypes.c_double),
+        ("activation_energy", ctypes.c_double), ("pre_exponential", ctypes.c_double),
+        ("controller", ctypes.c_int), ("variant", ctypes.c_int), ("wavespeed", ctypes.c_int),
+        ("cfl", ctypes.c_double), ("precision", ctypes.c_int), ("block_x", ctypes.c_int),
+        ("block_y", ctypes.c_int), ("zchunk", ctypes.c_int), ("spacing", ctypes.c_double * 3),
+        ("gamma_mode", ctypes.c_int),
+    ]
+
+
+class StepInfo(ctypes.Structure):
+    _fields_ = [
+        ("dt", ctypes.c_double), ("fallback", ctypes.c_int), ("retries", ctypes.c_int),
+        ("outward", ctypes.c_double * 5), ("source", ctypes.c_double * 5),
+        ("wave_max", ctypes.c_double * 3),
+    ]
+
+
+_lib = None
+_lock = threading.Lock()
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def library_path():
+    return LIBRARY
+
+
+def load():
+    """Load libtorchhip.so (raises if it has not been built)."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIBRARY):
+                raise RuntimeError(
+                    f"HIP library {LIBRARY} is missing; build it with `python -m torch_amd.build`")
+            lib = ctypes.CDLL(LIBRARY)
+            vp = ctypes.c_void_p
+            sig = {
+                "torch_ctx_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(vp)]),
+                "torch_ctx_destroy": (ctypes.c_int, [vp]),
+                "torch_last_error": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t]),
+                "torch_state_create": (ctypes.c_int, [vp, ctypes.POINTER(Config), ctypes.POINTER(vp)]),
+                "torch_state_destroy": (ctypes.c_int, [vp]),
+                "torch_state_upload": (ctypes.c_int, [vp, _dp, _dp, _dp]),
+                "torch_state_download": (ctypes.c_int, [vp, _dp, _dp, _dp]),
+                "torch_state_dt": (ctypes.c_int, [vp, _dp]),
+                "torch_state_step": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_double,
+                                                    ctypes.POINTER(StepInfo)]),
+                "torch_state_hydro": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_int,
+                                                     ctypes.POINTER(StepInfo)]),
+                "torch_state_fluxes": (ctypes.c_int, [vp, _dp, _dp, _dp]),
+                "torch_state_react": (ctypes.c_int, [vp, ctypes.c_double]),
+                "torch_state_totals": (ctypes.c_int, [vp, _dp]),
+                "torch_state_profile": (ctypes.c_int, [vp, ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_long)]),
+                "torch_hllc_faces": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                    ctypes.c_long, _dp, _dp, _dp, _dp, _dp]),
+            }
+            for name, (res, args) in sig.items():
+                fn = getattr(lib, name)
+                fn.restype, fn.argtypes = res, args
+            _lib = lib
+    return _lib
+
+
+def last_error():
+    buf = ctypes.create_string_buffer(1024)
+    load().torch_last_error(buf, len(buf))
+    return buf.value.decode(errors="replace")
+
+
+def check(code):
+    """Map a status code to the reference's exception types."""
+    if code == OK:
+        return
+    msg = last_error()
+    if code == INVALID:
+        raise ValueError(msg)
+    raise RuntimeError(msg)
+
+
+def ptr(array):
+    """Pointer to a C-contiguous float64 array (None passes NULL)."""
+    if array is None:
+        return None
+    if array.dtype != np.float64 or not array.flags.c_contiguous:
+        raise ValueError("expected a C-contiguous float64 array")
+    return array.ctypes.data_as(_dp)
+
+
+def boundary_code(name, dim, axis):
+    code = BOUNDARY_CODES.get(str(name).lower())
+    if code is None or (code == 3 and not (dim == 2 and axis == 1)):
+        raise ValueError("Unsupported session boundary")
+    return code
+
+
+def precision_code(precision):
+    code = PRECISIONS.get(str(precision).lower())
+    if code is None:
+        raise ValueError(f"Unknown precision: {precision}")
+    return code
+
+
+class Context:
+    """One device context (HIP stream + lock) of libtorchhip."""
+
+    def __init__(self, device=0):
+        lib = load()
+        handle = ctypes.c_void_p()
+        check(lib.torch_ctx_create(int(device), ctypes.byref(handle)))
+        self.handle = handle
+        self.device = int(device)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            load().torch_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceState:
+    """A device-resident state created from a Config."""
+
+    def __init__(self, context, config):
+        self.context = context
+        self.config = config
+        handle = ctypes.c_void_p()
+        check(load().torch_state_create(context.handle, ctypes.byref(config), ctypes.byref(handle)))
+        self.handle = handle
+        self.dim = config.dim
+        counts = [config.n[d] for d in range(config.dim)]
+        self.shape = tuple(counts[::-1])
+        self.nf = config.dim + 2
+
+    def upload(self, state, lam=None, beta=None):
+        state = np.ascontiguousarray(state, dtype=np.float64)
+        if state.shape != (self.nf,) + self.shape:
+            raise ValueError(f"state must have shape {(self.nf,) + self.shape}")
+        lam = None if lam is None else np.ascontiguousarray(lam, dtype=np.float64)
+        beta = None if beta is None else np.ascontiguousarray(beta, dtype=np.float64)
+        check(load().torch_state_upload(self.handle, ptr(state), ptr(lam), ptr(beta)))
+
+    def download(self, want_lam=False, want_beta=False):
+        state = np.empty((self.nf,) + self.shape)
+        lam = np.empty(self.shape) if want_lam else None
+        beta = np.empty(self.shape) if want_beta else None
+        check(load().torch_state_download(self.handle, ptr(state), ptr(lam), ptr(beta)))
+        return state, lam, beta
+
+    def dt_limit(self):
+        out = ctypes.c_double()
+        check(load().torch_state_dt(self.handle, ctypes.byref(out)))
+        return out.value
+
+    def step(self, time, t_end):
+        info = StepInfo()
+        check(load().torch_state_step(self.handle, float(time), float(t_end), ctypes.byref(info)))
+        return info
+
+    def hydro(self, dt, want_fluxes=False):
+        info = StepInfo()
+        check(load().torch_state_hydro(self.handle, float(dt), int(bool(want_fluxes)), ctypes.byref(info)))
+        return info
+
+    def fluxes(self):
+        nz, ny, nx = (1,) * (3 - self.dim) + self.shape
+        shapes = []
+        for d in range(self.dim):
+            full = [nz + (d == 2), ny + (d == 1), nx + (d == 0)]
+            shapes.append((self.nf,) + tuple(full[3 - self.dim:]))
+        arrays = [np.empty(s) for s in shapes] + [None] * (3 - self.dim)
+        check(load().torch_state_fluxes(self.handle, *(ptr(a) for a in arrays)))
+        return arrays[: self.dim]
+
+    def react(self, dt):
+        check(load().torch_state_react(self.handle, float(dt)))
+
+    def totals(self):
+        out = np.zeros(5)
+        check(load().torch_state_totals(self.handle, ptr(out)))
+        return out[: self.nf]
+
+    def profile(self, enable=True):
+        """Enable kernel timing; returns (kernel_ms, launches) since the last call."""
+        ms, n = ctypes.c_double(), ctypes.c_long()
+        check(load().torch_state_profile(self.handle, int(enable), ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+    def close(self):
+        if getattr(self, "handle", None):
+            load().torch_state_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def hllc_faces(context, nv, direction, left, right, gl, gr, precision=0):
+    left = np.ascontiguousarray(left, dtype=np.float64)
+    right = np.ascontiguousarray(right, dtype=np.float64)
+    m = left.shape[1]
+    gl = np.ascontiguousarray(np.broadcast_to(gl, (m,)), dtype=np.float64)
+    gr = np.ascontiguousarray(np.broadcast_to(gr, (m,)), dtype=np.float64)
+    out = np.empty_like(left)
+    check(load().torch_hllc_faces(context.handle, int(precision), int(nv), int(direction), m, ptr(left),
+                                  ptr(right), ptr(gl), ptr(gr), ptr(out)))
+    return out

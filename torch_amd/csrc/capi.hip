@@ -1,0 +1,756 @@
+// C ABI (include/torch_hip.h): device contexts, device-resident session
+// states and the Strang-split step driver of Simulation.step
+// (python/cfdSolver/torchSession.py:374-522), dispatching to the fv64 or
+// fv32 kernel builds.
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/torch_hip.h"
+#include "fv_launch.h"
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const std::string& msg) {
+    g_error = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                       \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail(TORCH_HIP_ERROR, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// Kernel entry points of one precision build.
+struct Ops {
+    size_t elem;
+    int (*step3d_blocks)(const fvl::Grid&, const fvl::Launch&, int);
+    hipError_t (*step3d)(const fvl::Grid&, const fvl::Step&, const fvl::Launch&);
+    hipError_t (*llf3d)(const fvl::Grid&, const fvl::Step&, hipStream_t);
+    int (*llf3d_blocks)(const fvl::Grid&);
+    hipError_t (*step2d)(const fvl::Grid&, const fvl::Step&, const fvl::Launch&);
+    int (*step2d_blocks)(const fvl::Grid&, const fvl::Launch&);
+    hipError_t (*llf2d)(const fvl::Grid&, const fvl::Step&, hipStream_t);
+    int (*llf2d_blocks)(const fvl::Grid&);
+    hipError_t (*step1d)(const fvl::Grid&, const fvl::Step&, void*, hipStream_t);
+    size_t (*step1d_scratch)(const fvl::Grid&);
+    int (*step1d_blocks)(const fvl::Grid&);
+    hipError_t (*cfl_cells)(const fvl::Grid&, const void*, const void*, double, double*, int, hipStream_t);
+    int (*cfl_blocks)(const fvl::Grid&);
+    hipError_t (*react)(const fvl::Grid&, const fvl::React&, hipStream_t);
+    int (*react_blocks)(const fvl::Grid&);
+    hipError_t (*hllc_faces)(int, int, long, const void*, const void*, const void*, const void*, void*,
+                             hipStream_t);
+    hipError_t (*totals)(const fvl::Grid&, const void*, double*, hipStream_t);
+};
+
+#define OPS_OF(NS, T)                                                                                    \
+    Ops{sizeof(T),        NS::step3d_blocks, NS::step3d,     NS::llf3d,        NS::llf3d_blocks,          \
+        NS::step2d,       NS::step2d_blocks, NS::llf2d,      NS::llf2d_blocks,                            \
+        NS::step1d,       NS::step1d_scratch, NS::step1d_blocks,                                         \
+        NS::cfl_cells,    NS::cfl_blocks,    NS::react,      NS::react_blocks, NS::hllc_faces, NS::totals}
+
+const Ops kOps64 = OPS_OF(fv64, double);
+const Ops kOps32 = OPS_OF(fv32, float);
+
+// Fold per-block partials in a fixed order (deterministic sums).
+__global__ void k_finalize(const double* partials, int nblocks, double* out) {
+    __shared__ double buf[256];
+    const int slot = blockIdx.x;
+    const bool is_max = slot < 3;
+    double v = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
+        double p = partials[(long)b * fvl::NSLOT + slot];
+        v = is_max ? fmax(v, p) : v + p;
+    }
+    buf[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            double a = buf[threadIdx.x], b = buf[threadIdx.x + s];
+            buf[threadIdx.x] = is_max ? fmax(a, b) : a + b;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[slot] = buf[0];
+}
+
+__global__ void k_clear_ints(int* p, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
+}
+
+hipError_t clear_flags(int* flags, hipStream_t st) {
+    hipLaunchKernelGGL(k_clear_ints, dim3(1), dim3(64), 0, st, flags, 2);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+namespace fvl {
+hipError_t finalize(const double* partials, int nblocks, double* out, hipStream_t st) {
+    k_finalize<<<NSLOT, 256, 0, st>>>(partials, nblocks, out);
+    return hipGetLastError();
+}
+}  // namespace fvl
+
+struct torch_ctx {
+    int device;
+    hipStream_t stream;
+    std::mutex mu;
+};
+
+// Device buffers of one session. Slots of 4 state buffers rotate so the
+// committed state is never overwritten by a step that may still fail.
+struct torch_state {
+    torch_ctx* ctx;
+    torch_config cfg;
+    const Ops* ops;
+    fvl::Grid grid;
+    long ncell;
+    int nf;
+    void* U[4] = {nullptr, nullptr, nullptr, nullptr};
+    void* lam[4] = {nullptr, nullptr, nullptr, nullptr};
+    void* beta[4] = {nullptr, nullptr, nullptr, nullptr};
+    int cur = 0;
+    void* F[3] = {nullptr, nullptr, nullptr};
+    void* scratch1d = nullptr;
+    double* partials = nullptr;  // 4 regions: hydro, first-order, chemistry, CFL
+    int pblocks = 0;
+    double* scalars = nullptr;   // 4 x NSLOT finalized values
+    int* flags = nullptr;
+    double* host = nullptr;      // pinned readback: 4 x NSLOT doubles + flags
+    bool maxima_valid = false;
+    double maxima[3] = {0, 0, 0};
+    double volume = 0;
+    bool profile = false, prof_pending = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double prof_ms = 0;
+    long prof_n = 0;
+};
+
+namespace {
+
+long face_count(const fvl::Grid& g, int d) {
+    long nx = g.nx + (d == 0), ny = g.ny + (d == 1), nz = g.nz + (d == 2);
+    return nx * ny * nz;
+}
+
+int alloc(void** p, size_t bytes) {
+    hipError_t e = hipMalloc(p, bytes ? bytes : 16);
+    if (e != hipSuccess) return fail(TORCH_HIP_ERROR, std::string("hipMalloc: ") + hipGetErrorString(e));
+    return TORCH_OK;
+}
+
+fvl::Launch launch_of(const torch_state* s) {
+    fvl::Launch l;
+    l.block_x = s->cfg.block_x;
+    l.block_y = s->cfg.block_y;
+    l.stream = s->ctx->stream;
+    return l;
+}
+
+// dt from the CFL maxima, in the reference's order of operations.
+double dt_from_maxima(const torch_state* s, const double* mx) {
+    const torch_config& c = s->cfg;
+    if (c.dim == 3) {
+        double sum = mx[0] / s->grid.h[0];
+        sum = sum + mx[1] / s->grid.h[1];
+        sum = sum + mx[2] / s->grid.h[2];
+        return c.cfl / sum;
+    }
+    if (c.dim == 2) {
+        // max((|u|+a)/dx) == max(|u|+a)/dx because rounding is monotone.
+        double total = mx[0] / s->grid.h[0] + mx[1] / s->grid.h[1];
+        if (total <= 1e-12) return 1e-3;
+        return c.cfl / total;
+    }
+    // 1D: min(cfl*dx/max|S|, cfl*dx/max(|u|+a)), with the 1e-6 default.
+    double a = mx[0] > 0 ? (c.cfl * s->grid.h[0]) / mx[0] : 1e-6;
+    double b = c.cfl * s->grid.h[0] / mx[1];
+    return b < a ? b : a;
+}
+
+double py_min(double a, double b) { return b < a ? b : a; }
+
+int sync_read(torch_state* s) {
+    HIP_TRY(hipMemcpyAsync(s->host, s->scalars, 4 * fvl::NSLOT * sizeof(double), hipMemcpyDeviceToHost,
+                           s->ctx->stream));
+    HIP_TRY(hipMemcpyAsync(s->host + 4 * fvl::NSLOT, s->flags, sizeof(int) * 2, hipMemcpyDeviceToHost,
+                           s->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(s->ctx->stream));
+    if (s->prof_pending) {
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+        s->prof_ms += ms;
+        s->prof_n += 1;
+        s->prof_pending = false;
+    }
+    return TORCH_OK;
+}
+
+int host_flags(const torch_state* s) {
+    int f;
+    memcpy(&f, s->host + 4 * fvl::NSLOT, sizeof(int));
+    return f;
+}
+
+fvl::Step base_step(torch_state* s, int in, int out, double dt, const void* lam) {
+    fvl::Step st;
+    memset(&st, 0, sizeof(st));
+    st.U = s->U[in];
+    st.Uout = s->U[out];
+    st.lam = lam;
+    st.gamma = s->cfg.gamma;
+    st.dt = dt;
+    for (int d = 0; d < 3; ++d) {
+        st.c[d] = dt / s->grid.h[d];
+        st.hc[d] = 0.5 * dt / s->grid.h[d];
+        st.w[d] = dt * s->volume / s->grid.h[d];
+    }
+    st.flags = s->flags;
+    st.zchunk = s->cfg.zchunk;
+    st.limiter = s->cfg.limiter;
+    st.controller = s->cfg.controller;
+    st.variant = s->cfg.variant;
+    st.wavespeed = s->cfg.wavespeed;
+    st.cfl = s->cfg.cfl;
+    return st;
+}
+
+double* region(torch_state* s, int r) { return s->partials + (long)r * s->pblocks * fvl::NSLOT; }
+
+// Wave maxima of the committed state (Simulation._dt), into s->maxima.
+int compute_maxima(torch_state* s) {
+    const void* lam = s->cfg.gamma_mode ? s->lam[s->cur] : nullptr;
+    hipStream_t st = s->ctx->stream;
+    HIP_TRY(clear_flags(s->flags, st));
+    HIP_TRY(s->ops->cfl_cells(s->grid, s->U[s->cur], lam, s->cfg.gamma, region(s, 3), s->cfg.gamma_mode == 2, st));
+    HIP_TRY(fvl::finalize(region(s, 3), s->ops->cfl_blocks(s->grid), s->scalars + 3 * fvl::NSLOT, st));
+    int rc = sync_read(s);
+    if (rc) return rc;
+    for (int m = 0; m < 3; ++m) s->maxima[m] = s->host[3 * fvl::NSLOT + m];
+    s->maxima_valid = true;
+    return TORCH_OK;
+}
+
+// 1D/2D CFL maxima of slot `u` (lambda of slot `l`) into region 3.
+hipError_t run_cfl(torch_state* s, int u, int l) {
+    const void* lam = s->cfg.gamma_mode ? s->lam[l] : nullptr;
+    hipStream_t st = s->ctx->stream;
+    hipError_t e = s->ops->cfl_cells(s->grid, s->U[u], lam, s->cfg.gamma, region(s, 3), s->cfg.gamma_mode == 2, st);
+    if (e != hipSuccess) return e;
+    return fvl::finalize(region(s, 3), s->ops->cfl_blocks(s->grid), s->scalars + 3 * fvl::NSLOT, st);
+}
+
+// Chemistry half-step slot `in` -> slot `out` (state, lambda, beta).
+int run_react(torch_state* s, int in, int out, double dt, int start, int old, bool want_cfl) {
+    fvl::React r;
+    memset(&r, 0, sizeof(r));
+    r.U = s->U[in];
+    r.Uout = s->U[out];
+    r.lam = s->lam[in];
+    r.lamout = s->lam[out];
+    r.beta = s->cfg.kinetic_model == 1 ? s->beta[in] : nullptr;
+    r.betaout = s->cfg.kinetic_model == 1 ? s->beta[out] : nullptr;
+    r.Ustart = start >= 0 ? s->U[start] : nullptr;
+    r.Uold = old >= 0 ? s->U[old] : nullptr;
+    r.model = s->cfg.kinetic_model;
+    r.dt = dt;
+    r.q_heat = s->cfg.q_heat;
+    r.activation_energy = s->cfg.activation_energy;
+    r.pre_exponential = s->cfg.pre_exponential;
+    r.flags = s->flags;
+    r.partials = region(s, 2);
+    r.want_cfl = want_cfl && s->cfg.dim == 3;
+    hipStream_t st = s->ctx->stream;
+    HIP_TRY(s->ops->react(s->grid, r, st));
+    HIP_TRY(fvl::finalize(region(s, 2), s->ops->react_blocks(s->grid), s->scalars + 2 * fvl::NSLOT, st));
+    if (want_cfl && s->cfg.dim != 3) HIP_TRY(run_cfl(s, out, out));
+    return TORCH_OK;
+}
+
+// Hydro slot `in` -> `out` with optional species; leaves finalized hydro
+// values in scalars[0] and first-order ones in scalars[1].
+int run_hydro(torch_state* s, int in, int out, double dt, int lam_in, int lam_out, bool want_fluxes) {
+    const bool reactive = s->cfg.reactive;
+    const void* lam = s->cfg.gamma_mode ? s->lam[lam_in] : nullptr;
+    fvl::Step st = base_step(s, in, out, dt, lam);
+    st.lam_is_gamma = s->cfg.gamma_mode == 2;
+    if (reactive) {
+        st.nq = s->cfg.kinetic_model == 1 ? 2 : 1;
+        st.q[0] = s->lam[lam_in];
+        st.qout[0] = s->lam[lam_out];
+        st.q[1] = s->beta[lam_in];
+        st.qout[1] = s->beta[lam_out];
+    }
+    st.want_cfl = !s->cfg.gamma_mode;
+    if (want_fluxes)
+        for (int d = 0; d < s->cfg.dim; ++d) st.F[d] = s->F[d];
+    hipStream_t stream = s->ctx->stream;
+    HIP_TRY(clear_flags(s->flags, stream));
+    if (s->cfg.dim == 3) {
+        fvl::Launch l = launch_of(s);
+        st.partials = region(s, 0);
+        if (s->profile) HIP_TRY(hipEventRecord(s->ev0, stream));
+        HIP_TRY(s->ops->step3d(s->grid, st, l));
+        if (s->profile) {
+            HIP_TRY(hipEventRecord(s->ev1, stream));
+            s->prof_pending = true;
+        }
+        HIP_TRY(fvl::finalize(region(s, 0), s->ops->step3d_blocks(s->grid, l, st.zchunk), s->scalars, stream));
+        st.partials = region(s, 1);
+        HIP_TRY(s->ops->llf3d(s->grid, st, stream));
+        HIP_TRY(fvl::finalize(region(s, 1), s->ops->llf3d_blocks(s->grid), s->scalars + fvl::NSLOT, stream));
+    } else if (s->cfg.dim == 2) {
+        fvl::Launch l = launch_of(s);
+        st.partials = region(s, 0);
+        HIP_TRY(s->ops->step2d(s->grid, st, l));
+        HIP_TRY(fvl::finalize(region(s, 0), s->ops->step2d_blocks(s->grid, l), s->scalars, stream));
+        st.partials = region(s, 1);
+        HIP_TRY(s->ops->llf2d(s->grid, st, stream));
+        HIP_TRY(fvl::finalize(region(s, 1), s->ops->llf2d_blocks(s->grid), s->scalars + fvl::NSLOT, stream));
+    } else {
+        st.partials = region(s, 0);
+        HIP_TRY(s->ops->step1d(s->grid, st, s->scratch1d, stream));
+        HIP_TRY(fvl::finalize(region(s, 0), s->ops->step1d_blocks(s->grid), s->scalars, stream));
+    }
+    if (s->cfg.dim != 3 && !reactive) HIP_TRY(run_cfl(s, out, lam_in));
+    return TORCH_OK;
+}
+
+const char* hydro_message(int dim) {
+    return dim == 3 ? "3D hydrodynamic update produced a nonphysical state"
+                    : "Hydrodynamic update produced a nonphysical state";
+}
+
+}  // namespace
+
+extern "C" {
+
+int torch_last_error(char* buffer, size_t size) {
+    if (!buffer || size == 0) return TORCH_INVALID;
+    snprintf(buffer, size, "%s", g_error.c_str());
+    return TORCH_OK;
+}
+
+int torch_ctx_create(int device, torch_ctx** out) {
+    if (!out) return fail(TORCH_INVALID, "null output pointer");
+    int count = 0;
+    HIP_TRY(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count) return fail(TORCH_INVALID, "no such HIP device");
+    HIP_TRY(hipSetDevice(device));
+    torch_ctx* c = new torch_ctx();
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(TORCH_HIP_ERROR, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    *out = c;
+    return TORCH_OK;
+}
+
+int torch_ctx_destroy(torch_ctx* ctx) {
+    if (!ctx) return TORCH_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return TORCH_OK;
+}
+
+int torch_state_destroy(torch_state* s) {
+    if (!s) return TORCH_OK;
+    (void)hipSetDevice(s->ctx->device);
+    (void)hipStreamSynchronize(s->ctx->stream);
+    for (int i = 0; i < 4; ++i) {
+        (void)hipFree(s->U[i]);
+        (void)hipFree(s->lam[i]);
+        (void)hipFree(s->beta[i]);
+    }
+    for (int d = 0; d < 3; ++d) (void)hipFree(s->F[d]);
+    (void)hipFree(s->scratch1d);
+    (void)hipFree(s->partials);
+    if (s->ev0) (void)hipEventDestroy(s->ev0);
+    if (s->ev1) (void)hipEventDestroy(s->ev1);
+    (void)hipFree(s->scalars);
+    (void)hipFree(s->flags);
+    (void)hipHostFree(s->host);
+    delete s;
+    return TORCH_OK;
+}
+
+int torch_state_create(torch_ctx* ctx, const torch_config* cfg, torch_state** out) {
+    if (!ctx || !cfg || !out) return fail(TORCH_INVALID, "null argument");
+    const torch_config& c = *cfg;
+    if (c.dim < 1 || c.dim > 3) return fail(TORCH_INVALID, "Invalid dimension");
+    for (int d = 0; d < c.dim; ++d) {
+        if (c.n[d] < 2) return fail(TORCH_INVALID, "Require cells >= 2");
+        if (!(c.spacing[d] > 0) && (!(c.lengths[d] > 0) || !isfinite(c.lengths[d])))
+            return fail(TORCH_INVALID, "Require length > 0");
+        if (c.bc[d] < 0 || c.bc[d] > 3 || (c.bc[d] == TORCH_INFLOW && !(c.dim == 2 && d == 1)))
+            return fail(TORCH_INVALID, "Unsupported session boundary");
+    }
+    if (c.limiter < 0 || c.limiter > 4) return fail(TORCH_INVALID, "Unknown limiter or chemistry model");
+    if (c.kinetic_model < 0 || c.kinetic_model > 1) return fail(TORCH_INVALID, "Unknown limiter or chemistry model");
+    if (!(c.gamma > 1) || !isfinite(c.gamma)) return fail(TORCH_INVALID, "gamma must be finite and greater than one");
+    if (!(c.cfl > 0) || !isfinite(c.cfl)) return fail(TORCH_INVALID, "CFL must be positive");
+    if (c.precision != TORCH_FP64 && c.precision != TORCH_FP32) return fail(TORCH_INVALID, "unknown precision");
+    if (c.gamma_mode < 0 || c.gamma_mode > 2 || (c.reactive && c.gamma_mode != 1))
+        return fail(TORCH_INVALID, "gamma_mode must be 1 for reactive states, 0 or 2 otherwise");
+
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    torch_state* s = new torch_state();
+    s->ctx = ctx;
+    s->cfg = c;
+    s->ops = c.precision == TORCH_FP64 ? &kOps64 : &kOps32;
+    if (s->cfg.block_x == 0 || s->cfg.block_y == 0) {
+        s->cfg.block_x = 64;
+        s->cfg.block_y = 8;
+    }
+    if (s->cfg.zchunk <= 0) s->cfg.zchunk = 32;
+    fvl::Grid& g = s->grid;
+    memset(&g, 0, sizeof(g));
+    g.dim = c.dim;
+    g.wavespeed = c.wavespeed;
+    g.nx = c.n[0];
+    g.ny = c.dim >= 2 ? c.n[1] : 1;
+    g.nz = c.dim >= 3 ? c.n[2] : 1;
+    for (int d = 0; d < 3; ++d) {
+        g.bc[d] = d < c.dim ? c.bc[d] : TORCH_OPEN;
+        g.h[d] = d < c.dim ? (c.spacing[d] > 0 ? c.spacing[d] : c.lengths[d] / (double)c.n[d]) : 1.0;
+    }
+    // Volume as np.prod(spacing): ((hx*hy)*hz) over the active axes.
+    s->volume = g.h[0];
+    if (c.dim >= 2) s->volume = s->volume * g.h[1];
+    if (c.dim >= 3) s->volume = s->volume * g.h[2];
+    if (c.dim == 2) {
+        // Inlet conservative state at gamma 1.4 with the reference's floors
+        // (torchSolver2D.py:61-82, 243).
+        const double* p = c.inlet_primitive;
+        double rho = fmax(p[0], 1e-12), pr = fmax(p[3], 1e-12);
+        double ke = 0.5 * rho * (p[1] * p[1] + p[2] * p[2]);
+        g.inlet[0] = rho;
+        g.inlet[1] = rho * p[1];
+        g.inlet[2] = rho * p[2];
+        g.inlet[3] = pr / (1.4 - 1.0) + ke;
+    }
+    s->ncell = (long)g.nx * g.ny * g.nz;
+    s->nf = c.dim + 2;
+    const size_t e = s->ops->elem;
+    const int nbuf = c.reactive ? 4 : 2;
+    int rc = TORCH_OK;
+    for (int i = 0; i < nbuf && !rc; ++i) rc = alloc(&s->U[i], s->nf * s->ncell * e);
+    if (c.gamma_mode)
+        for (int i = 0; i < 4 && !rc; ++i) {
+            rc = alloc(&s->lam[i], s->ncell * e);
+            if (!rc) rc = alloc(&s->beta[i], s->ncell * e);
+        }
+    fvl::Launch l = launch_of(s);
+    int pb = std::max(s->ops->cfl_blocks(g), s->ops->react_blocks(g));
+    if (c.dim == 3) pb = std::max(pb, std::max(s->ops->step3d_blocks(g, l, s->cfg.zchunk), s->ops->llf3d_blocks(g)));
+    if (c.dim == 2) pb = std::max(pb, std::max(s->ops->step2d_blocks(g, l), s->ops->llf2d_blocks(g)));
+    if (c.dim == 1) {
+        pb = std::max(pb, s->ops->step1d_blocks(g));
+        if (!rc) rc = alloc(&s->scratch1d, s->ops->step1d_scratch(g));
+    }
+    pb = std::max(pb, 256);
+    s->pblocks = pb;
+    if (!rc) rc = alloc((void**)&s->partials, 4 * (size_t)pb * fvl::NSLOT * sizeof(double));
+    if (!rc) rc = alloc((void**)&s->scalars, 4 * fvl::NSLOT * sizeof(double));
+    if (!rc) rc = alloc((void**)&s->flags, 16 * sizeof(int));
+    if (!rc && hipHostMalloc((void**)&s->host, (4 * fvl::NSLOT + 2) * sizeof(double)) != hipSuccess)
+        rc = fail(TORCH_HIP_ERROR, "hipHostMalloc failed");
+    if (rc) {
+        torch_state_destroy(s);
+        return rc;
+    }
+    *out = s;
+    return TORCH_OK;
+}
+
+static int upload_field(torch_state* s, void* dst, const double* src, long n) {
+    if (s->ops->elem == sizeof(double)) {
+        HIP_TRY(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyHostToDevice, s->ctx->stream));
+        HIP_TRY(hipStreamSynchronize(s->ctx->stream));
+    } else {
+        std::vector<float> tmp(n);
+        for (long i = 0; i < n; ++i) tmp[i] = (float)src[i];
+        HIP_TRY(hipMemcpyAsync(dst, tmp.data(), n * sizeof(float), hipMemcpyHostToDevice, s->ctx->stream));
+        HIP_TRY(hipStreamSynchronize(s->ctx->stream));
+    }
+    return TORCH_OK;
+}
+
+static int download_field(torch_state* s, double* dst, const void* src, long n) {
+    if (s->ops->elem == sizeof(double)) {
+        HIP_TRY(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToHost, s->ctx->stream));
+        HIP_TRY(hipStreamSynchronize(s->ctx->stream));
+    } else {
+        std::vector<float> tmp(n);
+        HIP_TRY(hipMemcpyAsync(tmp.data(), src, n * sizeof(float), hipMemcpyDeviceToHost, s->ctx->stream));
+        HIP_TRY(hipStreamSynchronize(s->ctx->stream));
+        for (long i = 0; i < n; ++i) dst[i] = tmp[i];
+    }
+    return TORCH_OK;
+}
+
+int torch_state_upload(torch_state* s, const double* U, const double* lam, const double* beta) {
+    if (!s || !U) return fail(TORCH_INVALID, "null argument");
+    if (s->cfg.gamma_mode && !lam) return fail(TORCH_INVALID, "state needs a lambda (or gamma) array");
+    std::lock_guard<std::mutex> lock(s->ctx->mu);
+    HIP_TRY(hipSetDevice(s->ctx->device));
+    int rc = upload_field(s, s->U[s->cur], U, s->nf * s->ncell);
+    if (!rc && s->cfg.gamma_mode) rc = upload_field(s, s->lam[s->cur], lam, s->ncell);
+    if (!rc && s->cfg.reactive) {
+        if (beta) {
+            rc = upload_field(s, s->beta[s->cur], beta, s->ncell);
+        } else {
+            std::vector<double> ones(s->ncell, 1.0);
+            rc = upload_field(s, s->beta[s->cur], ones.data(), s->ncell);
+        }
+    }
+    s->maxima_valid = false;
+    return rc;
+}
+
+int torch_state_download(torch_state* s, double* U, double* lam, double* beta) {
+    if (!s) return fail(TORCH_INVALID, "null argument");
+    std::lock_guard<std::mutex> lock(s->ctx->mu);
+    HIP_TRY(hipSetDevice(s->ctx->device));
+    int rc = TORCH_OK;
+    if (U) rc = download_field(s, U, s->U[s->cur], s->nf * s->ncell);
+    if (!rc && lam && s->cfg.reactive) rc = download_field(s, lam, s->lam[s->cur], s->ncell);
+    if (!rc && beta && s->cfg.reactive) rc = download_field(s, beta, s->beta[s->cur], s->ncell);
+    return rc;
+}
+
+int torch_state_dt(torch_state* s, double* dt_limit) {
+    if (!s || !dt_limit) return fail(TORCH_INVALID, "null argument");
+    std::lock_guard<std::mutex> lock(s->ctx->mu);
+    HIP_TRY(hipSetDevice(s->ctx->device));
+    if (!s->maxima_valid) {
+        int rc = compute_maxima(s);
+        if (rc) return rc;
+    }
+    *dt_limit = dt_from_maxima(s, s->maxima);
+    return TORCH_OK;
+}
+
+int torch_state_step(torch_state* s, double time, double t_end, torch_step_info* info) {
+    if (!s || !info) return fail(TORCH_INVALID, "null argument");
+    std::lock_guard<std::mutex> lock(s->ctx->mu);
+    HIP_TRY(hipSetDevice(s->ctx->device));
+    memset(info, 0, sizeof(*info));
+    const torch_config& c = s->cfg;
+    int rc;
+    if (!s->maxima_valid && (rc = compute_maxima(s))) return rc;
+    double dt = py_min(dt_from_maxima(s, s->maxima), t_end - time);
+    if (!isfinite(dt) || dt <= 0 || time + dt == time)
+        return fail(TORCH_STALLED, "Integration stalled before requested final time");
+    const int A = s->cur;
+    const int B = (A + 1) % 4, C = (A + 2) % 4, D = (A + 3) % 4;
+    int hin = A, hout = c.reactive ? C : (A + 1) % 2;
+    if (c.reactive) {
+        int it = 0;
+        for (; it < 60; ++it) {
+            HIP_TRY(clear_flags(s->flags, s->ctx->stream));
+            if ((rc = run_react(s, A, B, dt / 2, -1, -1, true))) return rc;
+            if ((rc = sync_read(s))) return rc;
+            if (host_flags(s) & fvl::F_REACT_BAD)
+                return fail(TORCH_INVALID, "Reaction input outside its domain (state, lambda or beta)");
+            double mx[3];
+            const double* src = s->host + (c.dim == 3 ? 2 : 3) * fvl::NSLOT;
+            for (int m = 0; m < 3; ++m) mx[m] = src[m];
+            double allowed = dt_from_maxima(s, mx);
+            if (dt <= allowed * (1 + 1e-12)) break;
+            dt = py_min(dt / 2, allowed);
+        }
+        if (it == 60) return fail(TORCH_NO_STABLE_STEP, "Could not find stable reactive step");
+        info->retries = it;
+        hin = B;
+    }
+    if ((rc = run_hydro(s, hin, hout, dt, B, C, false))) return rc;
+    if (c.reactive) {
+        if ((rc = run_react(s, C, D, dt / 2, A, B, true))) return rc;
+    }
+    if ((rc = sync_read(s))) return rc;
+    const int flags = host_flags(s);
+    const bool fb = (flags & (fvl::F_PRED_BAD | fvl::F_UPD_BAD)) != 0;
+    if (flags & fvl::F_LLF_BAD) return fail(TORCH_NONPHYSICAL, hydro_message(c.dim));
+    if (c.dim == 3 && (flags & (fb ? fvl::F_SPEC_BAD_LLF : fvl::F_SPEC_BAD)))
+        return fail(TORCH_SPECIES_RANGE, "3D scalar transport left the physical interval");
+    if (flags & fvl::F_REACT_BAD) return fail(TORCH_INVALID, "Reaction input outside its domain");
+    if (flags & fvl::F_STATE_BAD) return fail(TORCH_REACTION_STATE, "Reaction produced a nonphysical state");
+    const double* hv = s->host + (fb && c.dim >= 2 ? fvl::NSLOT : 0);
+    info->dt = dt;
+    info->fallback = fb;
+    for (int f = 0; f < s->nf; ++f) info->outward[f] = hv[fvl::S_OUT + f];
+    if (c.reactive) {
+        info->source[s->nf - 1] = s->volume * s->host[2 * fvl::NSLOT + fvl::S_SRC];
+        const double* mxs = s->host + (c.dim == 3 ? 2 : 3) * fvl::NSLOT;
+        for (int m = 0; m < 3; ++m) s->maxima[m] = mxs[m];
+    } else {
+        const double* mxs = c.dim == 3 ? hv : s->host + 3 * fvl::NSLOT;
+        for (int m = 0; m < 3; ++m) s->maxima[m] = mxs[m];
+    }
+    s->maxima_valid = true;
+    for (int m = 0; m < 3; ++m) info->wave_max[m] = s->maxima[m];
+    s->cur = c.reactive ? D : hout;
+    return TORCH_OK;
+}
+
+int torch_state_hydro(torch_state* s, double dt, int want_fluxes, torch_step_info* info) {
+    if (!s || !info) return fail(TORCH_INVALID, "null argument");
+    std::lock_guard<std::mutex> lock(s->ctx->mu);
+    HIP_TRY(hipSetDevice(s->ctx->device));
+    memset(info, 0, sizeof(*info));
+    int rc;
+    if (want_fluxes)
+        for (int d = 0; d < s->cfg.dim; ++d)
+            if (!s->F[d] && (rc = alloc(&s->F[d], s->nf * face_count(s->grid, d) * s->ops->elem))) return rc;
+    const int A = s->cur;
+    const int out = s->cfg.gamma_mode ? (A + 2) % 4 : (A + 1) % 2;
+    if ((rc = run_hydro(s, A, out, dt, A, out, want_fluxes != 0))) return rc;
+    if ((rc = sync_read(s))) return rc;
+    const int flags = host_flags(s);
+    const bool fb = (flags & (fvl::F_PRED_BAD | fvl::F_UPD_BAD)) != 0;
+    if (flags & fvl::F_LLF_BAD) return fail(TORCH_NONPHYSICAL, hydro_message(s->cfg.dim));
+    const double* hv = s->host + (fb && s->cfg.dim >= 2 ? fvl::NSLOT : 0);
+    info->dt = dt;
+    info->fallback = fb;
+    for (int f = 0; f < s->nf; ++f) info->outward[f] = hv[fvl::S_OUT + f];
+    s->cur = out;
+    s->maxima_valid = false;
+    return TORCH_OK;
+}
+
+int torch_state_fluxes(torch_state* s, double* fx, double* fy, double* fz) {
+    if (!s) return fail(TORCH_INVALID, "null argument");
+    std::lock_guard<std::mutex> lock(s->ctx->mu);
+    HIP_TRY(hipSetDevice(s->ctx->device));
+    double* dst[3] = {fx, fy, fz};
+    for (int d = 0; d < s->cfg.dim; ++d) {
+        if (!dst[d]) continue;
+        if (!s->F[d]) return fail(TORCH_INVALID, "no fluxes recorded");
+        int rc = download_field(s, dst[d], s->F[d], s->nf * face_count(s->grid, d));
+        if (rc) return rc;
+    }
+    return TORCH_OK;
+}
+
+int torch_state_react(torch_state* s, double dt) {
+    if (!s) return fail(TORCH_INVALID, "null argument");
+    if (!s->cfg.reactive) return fail(TORCH_INVALID, "state is not reactive");
+    if (!isfinite(dt) || dt < 0 || !isfinite(s->cfg.q_heat) || s->cfg.q_heat < 0 ||
+        !isfinite(s->cfg.activation_energy) || s->cfg.activation_energy < 0 ||
+        !isfinite(s->cfg.pre_exponential) || s->cfg.pre_exponential < 0)
+        return fail(TORCH_INVALID, "Reaction time and constants must be finite and nonnegative");
+    std::lock_guard<std::mutex> lock(s->ctx->mu);
+    HIP_TRY(hipSetDevice(s->ctx->device));
+    const int A = s->cur, B = (A + 1) % 4;
+    HIP_TRY(clear_flags(s->flags, s->ctx->stream));
+    int rc = run_react(s, A, B, dt, -1, -1, false);
+    if (!rc) rc = sync_read(s);
+    if (rc) return rc;
+    if (host_flags(s) & fvl::F_REACT_BAD)
+        return fail(TORCH_INVALID, "Reaction requires a physical state and fractions in [0,1]");
+    s->cur = B;
+    s->maxima_valid = false;
+    return TORCH_OK;
+}
+
+int torch_state_totals(torch_state* s, double* out) {
+    if (!s || !out) return fail(TORCH_INVALID, "null argument");
+    std::lock_guard<std::mutex> lock(s->ctx->mu);
+    HIP_TRY(hipSetDevice(s->ctx->device));
+    HIP_TRY(s->ops->totals(s->grid, s->U[s->cur], region(s, 2), s->ctx->stream));
+    HIP_TRY(fvl::finalize(region(s, 2), 256, s->scalars + 2 * fvl::NSLOT, s->ctx->stream));
+    int rc = sync_read(s);
+    if (rc) return rc;
+    for (int f = 0; f < s->nf; ++f) out[f] = s->host[2 * fvl::NSLOT + fvl::S_OUT + f] * s->volume;
+    return TORCH_OK;
+}
+
+int torch_state_profile(torch_state* s, int enable, double* kernel_ms, long* launches) {
+    if (!s) return fail(TORCH_INVALID, "null argument");
+    std::lock_guard<std::mutex> lock(s->ctx->mu);
+    HIP_TRY(hipSetDevice(s->ctx->device));
+    if (enable && !s->ev0) {
+        HIP_TRY(hipEventCreate(&s->ev0));
+        HIP_TRY(hipEventCreate(&s->ev1));
+    }
+    if (kernel_ms) *kernel_ms = s->prof_ms;
+    if (launches) *launches = s->prof_n;
+    s->prof_ms = 0;
+    s->prof_n = 0;
+    s->profile = enable != 0;
+    return TORCH_OK;
+}
+
+int torch_hllc_faces(torch_ctx* ctx, int precision, int nv, int direction, long m, const double* left,
+                     const double* right, const double* gl, const double* gr, double* out) {
+    if (!ctx || !left || !right || !gl || !gr || !out) return fail(TORCH_INVALID, "null argument");
+    if ((nv != 4 && nv != 5) || direction < 0 || direction >= nv - 2 || m < 0)
+        return fail(TORCH_INVALID, "HLLC requires matching four/five-component grids");
+    if (m == 0) return TORCH_OK;
+    const Ops* ops = precision == TORCH_FP32 ? &kOps32 : &kOps64;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    const size_t e = ops->elem;
+    void *dl = nullptr, *dr = nullptr, *dgl = nullptr, *dgr = nullptr, *dout = nullptr;
+    int rc = alloc(&dl, nv * m * e);
+    if (!rc) rc = alloc(&dr, nv * m * e);
+    if (!rc) rc = alloc(&dgl, m * e);
+    if (!rc) rc = alloc(&dgr, m * e);
+    if (!rc) rc = alloc(&dout, nv * m * e);
+    auto put = [&](void* dst, const double* src, long n) -> int {
+        if (e == sizeof(double)) {
+            HIP_TRY(hipMemcpyAsync(dst, src, n * e, hipMemcpyHostToDevice, ctx->stream));
+        } else {
+            std::vector<float> t(src, src + n);
+            HIP_TRY(hipMemcpyAsync(dst, t.data(), n * e, hipMemcpyHostToDevice, ctx->stream));
+            HIP_TRY(hipStreamSynchronize(ctx->stream));
+        }
+        return TORCH_OK;
+    };
+    if (!rc) rc = put(dl, left, nv * m);
+    if (!rc) rc = put(dr, right, nv * m);
+    if (!rc) rc = put(dgl, gl, m);
+    if (!rc) rc = put(dgr, gr, m);
+    if (!rc) {
+        hipError_t he = ops->hllc_faces(nv, direction, m, dl, dr, dgl, dgr, dout, ctx->stream);
+        if (he != hipSuccess) rc = fail(TORCH_HIP_ERROR, hipGetErrorString(he));
+    }
+    if (!rc) {
+        if (e == sizeof(double)) {
+            hipError_t he = hipMemcpyAsync(out, dout, nv * m * e, hipMemcpyDeviceToHost, ctx->stream);
+            if (he == hipSuccess) he = hipStreamSynchronize(ctx->stream);
+            if (he != hipSuccess) rc = fail(TORCH_HIP_ERROR, hipGetErrorString(he));
+        } else {
+            std::vector<float> t(nv * m);
+            hipError_t he = hipMemcpyAsync(t.data(), dout, nv * m * e, hipMemcpyDeviceToHost, ctx->stream);
+            if (he == hipSuccess) he = hipStreamSynchronize(ctx->stream);
+            if (he != hipSuccess) rc = fail(TORCH_HIP_ERROR, hipGetErrorString(he));
+            for (long i = 0; i < nv * m && !rc; ++i) out[i] = t[i];
+        }
+    }
+    (void)hipFree(dl);
+    (void)hipFree(dr);
+    (void)hipFree(dgl);
+    (void)hipFree(dgr);
+    (void)hipFree(dout);
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,683 @@
+// 3D Cartesian MUSCL-Hancock + HLLC step, its first-order fallback, the
+// chemistry half-step, CFL maxima and conserved totals.
+//
+// One fused kernel does the reference's whole `_hydro` second-order pass
+// (python/cfdSolver/torchSolver3D.py:122-203): ghost cells by index maps,
+// limited slopes, the half-step predictor with its positivity test, HLLC on
+// every face, the conservative update, the species transport of `_advect`
+// (:246-256), the outward boundary-flux sums of `Simulation.step`
+// (torchSession.py:498-508) and the CFL maxima of `_dt` (:356-360).
+//
+// Layout: state is structure-of-arrays, field-major, x fastest, exactly the
+// reference's (5, nz, ny, nx) C-order array. A block owns a tile of
+// (BX-2) x (BY-2) columns plus a one-cell halo ring of threads and marches
+// up a chunk of z planes: x neighbours travel by lane shuffles (a block row
+// is one 64-lane wave, or half of one for BX = 32), y neighbours through
+// LDS, z neighbours stay in registers. Each HLLC face is evaluated once per
+// tile; the update of plane k-1 completes when plane k's predictor has
+// produced the z face between them.
+#include "fv_device.h"
+#include "fv_launch.h"
+
+#include <string.h>
+
+namespace FV_NS {
+using namespace fv;
+using fvl::Grid;
+using fvl::Step;
+
+struct G3 {
+    int nx, ny, nz, bcx, bcy, bcz;
+};
+
+__device__ __forceinline__ long cell(int x, int y, int z, int nx, int ny) {
+    return ((long)z * ny + y) * nx + x;
+}
+
+// 5 components of ghost-mapped cell (x, y, z); reflective ghosts negate the
+// momentum normal to that wall.
+__device__ __forceinline__ void load5(const real* __restrict__ U, long N, const G3& g, int x, int y, int z,
+                                      real* a) {
+    long i = cell(ghost_src(x, g.nx, g.bcx), ghost_src(y, g.ny, g.bcy), ghost_src(z, g.nz, g.bcz), g.nx, g.ny);
+#pragma unroll
+    for (int c = 0; c < 5; ++c) a[c] = U[c * N + i];
+    if (ghost_flip(x, g.nx, g.bcx)) a[1] = -a[1];
+    if (ghost_flip(y, g.ny, g.bcy)) a[2] = -a[2];
+    if (ghost_flip(z, g.nz, g.bcz)) a[3] = -a[3];
+}
+
+__device__ __forceinline__ real load1(const real* __restrict__ q, const G3& g, int x, int y, int z) {
+    return q[cell(ghost_src(x, g.nx, g.bcx), ghost_src(y, g.ny, g.bcy), ghost_src(z, g.nz, g.bcz), g.nx, g.ny)];
+}
+
+// Fold one block's partial results: maxima and sums over the block's
+// threads, written to partials[block * NSLOT + slot] by thread 0.
+template <int NT>
+__device__ void block_partials(double (&acc)[5], double (&mx)[3], double src, double* partials, int* flags,
+                               int bad) {
+    __shared__ double red[NT / 64 > 0 ? NT / 64 : 1][9];
+    __shared__ int sbad;
+    const int tid = threadIdx.x + blockDim.x * threadIdx.y;
+    if (tid == 0) sbad = 0;
+    __syncthreads();
+    if (bad) atomicOr(&sbad, bad);
+    double v[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) v[i] = wave_max(mx[i]);
+#pragma unroll
+    for (int i = 0; i < 5; ++i) v[3 + i] = wave_sum(acc[i]);
+    v[8] = wave_sum(src);
+    const int w = tid / 64, nw = (NT + 63) / 64;
+    if ((tid & 63) == 0) {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) red[w][i] = v[i];
+    }
+    __syncthreads();
+    if (tid == 0) {
+        const long b = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
+        double* p = partials + b * fvl::NSLOT;
+        for (int i = 0; i < 9; ++i) {
+            double r = red[0][i];
+            for (int k = 1; k < nw; ++k) r = i < 3 ? fmax(r, red[k][i]) : r + red[k][i];
+            p[i < 3 ? i : (i < 8 ? fvl::S_OUT + (i - 3) : fvl::S_SRC)] = r;
+        }
+        if (sbad) atomicOr(flags, sbad);
+    }
+}
+
+// |u_m| + a of a committed 3D state (torchSession.py:356-360); double.
+__device__ __forceinline__ void wave3(const real* o, double g, double (&mx)[3]) {
+    double u[5];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) u[c] = (double)o[c];
+    double p = (g - 1.0) * (u[4] - (0.5 * (u[1] * u[1] + u[2] * u[2] + u[3] * u[3])) / u[0]);
+    double a = sqrt((g * p) / u[0]);
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+        double v = fabs(u[m + 1] / u[0]) + a;
+        mx[m] = (v != v || v > mx[m]) ? v : mx[m];
+    }
+}
+
+template <int LIM, int BX, int BY, bool SPEC>
+__global__ void __launch_bounds__(BX* BY) k_step3d(Step s, G3 g) {
+    constexpr int TX = BX - 2, TY = BY - 2;
+    constexpr int NQ = SPEC ? 2 : 0;
+    constexpr int PW = BX + 2, PH = BY + 2;  // LDS plane: x in [-2, TX+1], y in [-2, TY+1]
+    constexpr int NYS = 6 + NQ;              // y-face payload: state, gamma, fractions
+    constexpr int NFY = 5 + NQ;
+    constexpr int PLANE = 5 * PW * PH;
+    constexpr int FLUX = NFY * BX * BY;
+    __shared__ real sbuf[PLANE > FLUX ? PLANE : FLUX];  // A plane, later the y fluxes
+    __shared__ real sY[NYS][BY][BX];
+    real(*sA)[PH][PW] = reinterpret_cast<real(*)[PH][PW]>(sbuf);
+    real(*sF)[BY][BX] = reinterpret_cast<real(*)[BY][BX]>(sbuf);
+
+    const int lane = threadIdx.x, row = threadIdx.y;
+    const int lx = lane - 1, ly = row - 1;
+    const int bx0 = blockIdx.x * TX, by0 = blockIdx.y * TY;
+    const int x = bx0 + lx, y = by0 + ly;
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const long N = (long)nx * ny * nz;
+    const int z0 = blockIdx.z * s.zchunk, z1 = min(z0 + s.zchunk, nz);
+    const real* __restrict__ U = (const real*)s.U;
+    real* __restrict__ Uo = (real*)s.Uout;
+    const real* __restrict__ lamp = (const real*)s.lam;
+    const int nq = SPEC ? s.nq : 0;
+
+    const bool in_active = x >= -1 && x <= nx && y >= -1 && y <= ny;
+    const bool owns = lx >= 0 && lx < TX && ly >= 0 && ly < TY && x < nx && y < ny;
+    const bool xface = lx >= 0 && ly >= 0 && ly < TY && x <= nx && y < ny;
+    const bool yface = lx >= 0 && lx < TX && ly >= 0 && x < nx && y <= ny;
+    const real c0 = (real)s.c[0], c1 = (real)s.c[1], c2 = (real)s.c[2];
+    const real hc[3] = {(real)s.hc[0], (real)s.hc[1], (real)s.hc[2]};
+    const real gconst = (real)s.gamma;
+
+    // Ring of the LDS plane not covered by the block's own columns.
+    const int tid = lane + BX * row;
+    const int nring = 2 * PH + 2 * BX;
+    int rx = 0, ry = 0;
+    if (tid < 2 * PH) {
+        rx = (tid & 1) ? TX + 1 : -2;
+        ry = tid / 2 - 2;
+    } else if (tid < nring) {
+        int t = tid - 2 * PH;
+        rx = t / 2 - 1;
+        ry = (t & 1) ? TY + 1 : -2;
+    }
+
+    real Ap[5], Ac[5], An[5];
+    real gp = gconst, gc = gconst, gn = gconst;
+    real qp[2] = {0, 0}, qc[2] = {0, 0}, qn[2] = {0, 0};
+    real zr[5] = {0, 0, 0, 0, 0}, Fzp[5] = {0, 0, 0, 0, 0}, sfzp[2] = {0, 0};
+    real R[5] = {0, 0, 0, 0, 0}, rq[2] = {0, 0};
+    double acc[5] = {0, 0, 0, 0, 0}, mx[3] = {0, 0, 0};
+    int bad = 0;
+
+    load5(U, N, g, x, y, z0 - 2, Ap);
+    load5(U, N, g, x, y, z0 - 1, Ac);
+    const bool direct = s.lam_is_gamma != 0;
+    if (lamp) {
+        gp = gamma_from(load1(lamp, g, x, y, z0 - 2), direct);
+        gc = gamma_from(load1(lamp, g, x, y, z0 - 1), direct);
+    }
+    for (int j = 0; j < nq; ++j) {
+        qp[j] = load1((const real*)s.q[j], g, x, y, z0 - 2);
+        qc[j] = load1((const real*)s.q[j], g, x, y, z0 - 1);
+    }
+
+    for (int k = z0 - 1; k <= z1; ++k) {
+        load5(U, N, g, x, y, k + 1, An);
+        if (lamp) gn = gamma_from(load1(lamp, g, x, y, k + 1), direct);
+        for (int j = 0; j < nq; ++j) qn[j] = load1((const real*)s.q[j], g, x, y, k + 1);
+
+        __syncthreads();  // previous iteration's readers of sF are done
+#pragma unroll
+        for (int c = 0; c < 5; ++c) sA[c][ly + 2][lx + 2] = Ac[c];
+        if (tid < nring) {
+            real r[5];
+            load5(U, N, g, bx0 + rx, by0 + ry, k, r);
+#pragma unroll
+            for (int c = 0; c < 5; ++c) sA[c][ry + 2][rx + 2] = r[c];
+        }
+        __syncthreads();
+
+        real sx[5], sy[5], sz[5];
+#pragma unroll
+        for (int c = 0; c < 5; ++c) {
+            sx[c] = limited<LIM, 3>(Ac[c] - sA[c][ly + 2][lx + 1], sA[c][ly + 2][lx + 3] - Ac[c]);
+            sy[c] = limited<LIM, 3>(Ac[c] - sA[c][ly + 1][lx + 2], sA[c][ly + 3][lx + 2] - Ac[c]);
+            sz[c] = limited<LIM, 3>(Ac[c] - Ap[c], An[c] - Ac[c]);
+        }
+        // Half-step predictor (torchSolver3D.py:140-151).
+        real P[5];
+#pragma unroll
+        for (int c = 0; c < 5; ++c) P[c] = Ac[c];
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+            const real* sl = m == 0 ? sx : (m == 1 ? sy : sz);
+            real up[5], um[5], fp[5], fm[5];
+#pragma unroll
+            for (int c = 0; c < 5; ++c) {
+                up[c] = Ac[c] + (real)0.5 * sl[c];
+                um[c] = Ac[c] - (real)0.5 * sl[c];
+            }
+            flux3(up, gc, m + 1, fp);
+            flux3(um, gc, m + 1, fm);
+#pragma unroll
+            for (int c = 0; c < 5; ++c) P[c] = P[c] - hc[m] * (fp[c] - fm[c]);
+        }
+        // Face states: P -+ s/2 (torchSolver3D.py:178-180).
+        real xl[5], xr[5], yl[5], yr[5], zl[5], zn[5];
+#pragma unroll
+        for (int c = 0; c < 5; ++c) {
+            real hx = (real)0.5 * sx[c], hy = (real)0.5 * sy[c], hz = (real)0.5 * sz[c];
+            xl[c] = P[c] - hx; xr[c] = P[c] + hx;
+            yl[c] = P[c] - hy; yr[c] = P[c] + hy;
+            zl[c] = P[c] - hz; zn[c] = P[c] + hz;
+        }
+        // Positivity of every active predicted face state (:152-159).
+        if (in_active && k >= -1 && k <= nz) {
+            if (!physical5(xl) || !physical5(xr) || !physical5(yl) || !physical5(yr) || !physical5(zl) ||
+                !physical5(zn))
+                bad |= fvl::F_PRED_BAD;
+        }
+#pragma unroll
+        for (int c = 0; c < 5; ++c) sY[c][row][lane] = yr[c];
+        sY[5][row][lane] = gc;
+        for (int j = 0; j < NQ; ++j) sY[6 + j][row][lane] = qc[j];
+        __syncthreads();  // sA readers done; sY visible
+
+        const bool plane_out = k >= z0 && k < z1;
+        // x face x-1/2 from the left neighbour's right state.
+        real lxr[5], Fx[5] = {0, 0, 0, 0, 0}, sfx[2] = {0, 0};
+#pragma unroll
+        for (int c = 0; c < 5; ++c) lxr[c] = shfl_up1(xr[c]);
+        real lg = shfl_up1(gc);
+        real lq[2] = {0, 0};
+        for (int j = 0; j < NQ; ++j) lq[j] = shfl_up1(qc[j]);
+        if (plane_out && xface) {
+            hllc_cart5(lxr, xl, lg, gc, 0, Fx);
+            for (int j = 0; j < nq; ++j) sfx[j] = Fx[0] * (Fx[0] >= (real)0 ? lq[j] : qc[j]);
+        }
+        // y face y-1/2 from the row below.
+        real Fy[5] = {0, 0, 0, 0, 0}, sfy[2] = {0, 0};
+        if (plane_out && yface) {
+            real below[5];
+#pragma unroll
+            for (int c = 0; c < 5; ++c) below[c] = sY[c][row - 1][lane];
+            hllc_cart5(below, yl, sY[5][row - 1][lane], gc, 1, Fy);
+            for (int j = 0; j < nq; ++j) sfy[j] = Fy[0] * (Fy[0] >= (real)0 ? sY[6 + j][row - 1][lane] : qc[j]);
+        }
+        // z face k-1/2 from the carried state of plane k-1.
+        real Fz[5] = {0, 0, 0, 0, 0}, sfz[2] = {0, 0};
+        const bool zface = owns && k >= z0;
+        if (zface) {
+            hllc_cart5(zr, zl, gp, gc, 2, Fz);
+            for (int j = 0; j < nq; ++j) sfz[j] = Fz[0] * (Fz[0] >= (real)0 ? qp[j] : qc[j]);
+        }
+#pragma unroll
+        for (int c = 0; c < 5; ++c) sF[c][row][lane] = Fy[c];
+        for (int j = 0; j < NQ; ++j) sF[5 + j][row][lane] = sfy[j];
+        __syncthreads();
+
+        real FxR[5], sfxR[2] = {0, 0};
+#pragma unroll
+        for (int c = 0; c < 5; ++c) FxR[c] = shfl_down1(Fx[c]);
+        for (int j = 0; j < NQ; ++j) sfxR[j] = shfl_down1(sfx[j]);
+
+        // Complete plane k-1 with the z faces (torchSolver3D.py:192-202).
+        if (owns && k - 1 >= z0) {
+            real o[5];
+#pragma unroll
+            for (int c = 0; c < 5; ++c) o[c] = R[c] - c2 * (Fz[c] - Fzp[c]);
+            const long i = cell(x, y, k - 1, nx, ny);
+#pragma unroll
+            for (int c = 0; c < 5; ++c) Uo[c * N + i] = o[c];
+            if (!physical5(o)) bad |= fvl::F_UPD_BAD;
+            if (s.want_cfl) wave3(o, (double)gconst, mx);
+            for (int j = 0; j < nq; ++j) {
+                real v = rq[j] - c2 * (sfz[j] - sfzp[j]);
+                real f = v / o[0];
+                if (f < (real)-1e-10 || f > (real)(1 + 1e-10) || f != f) bad |= fvl::F_SPEC_BAD;
+                ((real*)s.qout[j])[i] = rmin(rmax(f, (real)0), (real)1);
+            }
+        }
+        // Partial update of plane k with the x and y faces.
+        if (owns && plane_out) {
+            real FyT[5];
+#pragma unroll
+            for (int c = 0; c < 5; ++c) FyT[c] = sF[c][row + 1][lane];
+#pragma unroll
+            for (int c = 0; c < 5; ++c) {
+                R[c] = Ac[c] - c0 * (FxR[c] - Fx[c]);
+                R[c] = R[c] - c1 * (FyT[c] - Fy[c]);
+            }
+            for (int j = 0; j < nq; ++j) {
+                rq[j] = Ac[0] * qc[j] - c0 * (sfxR[j] - sfx[j]);
+                rq[j] = rq[j] - c1 * (sF[5 + j][row + 1][lane] - sfy[j]);
+            }
+        }
+        // Outward boundary fluxes and optional face dumps.
+        if (plane_out && xface) {
+            if (x == 0 || x == nx) {
+                double sgn = x == 0 ? -s.w[0] : s.w[0];
+#pragma unroll
+                for (int c = 0; c < 5; ++c) acc[c] += sgn * (double)Fx[c];
+            }
+            if (s.F[0]) {
+                long f = ((long)k * ny + y) * (nx + 1) + x, M = (long)nz * ny * (nx + 1);
+#pragma unroll
+                for (int c = 0; c < 5; ++c) ((real*)s.F[0])[c * M + f] = Fx[c];
+            }
+        }
+        if (plane_out && yface) {
+            if (y == 0 || y == ny) {
+                double sgn = y == 0 ? -s.w[1] : s.w[1];
+#pragma unroll
+                for (int c = 0; c < 5; ++c) acc[c] += sgn * (double)Fy[c];
+            }
+            if (s.F[1]) {
+                long f = ((long)k * (ny + 1) + y) * nx + x, M = (long)nz * (ny + 1) * nx;
+#pragma unroll
+                for (int c = 0; c < 5; ++c) ((real*)s.F[1])[c * M + f] = Fy[c];
+            }
+        }
+        if (zface) {
+            if (k == 0 || k == nz) {
+                double sgn = k == 0 ? -s.w[2] : s.w[2];
+#pragma unroll
+                for (int c = 0; c < 5; ++c) acc[c] += sgn * (double)Fz[c];
+            }
+            if (s.F[2]) {
+                long f = ((long)k * ny + y) * nx + x, M = (long)(nz + 1) * ny * nx;
+#pragma unroll
+                for (int c = 0; c < 5; ++c) ((real*)s.F[2])[c * M + f] = Fz[c];
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 5; ++c) {
+            Ap[c] = Ac[c];
+            Ac[c] = An[c];
+            zr[c] = zn[c];
+            Fzp[c] = Fz[c];
+        }
+        gp = gc;
+        gc = gn;
+        for (int j = 0; j < NQ; ++j) {
+            qp[j] = qc[j];
+            qc[j] = qn[j];
+            sfzp[j] = sfz[j];
+        }
+    }
+    block_partials<BX * BY>(acc, mx, 0.0, s.partials, s.flags, bad);
+}
+
+// First-order LLF pass over the whole grid, run only when the second-order
+// pass flagged a failure (torchSolver3D.py:161-176, 192-202). The flag test
+// happens on the device so the launch can sit in a captured graph.
+__device__ __forceinline__ void llf_face(const real* L, const real* R, real gl, real gr, int m, real* F) {
+    real pl = (gl - (real)1) * (L[4] - ((real)0.5 * (L[1] * L[1] + L[2] * L[2] + L[3] * L[3])) / L[0]);
+    real pr = (gr - (real)1) * (R[4] - ((real)0.5 * (R[1] * R[1] + R[2] * R[2] + R[3] * R[3])) / R[0]);
+    real speed = nmax(fabs(L[m] / L[0]) + sqrt((gl * pl) / L[0]), fabs(R[m] / R[0]) + sqrt((gr * pr) / R[0]));
+    real fl[5], fr[5];
+    flux3(L, gl, m, fl);
+    flux3(R, gr, m, fr);
+#pragma unroll
+    for (int c = 0; c < 5; ++c) F[c] = (real)0.5 * (fl[c] + fr[c] - speed * (R[c] - L[c]));
+}
+
+__global__ void __launch_bounds__(256) k_llf3d(Step s, G3 g) {
+    double acc[5] = {0, 0, 0, 0, 0}, mx[3] = {0, 0, 0};
+    int bad = 0;
+    const int need = *s.flags & (fvl::F_PRED_BAD | fvl::F_UPD_BAD);
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const long N = (long)nx * ny * nz;
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (need && i < N) {
+        const int x = (int)(i % nx), y = (int)((i / nx) % ny), z = (int)(i / ((long)nx * ny));
+        const real* U = (const real*)s.U;
+        const real* lamp = (const real*)s.lam;
+        real Ac[5], o[5], rqv[2];
+        load5(U, N, g, x, y, z, Ac);
+        const bool direct = s.lam_is_gamma != 0;
+        const real gcen = lamp ? gamma_from(load1(lamp, g, x, y, z), direct) : (real)s.gamma;
+        real qcen[2] = {0, 0};
+        for (int j = 0; j < s.nq; ++j) {
+            qcen[j] = load1((const real*)s.q[j], g, x, y, z);
+            rqv[j] = Ac[0] * qcen[j];
+        }
+#pragma unroll
+        for (int c = 0; c < 5; ++c) o[c] = Ac[c];
+        for (int m = 0; m < 3; ++m) {
+            const int dx = m == 0, dy = m == 1, dz = m == 2;
+            real Lo[5], Hi[5], Fl[5], Fh[5];
+            load5(U, N, g, x - dx, y - dy, z - dz, Lo);
+            load5(U, N, g, x + dx, y + dy, z + dz, Hi);
+            real glo = lamp ? gamma_from(load1(lamp, g, x - dx, y - dy, z - dz), direct) : (real)s.gamma;
+            real ghi = lamp ? gamma_from(load1(lamp, g, x + dx, y + dy, z + dz), direct) : (real)s.gamma;
+            llf_face(Lo, Ac, glo, gcen, m + 1, Fl);
+            llf_face(Ac, Hi, gcen, ghi, m + 1, Fh);
+            const real cm = (real)s.c[m];
+#pragma unroll
+            for (int c = 0; c < 5; ++c) o[c] = o[c] - cm * (Fh[c] - Fl[c]);
+            for (int j = 0; j < s.nq; ++j) {
+                const real* q = (const real*)s.q[j];
+                real qlo = load1(q, g, x - dx, y - dy, z - dz), qhi = load1(q, g, x + dx, y + dy, z + dz);
+                real sl = Fl[0] * (Fl[0] >= (real)0 ? qlo : qcen[j]);
+                real sh = Fh[0] * (Fh[0] >= (real)0 ? qcen[j] : qhi);
+                rqv[j] = rqv[j] - cm * (sh - sl);
+            }
+            const int pos = m == 0 ? x : (m == 1 ? y : z), n = m == 0 ? nx : (m == 1 ? ny : nz);
+            if (pos == 0)
+                for (int c = 0; c < 5; ++c) acc[c] -= s.w[m] * (double)Fl[c];
+            if (pos == n - 1)
+                for (int c = 0; c < 5; ++c) acc[c] += s.w[m] * (double)Fh[c];
+            if (s.F[m]) {
+                const int ex = dx, ey = dy, ez = dz;
+                const long M = (long)(nz + ez) * (ny + ey) * (nx + ex);
+                long f = ((long)z * (ny + ey) + y) * (nx + ex) + x;
+                long fh = ((long)(z + ez) * (ny + ey) + (y + ey)) * (nx + ex) + (x + ex);
+                for (int c = 0; c < 5; ++c) ((real*)s.F[m])[c * M + f] = Fl[c];
+                if (pos == n - 1)
+                    for (int c = 0; c < 5; ++c) ((real*)s.F[m])[c * M + fh] = Fh[c];
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 5; ++c) ((real*)s.Uout)[c * N + i] = o[c];
+        if (!physical5(o)) bad |= fvl::F_LLF_BAD;
+        if (s.want_cfl) wave3(o, (double)s.gamma, mx);
+        for (int j = 0; j < s.nq; ++j) {
+            real f = rqv[j] / o[0];
+            if (f < (real)-1e-10 || f > (real)(1 + 1e-10) || f != f) bad |= fvl::F_SPEC_BAD_LLF;
+            ((real*)s.qout[j])[i] = rmin(rmax(f, (real)0), (real)1);
+        }
+    }
+    if (!need) {
+        if (threadIdx.x == 0) {
+            double* p = s.partials + blockIdx.x * (long)fvl::NSLOT;
+            for (int k = 0; k < 9; ++k) p[k < 3 ? k : (k < 8 ? fvl::S_OUT + k - 3 : fvl::S_SRC)] = 0.0;
+        }
+        return;
+    }
+    block_partials<256>(acc, mx, 0.0, s.partials, s.flags, bad);
+}
+
+// ---------------------------------------------------------------------------
+// Chemistry half-step (torchSolver3D.py:225-243 / torchSolver2D.py:474-509 /
+// torchSession.py:365-369 on top of torchSolver.py:708-789). Always double.
+// ---------------------------------------------------------------------------
+struct Kin {
+    int model;
+    double q, ea, a;
+};
+
+__device__ __forceinline__ double rate_at(const Kin& k, double lam, double n0, double n1, double n2) {
+    double g = (4.5 - lam) / (3.5 - lam);
+    double rho = fmax(n0, 1e-12);
+    double u = n1 / rho;
+    double p = fmax((n2 - (0.5 * rho) * (u * u)) * (g - 1.0), 1e-12);
+    double rho2 = fmax(rho, 1e-12);
+    double molar = lam * (2.0 / 3.0) * 2.016 + lam * (1.0 / 3.0) * 32.0 + (1.0 - lam) * 18.016;
+    double t = fmax((p * molar) / (rho2 * 8314.46), 1e-12);
+    return k.a * exp(-k.ea / (8.314 * t));
+}
+
+__device__ __forceinline__ void advance(const Kin& k, double lam0, double beta0, double n0, double n2, double rate,
+                                        double dur, double& lam, double& beta, double& e) {
+    if (k.model == 1) {
+        double active = fmax(0.0, dur - beta0 / fmax(rate, 1e-300));
+        lam = lam0 * exp(-5.0 * rate * active);
+        beta = fmin(fmax(beta0 - rate * dur, 0.0), 1.0);
+    } else {
+        lam = lam0 * exp(-rate * dur);
+        beta = 0.0;
+    }
+    e = n2 + k.q * n0 * (lam0 - lam);
+}
+
+__global__ void __launch_bounds__(256) k_react(fvl::React r, Grid g) {
+    const int nf = g.dim + 2;
+    const long N = (long)g.nx * g.ny * g.nz;
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    double acc[5] = {0, 0, 0, 0, 0}, mx[3] = {0, 0, 0}, src = 0;
+    int bad = 0;
+    if (i < N) {
+        const real* U = (const real*)r.U;
+        double u[5];
+        for (int c = 0; c < nf; ++c) u[c] = (double)U[c * N + i];
+        const double E = u[nf - 1];
+        double n2;
+        if (g.dim == 3) n2 = E - (0.5 * (u[2] * u[2] + u[3] * u[3])) / u[0];
+        else if (g.dim == 2) n2 = E - (0.5 * (u[2] * u[2])) / u[0];
+        else n2 = E;
+        const double lam0 = (double)((const real*)r.lam)[i];
+        const double beta0 = r.beta ? (double)((const real*)r.beta)[i] : 1.0;
+        bool ok = isfinite(u[0]) && isfinite(u[1]) && isfinite(n2) && u[0] > 1e-12 &&
+                  n2 - (0.5 * (u[1] * u[1])) / u[0] > 1e-12;
+        ok = ok && isfinite(lam0) && lam0 >= 0.0 && lam0 <= 1.0 && isfinite(beta0) && beta0 >= 0.0 && beta0 <= 1.0;
+        if (!ok) bad |= fvl::F_REACT_BAD;
+        Kin k{r.model, r.q_heat, r.activation_energy, r.pre_exponential};
+        double lm, bm, em, lam, beta, e;
+        advance(k, lam0, beta0, u[0], n2, rate_at(k, lam0, u[0], u[1], n2), 0.5 * r.dt, lm, bm, em);
+        advance(k, lam0, beta0, u[0], n2, rate_at(k, lm, u[0], u[1], em), r.dt, lam, beta, e);
+        const double Enew = E + (e - n2);
+        real* Uo = (real*)r.Uout;
+        for (int c = 0; c < nf - 1; ++c) Uo[c * N + i] = U[c * N + i];
+        Uo[(nf - 1) * N + i] = (real)Enew;
+        ((real*)r.lamout)[i] = (real)lam;
+        if (r.betaout) ((real*)r.betaout)[i] = (real)beta;
+        if (r.Ustart) {
+            // (old - start + state - hydro) of the energy row (torchSession.py:513-515).
+            const double es = (double)((const real*)r.Ustart)[(nf - 1) * N + i];
+            const double eo = (double)((const real*)r.Uold)[(nf - 1) * N + i];
+            src = ((eo - es) + (double)(real)Enew) - E;
+        }
+        double o[5];
+        for (int c = 0; c < nf; ++c) o[c] = (double)Uo[c * N + i];
+        double s2 = 0;
+        for (int c = 1; c < nf - 1; ++c) s2 = c == 1 ? o[c] * o[c] : s2 + o[c] * o[c];
+        if (!(o[0] > 1e-12 && o[nf - 1] - (0.5 * s2) / o[0] > 1e-12 && isfinite(o[nf - 1])))
+            bad |= fvl::F_STATE_BAD;
+        if (r.want_cfl && g.dim == 3) {
+            double gg = (4.5 - (double)(real)lam) / (3.5 - (double)(real)lam);
+            double p = (gg - 1.0) * (o[4] - (0.5 * (o[1] * o[1] + o[2] * o[2] + o[3] * o[3])) / o[0]);
+            double a = sqrt((gg * p) / o[0]);
+            for (int m = 0; m < 3; ++m) {
+                double v = fabs(o[m + 1] / o[0]) + a;
+                mx[m] = (v != v || v > mx[m]) ? v : mx[m];
+            }
+        }
+    }
+    block_partials<256>(acc, mx, src, r.partials, r.flags, bad);
+}
+
+// CFL maxima of a stored 3D state (torchSession.py:356-360).
+__global__ void __launch_bounds__(256) k_cfl3d(const real* U, const real* lam, double gamma, Grid g,
+                                               double* partials, int direct) {
+    const long N = (long)g.nx * g.ny * g.nz;
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    double acc[5] = {0, 0, 0, 0, 0}, mx[3] = {0, 0, 0};
+    if (i < N) {
+        real o[5];
+        for (int c = 0; c < 5; ++c) o[c] = U[c * N + i];
+        double gg = lam ? (direct ? (double)lam[i] : (4.5 - (double)lam[i]) / (3.5 - (double)lam[i])) : gamma;
+        wave3(o, gg, mx);
+    }
+    block_partials<256>(acc, mx, 0.0, partials, nullptr, 0);
+}
+
+// Conserved totals (5 row sums) for drift reporting.
+__global__ void __launch_bounds__(256) k_totals(const real* U, Grid g, double* partials, int* flags) {
+    const int nf = g.dim + 2;
+    const long N = (long)g.nx * g.ny * g.nz;
+    double acc[5] = {0, 0, 0, 0, 0}, mx[3] = {0, 0, 0};
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x)
+        for (int c = 0; c < nf; ++c) acc[c] += (double)U[c * N + i];
+    block_partials<256>(acc, mx, 0.0, partials, flags, 0);
+}
+
+// Stand-alone HLLC over face arrays, Cartesian component order
+// (hllc_flux_grid, torchFlux.py:6-77): the drop-in for the CPU flux workers.
+__global__ void __launch_bounds__(256) k_hllc_faces(int nv, int dir, long m, const real* l, const real* r,
+                                                    const real* gl, const real* gr, real* out) {
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    real a[5], b[5], f[5];
+    for (int c = 0; c < nv; ++c) {
+        a[c] = l[c * m + i];
+        b[c] = r[c * m + i];
+    }
+    if (nv == 5) hllc_cart5(a, b, gl[i], gr[i], dir, f);
+    else hllc_cart4(a, b, gl[i], gr[i], dir, f);
+    for (int c = 0; c < nv; ++c) out[c * m + i] = f[c];
+}
+
+__global__ void k_convert_in(const double* src, real* dst, long n) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        dst[i] = (real)src[i];
+}
+__global__ void k_convert_out(const real* src, double* dst, long n) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        dst[i] = (double)src[i];
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------
+static G3 g3(const Grid& g) { return G3{g.nx, g.ny, g.nz, g.bc[0], g.bc[1], g.bc[2]}; }
+
+int step3d_blocks(const Grid& g, const fvl::Launch& l, int zchunk) {
+    int tx = l.block_x - 2, ty = l.block_y - 2;
+    return ((g.nx + tx - 1) / tx) * ((g.ny + ty - 1) / ty) * ((g.nz + zchunk - 1) / zchunk);
+}
+
+template <int BX, int BY, bool SPEC>
+static hipError_t launch_lim(const Grid& g, const Step& s, hipStream_t st) {
+    dim3 grid((g.nx + BX - 3) / (BX - 2), (g.ny + BY - 3) / (BY - 2), (g.nz + s.zchunk - 1) / s.zchunk);
+    dim3 block(BX, BY);
+    G3 q = g3(g);
+    switch (s.limiter) {
+        case VANLEER: k_step3d<VANLEER, BX, BY, SPEC><<<grid, block, 0, st>>>(s, q); break;
+        case ALBADA: k_step3d<ALBADA, BX, BY, SPEC><<<grid, block, 0, st>>>(s, q); break;
+        case MINBEE: k_step3d<MINBEE, BX, BY, SPEC><<<grid, block, 0, st>>>(s, q); break;
+        case SUPERBEE: k_step3d<SUPERBEE, BX, BY, SPEC><<<grid, block, 0, st>>>(s, q); break;
+        default: k_step3d<UNOS, BX, BY, SPEC><<<grid, block, 0, st>>>(s, q); break;
+    }
+    return hipGetLastError();
+}
+
+template <bool SPEC>
+static hipError_t launch_shape(const Grid& g, const Step& s, const fvl::Launch& l) {
+    if (l.block_x == 64 && l.block_y == 4) return launch_lim<64, 4, SPEC>(g, s, l.stream);
+    if (l.block_x == 64 && l.block_y == 8) return launch_lim<64, 8, SPEC>(g, s, l.stream);
+    if (l.block_x == 32 && l.block_y == 8) return launch_lim<32, 8, SPEC>(g, s, l.stream);
+    if (l.block_x == 32 && l.block_y == 16) return launch_lim<32, 16, SPEC>(g, s, l.stream);
+    return hipErrorInvalidValue;
+}
+
+hipError_t step3d(const Grid& g, const Step& s, const fvl::Launch& l) {
+    return s.nq > 0 ? launch_shape<true>(g, s, l) : launch_shape<false>(g, s, l);
+}
+
+int llf3d_blocks(const Grid& g) { return (int)(((long)g.nx * g.ny * g.nz + 255) / 256); }
+
+hipError_t llf3d(const Grid& g, const Step& s, hipStream_t st) {
+    k_llf3d<<<llf3d_blocks(g), 256, 0, st>>>(s, g3(g));
+    return hipGetLastError();
+}
+
+int react_blocks(const Grid& g) { return (int)(((long)g.nx * g.ny * g.nz + 255) / 256); }
+
+hipError_t react(const Grid& g, const fvl::React& r, hipStream_t st) {
+    k_react<<<react_blocks(g), 256, 0, st>>>(r, g);
+    return hipGetLastError();
+}
+
+int cfl_blocks(const Grid& g) {
+    if (g.dim == 2) return cfl2d_blocks(g);
+    if (g.dim == 1) return cfl1d_blocks(g);
+    return (int)(((long)g.nx * g.ny * g.nz + 255) / 256);
+}
+
+hipError_t cfl_cells(const Grid& g, const void* U, const void* lam, double gamma, double* partials, int direct,
+                     hipStream_t st) {
+    if (g.dim == 2) return cfl2d(g, U, lam, gamma, partials, direct, st);
+    if (g.dim == 1) {
+        Step s;
+        memset(&s, 0, sizeof(s));
+        s.U = U;
+        s.lam = lam;
+        s.lam_is_gamma = direct;
+        s.gamma = gamma;
+        s.wavespeed = g.wavespeed;
+        s.partials = partials;
+        return cfl1d(g, s, st);
+    }
+    k_cfl3d<<<cfl_blocks(g), 256, 0, st>>>((const real*)U, (const real*)lam, gamma, g, partials, direct);
+    return hipGetLastError();
+}
+
+hipError_t hllc_faces(int nv, int dir, long m, const void* l, const void* r, const void* gl, const void* gr,
+                      void* out, hipStream_t st) {
+    if (m <= 0) return hipSuccess;
+    k_hllc_faces<<<(unsigned)((m + 255) / 256), 256, 0, st>>>(nv, dir, m, (const real*)l, (const real*)r,
+                                                              (const real*)gl, (const real*)gr, (real*)out);
+    return hipGetLastError();
+}
+
+hipError_t totals(const Grid& g, const void* U, double* partials, hipStream_t st) {
+    k_totals<<<256, 256, 0, st>>>((const real*)U, g, partials, nullptr);
+    return hipGetLastError();
+}
+
+hipError_t convert_in(const double* src, void* dst, long n, hipStream_t st) {
+    k_convert_in<<<1024, 256, 0, st>>>(src, (real*)dst, n);
+    return hipGetLastError();
+}
+hipError_t convert_out(const void* src, double* dst, long n, hipStream_t st) {
+    k_convert_out<<<1024, 256, 0, st>>>((const real*)src, dst, n);
+    return hipGetLastError();
+}
+
+}  // namespace FV_NS

@@ -1,0 +1,249 @@
+// Device-side building blocks of the Cartesian finite-volume step.
+//
+// Compiled twice from the same source: with FV_REAL=double (the plain method,
+// -ffp-contract=off so every product and sum rounds like the reference's
+// NumPy arithmetic) and with FV_REAL=float (the FP32 form, contraction and
+// fast division/square root). Operation order follows the reference
+// expressions cited beside each function so the FP64 build stays within a
+// few ulps of python/cfdSolver (torchFlux.py, torchSolver3D.py,
+// torchSolver2D.py, torchSolver.py).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#ifndef FV_REAL
+#define FV_REAL double
+#endif
+
+typedef FV_REAL real;
+
+namespace fv {
+
+enum Limiter { VANLEER = 0, ALBADA = 1, MINBEE = 2, SUPERBEE = 3, UNOS = 4 };
+enum Boundary { BC_OPEN = 0, BC_PERIODIC = 1, BC_REFLECTIVE = 2, BC_INFLOW = 3 };
+
+__device__ __forceinline__ real rmax(real a, real b) { return a > b ? a : (b > a ? b : (a == a ? a : b)); }
+__device__ __forceinline__ real rmin(real a, real b) { return a < b ? a : (b < a ? b : (a == a ? a : b)); }
+// NumPy's maximum/minimum propagate NaN; so do these (a NaN operand wins).
+__device__ __forceinline__ real nmax(real a, real b) { return (a != a || b != b) ? (a + b) : (a > b ? a : b); }
+__device__ __forceinline__ real nmin(real a, real b) { return (a != a || b != b) ? (a + b) : (a < b ? a : b); }
+
+// |x| < 1e-14 -> +1e-14 (torchFlux.py:190, torchSolver3D.py:90).
+__device__ __forceinline__ real guard14(real x) { return fabs(x) < (real)1e-14 ? (real)1e-14 : x; }
+
+// Ghost-cell source index along one axis of length n for a (possibly
+// far out-of-range) index i, and whether i is a ghost. Periodic wraps,
+// Open repeats the edge cell, Reflective mirrors (np.pad wrap/edge/
+// symmetric, torchSolver3D.py:47-70). Indices beyond two ghost layers are
+// first clamped so that masked lanes still read inside the array.
+__device__ __forceinline__ int ghost_src(int i, int n, int kind) {
+    if (i < -2) i = -2;
+    if (i > n + 1) i = n + 1;
+    if (i >= 0 && i < n) return i;
+    if (kind == BC_PERIODIC) return i < 0 ? i + n : i - n;
+    if (kind == BC_REFLECTIVE) return i < 0 ? -i - 1 : 2 * n - 1 - i;
+    return i < 0 ? 0 : n - 1;  // open / transmissive (also Inflow's upper side)
+}
+
+__device__ __forceinline__ bool ghost_flip(int i, int n, int kind) {
+    return kind == BC_REFLECTIVE && (i < 0 || i >= n);
+}
+
+// Limiter phi(r) of the 2D/3D MUSCL schemes (torchSolver3D.py:91-110).
+template <int LIM>
+__device__ __forceinline__ real phi3(real r) {
+    if (LIM == VANLEER) {
+        real pos = nmax(r, (real)0);
+        return ((real)2 * pos) / ((real)1 + pos);
+    } else if (LIM == ALBADA) {
+        real pos = nmax(r, (real)0);
+        return (pos * pos + pos) / (pos * pos + (real)1);
+    } else if (LIM == MINBEE) {
+        return nmax((real)0, nmin(r, (real)1));
+    } else if (LIM == SUPERBEE) {
+        return nmax((real)0, nmax(nmin((real)2 * r, (real)1), nmin(r, (real)2)));
+    } else {
+        return nmax((real)0, nmin(nmin(nmin((real)2, (real)2 * r), ((real)2 + r) / (real)3),
+                                  ((real)1 + (real)2 * r) / (real)3));
+    }
+}
+
+// 2D limiters differ: VanLeer/Albada use where(r > 0, ..., 0) and Unos has no
+// (1+2r)/3 bound (torchSolver2D.py:144-168).
+template <int LIM>
+__device__ __forceinline__ real phi2(real r) {
+    if (LIM == VANLEER) {
+        real pos = nmax(r, (real)0);
+        return r > (real)0 ? ((real)2 * pos) / ((real)1 + pos) : (real)0;
+    } else if (LIM == ALBADA) {
+        real pos = nmax(r, (real)0);
+        return r > (real)0 ? (pos * pos + pos) / (pos * pos + (real)1) : (real)0;
+    } else if (LIM == MINBEE) {
+        return nmax((real)0, nmin(r, (real)1));
+    } else if (LIM == SUPERBEE) {
+        return nmax((real)0, nmax(nmin((real)2 * r, (real)1), nmin(r, (real)2)));
+    } else {
+        return nmax((real)0, nmin(nmin((real)2, (real)2 * r), ((real)2 + r) / (real)3));
+    }
+}
+
+// Limited slope phi(minus/plus) * plus (torchSolver3D.py:86-111).
+template <int LIM, int DIM>
+__device__ __forceinline__ real limited(real minus, real plus) {
+    real r = minus / guard14(plus);
+    return (DIM == 3 ? phi3<LIM>(r) : phi2<LIM>(r)) * plus;
+}
+
+// Physical flux along momentum component m of a 5-component state, no
+// floors (torchSolver3D.py:37-44, 77-83).
+__device__ __forceinline__ void flux3(const real* u, real g, int m, real* f) {
+    real vel = u[m] / u[0];
+    real p = (g - (real)1) * (u[4] - ((real)0.5 * (u[1] * u[1] + u[2] * u[2] + u[3] * u[3])) / u[0]);
+#pragma unroll
+    for (int c = 0; c < 5; ++c) f[c] = u[c] * vel;
+    f[m] = f[m] + p;
+    f[4] = vel * (u[4] + p);
+}
+
+// physical_state on one 5-component state (torchSolver.py:1038-1046).
+__device__ __forceinline__ bool physical5(const real* s) {
+    bool finite = true;
+#pragma unroll
+    for (int c = 0; c < 5; ++c) finite = finite && isfinite(s[c]);
+    real e = s[4] - ((real)0.5 * (s[1] * s[1] + s[2] * s[2] + s[3] * s[3])) / s[0];
+    return finite && s[0] > (real)1e-12 && e > (real)1e-12;
+}
+
+__device__ __forceinline__ bool physical4(const real* s) {
+    bool finite = isfinite(s[0]) && isfinite(s[1]) && isfinite(s[2]) && isfinite(s[3]);
+    real e = s[3] - ((real)0.5 * (s[1] * s[1] + s[2] * s[2])) / s[0];
+    return finite && s[0] > (real)1e-12 && e > (real)1e-12;
+}
+
+// Normal-basis HLLC flux (torchFlux.py:125-248). NV = 5 (3D) or 4 (2D);
+// rows [rho, m_n, m_t..., E]. Region choice reproduces the reference's
+// ordered masks L, *L, *R, R (last assignment wins); a face that matches
+// none (NaN speeds) yields NaN.
+template <int NV>
+__device__ __forceinline__ void hllc(const real* ul, const real* ur, real gl, real gr, real* out) {
+    const int E = NV - 1;
+    real gbar = (real)0.5 * (gl + gr);
+    real rl = rmax(ul[0], (real)1e-12), rr = rmax(ur[0], (real)1e-12);
+    real unl = ul[1] / rl, unr = ur[1] / rr;
+    real utl[NV - 3], utr[NV - 3];
+    real sl2 = 0, sr2 = 0;
+#pragma unroll
+    for (int t = 0; t < NV - 3; ++t) {
+        utl[t] = ul[2 + t] / rl;
+        utr[t] = ur[2 + t] / rr;
+        sl2 = t == 0 ? utl[t] * utl[t] : sl2 + utl[t] * utl[t];
+        sr2 = t == 0 ? utr[t] * utr[t] : sr2 + utr[t] * utr[t];
+    }
+    real kel = ((real)0.5 * rl) * (unl * unl + sl2);
+    real ker = ((real)0.5 * rr) * (unr * unr + sr2);
+    real pl = rmax((gl - (real)1) * (ul[E] - kel), (real)1e-12);
+    real pr = rmax((gr - (real)1) * (ur[E] - ker), (real)1e-12);
+    real al = sqrt((gl * pl) / rl), ar = sqrt((gr * pr) / rr);
+
+    real sql = sqrt(rl), sqr = sqrt(rr);
+    real inv = (real)1 / (sql + sqr);
+    real u_roe = (sql * unl + sqr * unr) * inv;
+    real kv = 0;
+#pragma unroll
+    for (int t = 0; t < NV - 3; ++t) {
+        real v = (sql * utl[t] + sqr * utr[t]) * inv;
+        kv = t == 0 ? v * v : kv + v * v;
+    }
+    real hl = (ul[E] + pl) / rl, hr = (ur[E] + pr) / rr;
+    real h_roe = (sql * hl + sqr * hr) * inv;
+    real ke_roe = (real)0.5 * (u_roe * u_roe + kv);
+    real a_roe = sqrt(rmax((gbar - (real)1) * (h_roe - ke_roe), (real)1e-12));
+    real s_l = nmin(unl - al, u_roe - a_roe);
+    real s_r = nmax(unr + ar, u_roe + a_roe);
+    real den = guard14(rl * (s_l - unl) - rr * (s_r - unr));
+    real s_star = (pr - pl + (rl * unl) * (s_l - unl) - (rr * unr) * (s_r - unr)) / den;
+
+    // Final region after the ordered masks.
+    int region;  // 0 L, 1 *L, 2 *R, 3 R, -1 none
+    if ((real)0 >= s_r) region = 3;
+    else if (s_star <= (real)0 && (real)0 <= s_r) region = 2;
+    else if (s_l <= (real)0 && (real)0 <= s_star) region = 1;
+    else if ((real)0 <= s_l) region = 0;
+    else region = -1;
+    if (region < 0) {
+#pragma unroll
+        for (int c = 0; c < NV; ++c) out[c] = (real)NAN;
+        return;
+    }
+    bool left = region <= 1;
+    const real* u = left ? ul : ur;
+    real rho = left ? rl : rr, un = left ? unl : unr, p = left ? pl : pr;
+    real s = left ? s_l : s_r;
+    real f[NV];
+    f[0] = rho * un;
+    f[1] = rho * (un * un) + p;
+#pragma unroll
+    for (int t = 0; t < NV - 3; ++t) f[2 + t] = (rho * un) * (left ? utl[t] : utr[t]);
+    f[E] = un * (u[E] + p);
+    bool star = (region == 1 || region == 2) && !(fabs(s - s_star) < (real)1e-12);
+    if (!star) {
+#pragma unroll
+        for (int c = 0; c < NV; ++c) out[c] = f[c];
+        return;
+    }
+    real fac = (s - un) / guard14(s - s_star);
+    real rs = rmax(rho * fac, (real)1e-12);
+    real us[NV];
+    us[0] = rs;
+    us[1] = rs * s_star;
+#pragma unroll
+    for (int t = 0; t < NV - 3; ++t) us[2 + t] = rs * (left ? utl[t] : utr[t]);
+    us[E] = rs * (u[E] / rho + (s_star - un) * (s_star + p / (rho * guard14(s - un))));
+#pragma unroll
+    for (int c = 0; c < NV; ++c) out[c] = f[c] + s * (us[c] - u[c]);
+}
+
+// HLLC between Cartesian-ordered 5-component states across a face normal to
+// axis `dir` (0 x, 1 y, 2 z): rotate to [rho, m_n, m_t(asc), E] and back
+// (torchFlux.py:36-39, 75-76).
+__device__ __forceinline__ void hllc_cart5(const real* l, const real* r, real gl, real gr, int dir, real* f) {
+    real a[5], b[5], o[5];
+    int t0 = dir == 0 ? 2 : 1, t1 = dir == 2 ? 2 : 3;
+    a[0] = l[0]; a[1] = l[dir + 1]; a[2] = l[t0]; a[3] = l[t1]; a[4] = l[4];
+    b[0] = r[0]; b[1] = r[dir + 1]; b[2] = r[t0]; b[3] = r[t1]; b[4] = r[4];
+    hllc<5>(a, b, gl, gr, o);
+    f[0] = o[0]; f[dir + 1] = o[1]; f[t0] = o[2]; f[t1] = o[3]; f[4] = o[4];
+}
+
+__device__ __forceinline__ void hllc_cart4(const real* l, const real* r, real gl, real gr, int dir, real* f) {
+    real a[4], b[4], o[4];
+    int t = dir == 0 ? 2 : 1;
+    a[0] = l[0]; a[1] = l[dir + 1]; a[2] = l[t]; a[3] = l[3];
+    b[0] = r[0]; b[1] = r[dir + 1]; b[2] = r[t]; b[3] = r[3];
+    hllc<4>(a, b, gl, gr, o);
+    f[0] = o[0]; f[dir + 1] = o[1]; f[t] = o[2]; f[3] = o[3];
+}
+
+__device__ __forceinline__ real gamma_of(real lam) { return ((real)4.5 - lam) / ((real)3.5 - lam); }
+// Per-cell gamma from a lambda array (gamma_calc, torchSolver.py:71-78) or
+// taken as stored when the array already holds gamma.
+__device__ __forceinline__ real gamma_from(real v, bool direct) { return direct ? v : gamma_of(v); }
+
+// 64-bit-safe lane shuffles for one wave (width 64).
+__device__ __forceinline__ real shfl_up1(real v) { return __shfl_up(v, 1, 64); }
+__device__ __forceinline__ real shfl_down1(real v) { return __shfl_down(v, 1, 64); }
+
+// Wave/block max of a nonnegative double (order independent, so exact).
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+}  // namespace fv
