@@ -271,6 +271,9 @@ int run_react(torch_state* s, int in, int out, double dt, int start, int old, bo
     r.flags = s->flags;
     r.partials = region(s, 2);
     r.want_cfl = want_cfl && s->cfg.dim == 3;
+    if (!r.U || !r.Uout || !r.lam || !r.lamout || (s->cfg.kinetic_model == 1 && (!r.beta || !r.betaout)) ||
+        (start >= 0 && (!r.Ustart || !r.Uold)))
+        return fail(TORCH_INVALID, "internal: chemistry buffers missing for this state");
     hipStream_t st = s->ctx->stream;
     HIP_TRY(s->ops->react(s->grid, r, st));
     HIP_TRY(fvl::finalize(region(s, 2), s->ops->react_blocks(s->grid), s->scalars + 2 * fvl::NSLOT, st));
@@ -284,6 +287,13 @@ int run_hydro(torch_state* s, int in, int out, double dt, int lam_in, int lam_ou
     const bool reactive = s->cfg.reactive;
     const void* lam = s->cfg.gamma_mode ? s->lam[lam_in] : nullptr;
     fvl::Step st = base_step(s, in, out, dt, lam);
+    if (!st.U || !st.Uout || (s->cfg.gamma_mode && !lam) || (s->cfg.dim == 1 && !s->scratch1d))
+        return fail(TORCH_INVALID, "internal: hydro buffers missing for this state");
+    if (reactive && (!s->lam[lam_in] || !s->lam[lam_out] || !s->beta[lam_in] || !s->beta[lam_out]))
+        return fail(TORCH_INVALID, "internal: species buffers missing for this state");
+    if (want_fluxes)
+        for (int d = 0; d < s->cfg.dim; ++d)
+            if (!s->F[d]) return fail(TORCH_INVALID, "internal: flux buffers missing");
     st.lam_is_gamma = s->cfg.gamma_mode == 2;
     if (reactive) {
         st.nq = s->cfg.kinetic_model == 1 ? 2 : 1;
@@ -448,7 +458,9 @@ int torch_state_create(torch_ctx* ctx, const torch_config* cfg, torch_state** ou
     s->ncell = (long)g.nx * g.ny * g.nz;
     s->nf = c.dim + 2;
     const size_t e = s->ops->elem;
-    const int nbuf = c.reactive ? 4 : 2;
+    // Inert states with a scalar gamma rotate between 2 slots; states with a
+    // per-cell lambda/gamma array use 4 (torch_state_hydro writes slot +2).
+    const int nbuf = (c.reactive || c.gamma_mode) ? 4 : 2;
     int rc = TORCH_OK;
     for (int i = 0; i < nbuf && !rc; ++i) rc = alloc(&s->U[i], s->nf * s->ncell * e);
     if (c.gamma_mode)
