@@ -123,6 +123,28 @@ def cpu_baseline(budget_s=12.0, n=40):
                       f"(restated reference CPU path, 1 thread), {wall:.1f} s"}
 
 
+def copy_calibration(mib=1024, reps=3):
+    """Device-to-device copies of a known size, for scaling rocprof's
+    FETCH_SIZE/WRITE_SIZE counters (each copy reads and writes `mib` MiB)."""
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for p in (a, b):
+        if hip.hipMalloc(ctypes.byref(p), ctypes.c_size_t(n)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, ctypes.c_size_t(n))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3)  # device to device
+    hip.hipDeviceSynchronize()
+    wall = time.perf_counter() - t0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_mib": mib, "reps": reps, "GBps_rw": round(2 * n * reps / wall / 1e9, 1)}))
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -131,7 +153,12 @@ def main(argv=None):
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--error-steps", type=int, default=50, help="steps of the FP32-vs-FP64 error pass")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run device copies of MIB MiB (counter calibration) and exit")
     args = ap.parse_args(argv)
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return 0
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (the workload is the "
               "single-GPU 3D Sod tube; multi-GPU sharding is described, not run).", file=sys.stderr)

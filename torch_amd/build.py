@@ -29,7 +29,7 @@ COMMON = [
 ]
 PRECISION = {
     "fv64": ["-DFV_REAL=double", "-DFV_NS=fv64", "-ffp-contract=off"],
-    "fv32": ["-DFV_REAL=float", "-DFV_NS=fv32", "-ffp-contract=fast",
+    "fv32": ["-DFV_REAL=float", "-DFV_NS=fv32", "-DFV_FAST=1", "-ffp-contract=fast",
              "-fno-hip-fp32-correctly-rounded-divide-sqrt"],
 }
 

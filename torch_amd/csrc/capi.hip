@@ -63,8 +63,9 @@ const Ops kOps64 = OPS_OF(fv64, double);
 const Ops kOps32 = OPS_OF(fv32, float);
 
 // Fold per-block partials in a fixed order (deterministic sums).
-__global__ void k_finalize(const double* partials, int nblocks, double* out) {
+__global__ void k_finalize(const double* partials, int nblocks, double* out, const int* flags, int mask) {
     __shared__ double buf[256];
+    if (flags && !(*flags & mask)) return;  // a pass that did not run left no partials
     const int slot = blockIdx.x;
     const bool is_max = slot < 3;
     double v = 0.0;
@@ -97,7 +98,12 @@ hipError_t clear_flags(int* flags, hipStream_t st) {
 
 namespace fvl {
 hipError_t finalize(const double* partials, int nblocks, double* out, hipStream_t st) {
-    k_finalize<<<NSLOT, 256, 0, st>>>(partials, nblocks, out);
+    k_finalize<<<NSLOT, 256, 0, st>>>(partials, nblocks, out, nullptr, 0);
+    return hipGetLastError();
+}
+hipError_t finalize_if(const double* partials, int nblocks, double* out, const int* flags, int mask,
+                       hipStream_t st) {
+    k_finalize<<<NSLOT, 256, 0, st>>>(partials, nblocks, out, flags, mask);
     return hipGetLastError();
 }
 }  // namespace fvl
@@ -319,7 +325,8 @@ int run_hydro(torch_state* s, int in, int out, double dt, int lam_in, int lam_ou
         HIP_TRY(fvl::finalize(region(s, 0), s->ops->step3d_blocks(s->grid, l, st.zchunk), s->scalars, stream));
         st.partials = region(s, 1);
         HIP_TRY(s->ops->llf3d(s->grid, st, stream));
-        HIP_TRY(fvl::finalize(region(s, 1), s->ops->llf3d_blocks(s->grid), s->scalars + fvl::NSLOT, stream));
+        HIP_TRY(fvl::finalize_if(region(s, 1), s->ops->llf3d_blocks(s->grid), s->scalars + fvl::NSLOT, s->flags,
+                                 fvl::F_PRED_BAD | fvl::F_UPD_BAD, stream));
     } else if (s->cfg.dim == 2) {
         fvl::Launch l = launch_of(s);
         st.partials = region(s, 0);
@@ -327,7 +334,8 @@ int run_hydro(torch_state* s, int in, int out, double dt, int lam_in, int lam_ou
         HIP_TRY(fvl::finalize(region(s, 0), s->ops->step2d_blocks(s->grid, l), s->scalars, stream));
         st.partials = region(s, 1);
         HIP_TRY(s->ops->llf2d(s->grid, st, stream));
-        HIP_TRY(fvl::finalize(region(s, 1), s->ops->llf2d_blocks(s->grid), s->scalars + fvl::NSLOT, stream));
+        HIP_TRY(fvl::finalize_if(region(s, 1), s->ops->llf2d_blocks(s->grid), s->scalars + fvl::NSLOT, s->flags,
+                                 fvl::F_UPD_BAD, stream));
     } else {
         st.partials = region(s, 0);
         HIP_TRY(s->ops->step1d(s->grid, st, s->scratch1d, stream));

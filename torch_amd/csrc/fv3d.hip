@@ -85,18 +85,31 @@ __device__ void block_partials(double (&acc)[5], double (&mx)[3], double src, do
     }
 }
 
-// |u_m| + a of a committed 3D state (torchSession.py:356-360); double.
+// |u_m| + a of a committed 3D state (torchSession.py:356-360), reduced in
+// double. The FP32 form evaluates the speeds in float (exact when widened).
 __device__ __forceinline__ void wave3(const real* o, double g, double (&mx)[3]) {
+#ifdef FV_FAST
+    const real ir = rcp(o[0]), gg = (real)g;
+    const real p = (gg - (real)1) * (o[4] - ((real)0.5 * (o[1] * o[1] + o[2] * o[2] + o[3] * o[3])) * ir);
+    const real a = rsqrt_((gg * p) * ir);
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+        double v = (double)(fabs(o[m + 1] * ir) + a);
+        mx[m] = (v != v || v > mx[m]) ? v : mx[m];
+    }
+#else
     double u[5];
 #pragma unroll
     for (int c = 0; c < 5; ++c) u[c] = (double)o[c];
-    double p = (g - 1.0) * (u[4] - (0.5 * (u[1] * u[1] + u[2] * u[2] + u[3] * u[3])) / u[0]);
-    double a = sqrt((g * p) / u[0]);
+    const double ir = 1.0 / u[0];
+    double p = (g - 1.0) * (u[4] - (0.5 * (u[1] * u[1] + u[2] * u[2] + u[3] * u[3])) * ir);
+    double a = sqrt((g * p) * ir);
 #pragma unroll
     for (int m = 0; m < 3; ++m) {
-        double v = fabs(u[m + 1] / u[0]) + a;
+        double v = fabs(u[m + 1] * ir) + a;
         mx[m] = (v != v || v > mx[m]) ? v : mx[m];
     }
+#endif
 }
 
 template <int LIM, int BX, int BY, bool SPEC>
@@ -373,8 +386,8 @@ __global__ void __launch_bounds__(256) k_llf3d(Step s, G3 g) {
     const int need = *s.flags & (fvl::F_PRED_BAD | fvl::F_UPD_BAD);
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const long N = (long)nx * ny * nz;
-    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (need && i < N) {
+    if (!need) return;  // the host reads these partials only after a fallback
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
         const int x = (int)(i % nx), y = (int)((i / nx) % ny), z = (int)(i / ((long)nx * ny));
         const real* U = (const real*)s.U;
         const real* lamp = (const real*)s.lam;
@@ -432,13 +445,6 @@ __global__ void __launch_bounds__(256) k_llf3d(Step s, G3 g) {
             if (f < (real)-1e-10 || f > (real)(1 + 1e-10) || f != f) bad |= fvl::F_SPEC_BAD_LLF;
             ((real*)s.qout[j])[i] = rmin(rmax(f, (real)0), (real)1);
         }
-    }
-    if (!need) {
-        if (threadIdx.x == 0) {
-            double* p = s.partials + blockIdx.x * (long)fvl::NSLOT;
-            for (int k = 0; k < 9; ++k) p[k < 3 ? k : (k < 8 ? fvl::S_OUT + k - 3 : fvl::S_SRC)] = 0.0;
-        }
-        return;
     }
     block_partials<256>(acc, mx, 0.0, s.partials, s.flags, bad);
 }
@@ -620,7 +626,10 @@ hipError_t step3d(const Grid& g, const Step& s, const fvl::Launch& l) {
     return s.nq > 0 ? launch_shape<true>(g, s, l) : launch_shape<false>(g, s, l);
 }
 
-int llf3d_blocks(const Grid& g) { return (int)(((long)g.nx * g.ny * g.nz + 255) / 256); }
+int llf3d_blocks(const Grid& g) {
+    long b = ((long)g.nx * g.ny * g.nz + 255) / 256;
+    return (int)(b < 2048 ? b : 2048);
+}
 
 hipError_t llf3d(const Grid& g, const Step& s, hipStream_t st) {
     k_llf3d<<<llf3d_blocks(g), 256, 0, st>>>(s, g3(g));

@@ -88,22 +88,88 @@ __device__ __forceinline__ real phi2(real r) {
     }
 }
 
-// Limited slope phi(minus/plus) * plus (torchSolver3D.py:86-111).
+// Reciprocal and division. The FP32 form uses the hardware reciprocal and
+// square root (one instruction each, about 1 ulp). The FP64 plain method
+// keeps IEEE division where the reference divides once, and reuses one
+// reciprocal where the reference divides several values by the same
+// density (a <= 2 ulp change per product, far inside the 1e-12 bar).
+#ifdef FV_FAST
+__device__ __forceinline__ real rcp(real x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ real dv(real a, real b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ real rsqrt_(real x) { return __builtin_amdgcn_sqrtf(x); }
+#else
+__device__ __forceinline__ real rcp(real x) { return (real)1 / x; }
+__device__ __forceinline__ real dv(real a, real b) { return a / b; }
+__device__ __forceinline__ real rsqrt_(real x) { return sqrt(x); }
+#endif
+
+// Limited slope phi(minus/plus) * plus (torchSolver3D.py:86-111,
+// torchSolver2D.py:144-168) without the division by `plus` where the
+// limiter allows it. With r = minus/plus, phi(r)*plus is piecewise linear in
+// (minus, plus) for MINBEE, Superbee and Unos, and a single quotient for
+// VanLeer (2 m p / (m + p)) and Albada ((m^2 + m p) p / (m^2 + p^2)); the
+// piece is chosen by comparing |minus| with multiples of |plus| (same sign
+// <=> r > 0). At the piece boundaries the limiter is continuous, so a tie
+// decided one ulp differently moves the slope by at most an ulp. When
+// |plus| < 1e-14 the reference divides by +1e-14 instead (so r may take
+// minus's sign alone); that branch is evaluated literally.
 template <int LIM, int DIM>
 __device__ __forceinline__ real limited(real minus, real plus) {
-    real r = minus / guard14(plus);
-    return (DIM == 3 ? phi3<LIM>(r) : phi2<LIM>(r)) * plus;
+    if (fabs(plus) < (real)1e-14) {
+        real r = minus / (real)1e-14;
+        return (DIM == 3 ? phi3<LIM>(r) : phi2<LIM>(r)) * plus;
+    }
+    if (!(minus * plus > (real)0)) {
+        // r <= 0 (or NaN): every limiter gives phi = 0 except through NaN.
+        return (minus == minus && plus == plus) ? (real)0 * plus : minus + plus;
+    }
+    const real am = fabs(minus), ap = fabs(plus);
+    if (LIM == MINBEE) {
+        return am >= ap ? plus : minus;
+    } else if (LIM == SUPERBEE) {
+        if ((real)2 * am <= ap) return (real)2 * minus;
+        if (am <= ap) return plus;
+        if (am <= (real)2 * ap) return minus;
+        return (real)2 * plus;
+    } else if (LIM == UNOS) {
+        const real third = (real)(1.0 / 3.0);
+        real m = rmin((real)2 * ap, (real)2 * am);
+        m = rmin(m, ((real)2 * ap + am) * third);
+        if (DIM == 3) m = rmin(m, (ap + (real)2 * am) * third);
+        return plus > (real)0 ? m : -m;
+    } else if (LIM == VANLEER) {
+        return dv((real)2 * minus * plus, plus + minus);
+    } else {
+        return dv((minus * minus + minus * plus) * plus, minus * minus + plus * plus);
+    }
 }
 
 // Physical flux along momentum component m of a 5-component state, no
 // floors (torchSolver3D.py:37-44, 77-83).
 __device__ __forceinline__ void flux3(const real* u, real g, int m, real* f) {
-    real vel = u[m] / u[0];
-    real p = (g - (real)1) * (u[4] - ((real)0.5 * (u[1] * u[1] + u[2] * u[2] + u[3] * u[3])) / u[0]);
+    const real ir = rcp(u[0]);
+    real vel = u[m] * ir;
+    real p = (g - (real)1) * (u[4] - ((real)0.5 * (u[1] * u[1] + u[2] * u[2] + u[3] * u[3])) * ir);
 #pragma unroll
     for (int c = 0; c < 5; ++c) f[c] = u[c] * vel;
     f[m] = f[m] + p;
     f[4] = vel * (u[4] + p);
+}
+
+// internal energy > 1e-12 decided exactly: a cheap test with a relative
+// margin far wider than its rounding settles almost every state; states
+// within the margin of the threshold take the literal IEEE division.
+__device__ __forceinline__ bool energy_ok(real e_total, real half_s2, real rho) {
+#ifdef FV_FAST
+    return e_total - half_s2 * rcp(rho) > (real)1e-12;
+#else
+    const real approx = half_s2 * (real)__builtin_amdgcn_rcp((double)rho);
+    const real slack = fabs(approx) * (real)1e-6 + (real)1e-300;
+    const real e = e_total - approx;
+    if (e - slack > (real)1e-12) return true;
+    if (e + slack <= (real)1e-12) return false;
+    return e_total - half_s2 / rho > (real)1e-12;
+#endif
 }
 
 // physical_state on one 5-component state (torchSolver.py:1038-1046).
@@ -111,14 +177,14 @@ __device__ __forceinline__ bool physical5(const real* s) {
     bool finite = true;
 #pragma unroll
     for (int c = 0; c < 5; ++c) finite = finite && isfinite(s[c]);
-    real e = s[4] - ((real)0.5 * (s[1] * s[1] + s[2] * s[2] + s[3] * s[3])) / s[0];
-    return finite && s[0] > (real)1e-12 && e > (real)1e-12;
+    if (!(finite && s[0] > (real)1e-12)) return false;
+    return energy_ok(s[4], (real)0.5 * (s[1] * s[1] + s[2] * s[2] + s[3] * s[3]), s[0]);
 }
 
 __device__ __forceinline__ bool physical4(const real* s) {
     bool finite = isfinite(s[0]) && isfinite(s[1]) && isfinite(s[2]) && isfinite(s[3]);
-    real e = s[3] - ((real)0.5 * (s[1] * s[1] + s[2] * s[2])) / s[0];
-    return finite && s[0] > (real)1e-12 && e > (real)1e-12;
+    if (!(finite && s[0] > (real)1e-12)) return false;
+    return energy_ok(s[3], (real)0.5 * (s[1] * s[1] + s[2] * s[2]), s[0]);
 }
 
 // Normal-basis HLLC flux (torchFlux.py:125-248). NV = 5 (3D) or 4 (2D);
@@ -130,13 +196,14 @@ __device__ __forceinline__ void hllc(const real* ul, const real* ur, real gl, re
     const int E = NV - 1;
     real gbar = (real)0.5 * (gl + gr);
     real rl = rmax(ul[0], (real)1e-12), rr = rmax(ur[0], (real)1e-12);
-    real unl = ul[1] / rl, unr = ur[1] / rr;
+    const real irl = rcp(rl), irr = rcp(rr);
+    real unl = ul[1] * irl, unr = ur[1] * irr;
     real utl[NV - 3], utr[NV - 3];
     real sl2 = 0, sr2 = 0;
 #pragma unroll
     for (int t = 0; t < NV - 3; ++t) {
-        utl[t] = ul[2 + t] / rl;
-        utr[t] = ur[2 + t] / rr;
+        utl[t] = ul[2 + t] * irl;
+        utr[t] = ur[2 + t] * irr;
         sl2 = t == 0 ? utl[t] * utl[t] : sl2 + utl[t] * utl[t];
         sr2 = t == 0 ? utr[t] * utr[t] : sr2 + utr[t] * utr[t];
     }
@@ -144,10 +211,10 @@ __device__ __forceinline__ void hllc(const real* ul, const real* ur, real gl, re
     real ker = ((real)0.5 * rr) * (unr * unr + sr2);
     real pl = rmax((gl - (real)1) * (ul[E] - kel), (real)1e-12);
     real pr = rmax((gr - (real)1) * (ur[E] - ker), (real)1e-12);
-    real al = sqrt((gl * pl) / rl), ar = sqrt((gr * pr) / rr);
+    real al = rsqrt_((gl * pl) * irl), ar = rsqrt_((gr * pr) * irr);
 
-    real sql = sqrt(rl), sqr = sqrt(rr);
-    real inv = (real)1 / (sql + sqr);
+    real sql = rsqrt_(rl), sqr = rsqrt_(rr);
+    real inv = rcp(sql + sqr);
     real u_roe = (sql * unl + sqr * unr) * inv;
     real kv = 0;
 #pragma unroll
@@ -155,14 +222,14 @@ __device__ __forceinline__ void hllc(const real* ul, const real* ur, real gl, re
         real v = (sql * utl[t] + sqr * utr[t]) * inv;
         kv = t == 0 ? v * v : kv + v * v;
     }
-    real hl = (ul[E] + pl) / rl, hr = (ur[E] + pr) / rr;
+    real hl = (ul[E] + pl) * irl, hr = (ur[E] + pr) * irr;
     real h_roe = (sql * hl + sqr * hr) * inv;
     real ke_roe = (real)0.5 * (u_roe * u_roe + kv);
-    real a_roe = sqrt(rmax((gbar - (real)1) * (h_roe - ke_roe), (real)1e-12));
+    real a_roe = rsqrt_(rmax((gbar - (real)1) * (h_roe - ke_roe), (real)1e-12));
     real s_l = nmin(unl - al, u_roe - a_roe);
     real s_r = nmax(unr + ar, u_roe + a_roe);
     real den = guard14(rl * (s_l - unl) - rr * (s_r - unr));
-    real s_star = (pr - pl + (rl * unl) * (s_l - unl) - (rr * unr) * (s_r - unr)) / den;
+    real s_star = dv(pr - pl + (rl * unl) * (s_l - unl) - (rr * unr) * (s_r - unr), den);
 
     // Final region after the ordered masks.
     int region;  // 0 L, 1 *L, 2 *R, 3 R, -1 none
@@ -192,14 +259,14 @@ __device__ __forceinline__ void hllc(const real* ul, const real* ur, real gl, re
         for (int c = 0; c < NV; ++c) out[c] = f[c];
         return;
     }
-    real fac = (s - un) / guard14(s - s_star);
+    real fac = dv(s - un, guard14(s - s_star));
     real rs = rmax(rho * fac, (real)1e-12);
     real us[NV];
     us[0] = rs;
     us[1] = rs * s_star;
 #pragma unroll
     for (int t = 0; t < NV - 3; ++t) us[2 + t] = rs * (left ? utl[t] : utr[t]);
-    us[E] = rs * (u[E] / rho + (s_star - un) * (s_star + p / (rho * guard14(s - un))));
+    us[E] = rs * (u[E] * (left ? irl : irr) + (s_star - un) * (s_star + dv(p, rho * guard14(s - un))));
 #pragma unroll
     for (int c = 0; c < NV; ++c) out[c] = f[c] + s * (us[c] - u[c]);
 }

@@ -124,4 +124,7 @@ namespace fvl {
 // Fold per-block partials: maxima are max-reduced, sums added in block order
 // (deterministic). Writes NSLOT doubles to out.
 hipError_t finalize(const double* partials, int nblocks, double* out, hipStream_t st);
+// The same, skipped on the device unless (*flags & mask) is set.
+hipError_t finalize_if(const double* partials, int nblocks, double* out, const int* flags, int mask,
+                       hipStream_t st);
 }
