@@ -33,6 +33,7 @@ extern "C" {
 #define TORCH_NO_STABLE_STEP 6    /* "Could not find stable reactive step" */
 #define TORCH_SPECIES_RANGE 7     /* "3D scalar transport left the physical interval" */
 #define TORCH_REACTION_STATE 8    /* "Reaction produced a nonphysical state" */
+#define TORCH_BUDGET 9            /* a batch instance does not fit the batch's HBM budget */
 
 /* Enumerations shared with the host mirror (torch_amd/_abi.py). */
 enum torch_limiter { TORCH_VANLEER = 0, TORCH_ALBADA = 1, TORCH_MINBEE = 2, TORCH_SUPERBEE = 3, TORCH_UNOS = 4 };
@@ -130,6 +131,58 @@ int torch_state_profile(torch_state* state, int enable, double* kernel_ms, long*
  * have m entries, out is (nv, m). */
 int torch_hllc_faces(torch_ctx* ctx, int precision, int nv, int direction, long m, const double* left,
                      const double* right, const double* gl, const double* gr, double* out);
+
+/* ---------------------------------------------------------------------------
+ * Batches. No counterpart in the reference, where every Simulation steps on
+ * its own (torchSession.py:374-522) and a batch of problems is a Python loop
+ * over sessions. Here many inert instances (any dimension, size, limiter,
+ * boundaries, gamma, CFL and end time) live on the device together and one
+ * torch_batch_run step advances all of them: a fixed sequence of work-list
+ * launches, captured once in a HIP graph and replayed. Each instance keeps
+ * the single-session arithmetic and its own dt, time, step count and
+ * boundary integral (the bookkeeping of Simulation.step, done on the
+ * device), so its trajectory is the one torch_state_step gives alone.
+ * Instances may be added and removed between runs (continuous batching);
+ * the batch caps its device memory at hbm_budget bytes (0: no cap).
+ * ------------------------------------------------------------------------- */
+typedef struct torch_batch torch_batch;
+
+typedef struct torch_batch_progress {
+    double time;         /* committed time */
+    double dt;           /* dt of the next step */
+    int steps;           /* committed steps */
+    int fallbacks;       /* steps that used the first-order fallback */
+    int status;          /* TORCH_OK, or the code that stopped the instance */
+    int active;          /* still stepping */
+    int completed;       /* time == t_end */
+    long cells;          /* cells of the instance */
+    double outward[5];   /* accumulated boundary integral */
+    double wave_max[3];  /* CFL maxima of the committed state */
+} torch_batch_progress;
+
+/* Device bytes an instance of this configuration takes in a batch. */
+int torch_batch_footprint(const torch_config* config, int precision, size_t* bytes);
+int torch_batch_create(torch_ctx* ctx, int precision, size_t hbm_budget, torch_batch** out);
+int torch_batch_destroy(torch_batch* batch);
+/* Admits an inert instance (Simulation(...) followed by run(t_end) with at
+ * most max_steps steps, torchSession.py:19-183, 563-600) with state U in the
+ * reference layout at `time`. TORCH_BUDGET when it does not fit now. */
+int torch_batch_add(torch_batch* batch, const torch_config* config, const double* U, double time, double t_end,
+                    int max_steps, int* id);
+int torch_batch_remove(torch_batch* batch, int id);
+/* Advances every active instance by up to `steps` steps, then waits. An
+ * instance stops at t_end, at max_steps, or on the error its status names
+ * (its committed state is then the last good one). */
+int torch_batch_run(torch_batch* batch, int steps);
+int torch_batch_info(torch_batch* batch, int id, torch_batch_progress* info);
+int torch_batch_download(torch_batch* batch, int id, double* U);
+/* Autotuner: times the 3D second-order work list over the batch's current
+ * instances for every supported tile shape and z chunk, keeps the fastest. */
+int torch_batch_tune(torch_batch* batch, int reps, int* block_x, int* block_y, int* zchunk, double* best_ms);
+int torch_batch_set_tiles(torch_batch* batch, int block_x, int block_y, int zchunk);
+/* Measurement hook: HIP events around the graph replays of each run. Returns
+ * the accumulated milliseconds and steps since the last call, then resets. */
+int torch_batch_profile(torch_batch* batch, int enable, double* ms, long* steps);
 
 #ifdef __cplusplus
 }

@@ -13,8 +13,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBRARY = os.path.join(HERE, "libtorchhip.so")
 
-OK, NONPHYSICAL, INVALID, HIP_ERROR, STALLED, NO_STABLE_STEP, SPECIES_RANGE, REACTION_STATE = (
-    0, 2, 3, 4, 5, 6, 7, 8)
+OK, NONPHYSICAL, INVALID, HIP_ERROR, STALLED, NO_STABLE_STEP, SPECIES_RANGE, REACTION_STATE, BUDGET = (
+    0, 2, 3, 4, 5, 6, 7, 8, 9)
 
 LIMITERS = ("VanLeer", "Albada", "MINBEE", "Superbee", "Unos")
 BOUNDARY_CODES = {"open": 0, "transmissive": 0, "outflow": 0, "periodic": 1, "wrap": 1,
@@ -29,6 +29,9 @@ EXPORTS = (
     "torch_state_destroy", "torch_state_upload", "torch_state_download", "torch_state_dt",
     "torch_state_step", "torch_state_hydro", "torch_state_fluxes", "torch_state_react",
     "torch_state_totals", "torch_state_profile", "torch_hllc_faces",
+    "torch_batch_footprint", "torch_batch_create", "torch_batch_destroy", "torch_batch_add",
+    "torch_batch_remove", "torch_batch_run", "torch_batch_info", "torch_batch_download",
+    "torch_batch_tune", "torch_batch_set_tiles", "torch_batch_profile",
 )
 
 
@@ -50,6 +53,15 @@ class StepInfo(ctypes.Structure):
     _fields_ = [
         ("dt", ctypes.c_double), ("fallback", ctypes.c_int), ("retries", ctypes.c_int),
         ("outward", ctypes.c_double * 5), ("source", ctypes.c_double * 5),
+        ("wave_max", ctypes.c_double * 3),
+    ]
+
+
+class BatchProgress(ctypes.Structure):
+    _fields_ = [
+        ("time", ctypes.c_double), ("dt", ctypes.c_double), ("steps", ctypes.c_int),
+        ("fallbacks", ctypes.c_int), ("status", ctypes.c_int), ("active", ctypes.c_int),
+        ("completed", ctypes.c_int), ("cells", ctypes.c_long), ("outward", ctypes.c_double * 5),
         ("wave_max", ctypes.c_double * 3),
     ]
 
@@ -92,6 +104,21 @@ def load():
                 "torch_state_profile": (ctypes.c_int, [vp, ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_long)]),
                 "torch_hllc_faces": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                                     ctypes.c_long, _dp, _dp, _dp, _dp, _dp]),
+                "torch_batch_footprint": (ctypes.c_int, [ctypes.POINTER(Config), ctypes.c_int,
+                                                         ctypes.POINTER(ctypes.c_size_t)]),
+                "torch_batch_create": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_size_t, ctypes.POINTER(vp)]),
+                "torch_batch_destroy": (ctypes.c_int, [vp]),
+                "torch_batch_add": (ctypes.c_int, [vp, ctypes.POINTER(Config), _dp, ctypes.c_double,
+                                                   ctypes.c_double, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
+                "torch_batch_remove": (ctypes.c_int, [vp, ctypes.c_int]),
+                "torch_batch_run": (ctypes.c_int, [vp, ctypes.c_int]),
+                "torch_batch_info": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(BatchProgress)]),
+                "torch_batch_download": (ctypes.c_int, [vp, ctypes.c_int, _dp]),
+                "torch_batch_tune": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                                    ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                                    _dp]),
+                "torch_batch_set_tiles": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+                "torch_batch_profile": (ctypes.c_int, [vp, ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_long)]),
             }
             for name, (res, args) in sig.items():
                 fn = getattr(lib, name)
@@ -251,3 +278,80 @@ def hllc_faces(context, nv, direction, left, right, gl, gr, precision=0):
     check(load().torch_hllc_faces(context.handle, int(precision), int(nv), int(direction), m, ptr(left),
                                   ptr(right), ptr(gl), ptr(gr), ptr(out)))
     return out
+
+
+class DeviceBatch:
+    """A device batch of inert instances (torch_batch_* in include/torch_hip.h)."""
+
+    def __init__(self, context, precision=1, hbm_budget=0):
+        self.context = context
+        self.precision = int(precision)
+        handle = ctypes.c_void_p()
+        check(load().torch_batch_create(context.handle, self.precision, int(hbm_budget), ctypes.byref(handle)))
+        self.handle = handle
+        self.shapes = {}
+
+    @staticmethod
+    def footprint(config, precision):
+        out = ctypes.c_size_t()
+        check(load().torch_batch_footprint(ctypes.byref(config), int(precision), ctypes.byref(out)))
+        return out.value
+
+    def add(self, config, state, time, t_end, max_steps):
+        """Admit an instance; returns its id, or None when the HBM budget is full."""
+        state = np.ascontiguousarray(state, dtype=np.float64)
+        counts = [config.n[d] for d in range(config.dim)]
+        shape = (config.dim + 2,) + tuple(counts[::-1])
+        if state.shape != shape:
+            raise ValueError(f"state must have shape {shape}")
+        out = ctypes.c_int()
+        code = load().torch_batch_add(self.handle, ctypes.byref(config), ptr(state), float(time), float(t_end),
+                                      int(max_steps), ctypes.byref(out))
+        if code == BUDGET:
+            return None
+        check(code)
+        self.shapes[out.value] = shape
+        return out.value
+
+    def remove(self, ident):
+        check(load().torch_batch_remove(self.handle, int(ident)))
+        self.shapes.pop(ident, None)
+
+    def run(self, steps):
+        check(load().torch_batch_run(self.handle, int(steps)))
+
+    def info(self, ident):
+        out = BatchProgress()
+        check(load().torch_batch_info(self.handle, int(ident), ctypes.byref(out)))
+        return out
+
+    def download(self, ident):
+        state = np.empty(self.shapes[ident])
+        check(load().torch_batch_download(self.handle, int(ident), ptr(state)))
+        return state
+
+    def tune(self, reps=3):
+        bx, by, zc, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        check(load().torch_batch_tune(self.handle, int(reps), ctypes.byref(bx), ctypes.byref(by),
+                                      ctypes.byref(zc), ctypes.byref(ms)))
+        return (bx.value, by.value, zc.value), ms.value
+
+    def set_tiles(self, block_x, block_y, zchunk):
+        check(load().torch_batch_set_tiles(self.handle, int(block_x), int(block_y), int(zchunk)))
+
+    def profile(self, enable=True):
+        """Enable replay timing; returns (ms, steps) since the last call."""
+        ms, n = ctypes.c_double(), ctypes.c_long()
+        check(load().torch_batch_profile(self.handle, int(enable), ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+    def close(self):
+        if getattr(self, "handle", None):
+            load().torch_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

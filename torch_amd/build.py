@@ -22,7 +22,8 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libtorchhip.so")
 ARCH = "gfx950"
 
-KERNELS = ("fv3d.hip", "fv2d.hip", "fv1d.hip")
+KERNELS = ("fv3d.hip", "fv2d.hip", "fv1d.hip", "fvbatch.hip")
+HOST = ("capi.hip", "batch.hip")
 COMMON = [
     "--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -58,8 +59,9 @@ def build(jobs=None, verbose=False):
         for tag, flags in PRECISION.items():
             obj = os.path.join(OBJ, f"{os.path.splitext(src)[0]}_{tag}.o")
             units.append([cc, *COMMON, *flags, "-c", os.path.join(CSRC, src), "-o", obj])
-    capi_obj = os.path.join(OBJ, "capi.o")
-    units.append([cc, *COMMON, "-c", os.path.join(CSRC, "capi.hip"), "-o", capi_obj])
+    for src in HOST:
+        obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
+        units.append([cc, *COMMON, "-c", os.path.join(CSRC, src), "-o", obj])
     jobs = jobs or min(len(units), os.cpu_count() or 4, 8)
     with concurrent.futures.ThreadPoolExecutor(max_workers=jobs) as pool:
         for out in pool.map(_run, units):

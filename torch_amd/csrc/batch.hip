@@ -1,0 +1,579 @@
+// Batched sessions (torch_batch_*): many inert instances of any dimension,
+// size, limiter and boundary advance together. Each step of the whole batch
+// is a fixed launch sequence (per dimension: second-order work list,
+// first-order work list gated per instance on the device, 1D CFL pass; then
+// one post block per instance) captured once in a HIP graph and replayed.
+// Every instance runs exactly the single-session arithmetic of
+// Simulation.step (python/cfdSolver/torchSession.py:374-522) with its own
+// dt, so its trajectory does not depend on what else is in the batch.
+//
+// Device layout: instances are ordered [3D | 2D | 1D]; BatchItem and
+// BatchCtl arrays follow that order; each work-list launch gets a block
+// table (first block of every instance of its segment). Partials of an
+// instance are [second-order | first-order | CFL] slots.
+#include <math.h>
+#include <stddef.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "capi_common.h"
+
+using namespace tcapi;
+
+namespace {
+
+constexpr int kFlagStride = 16;  // ints per instance flag word (one cache line each)
+
+struct Inst {
+    int id = 0;
+    torch_config cfg;
+    fvl::Grid grid;
+    long ncell = 0;
+    int nf = 0;
+    void* U[2] = {nullptr, nullptr};
+    void* scratch = nullptr;  // 1D interface fluxes
+    size_t bytes = 0;
+    double volume = 0;
+    fvl::BatchCtl ctl;  // host copy, current whenever no run is in flight
+    bool fresh = true;  // first dt not derived yet
+    // layout of the current build
+    int tiles[3] = {0, 0, 0};
+    int nblk = 0, nllf = 0, ncfl = 0;  // blocks of the three passes
+    long pslot = 0;                    // first partial slot
+};
+
+struct Table {
+    int offset = 0;  // into the concatenated int buffer
+    int count = 0;   // instances
+    int blocks = 0;  // sum of their blocks
+};
+
+size_t state_bytes(const torch_config& c, size_t elem) {
+    long ncell = (long)c.n[0] * (c.dim >= 2 ? c.n[1] : 1) * (c.dim >= 3 ? c.n[2] : 1);
+    size_t b = 2 * (size_t)(c.dim + 2) * ncell * elem;
+    if (c.dim == 1) b += elem * (size_t)(3 * (c.n[0] + 1) + 64);
+    // partial slots (default tiles) and bookkeeping
+    size_t slots = (size_t)(ncell / 256 + 8) * 3;
+    return b + slots * fvl::NSLOT * sizeof(double) + sizeof(fvl::BatchItem) + sizeof(fvl::BatchCtl);
+}
+
+}  // namespace
+
+struct torch_batch {
+    torch_ctx* ctx = nullptr;
+    const Ops* ops = nullptr;
+    int precision = 0;
+    size_t budget = 0, used = 0;
+    std::vector<Inst*> inst;
+    int next_id = 1;
+    bool dirty = true;
+    int seg[4] = {0, 0, 0, 0};  // [3D | 2D | 1D] segment starts, seg[3] = n
+    fvl::BatchItem* items = nullptr;
+    fvl::BatchCtl* ctl = nullptr;
+    int* flags = nullptr;
+    double* partials = nullptr;
+    int* tables = nullptr;
+    fvl::BatchCtl* host_ctl = nullptr;  // pinned
+    size_t cap_n = 0, cap_partials = 0, cap_tables = 0;
+    Table h3, l3, h2, l2, h1, c1;  // step tables
+    Table p3, p2, p1;              // prime tables (fresh suffix of each segment)
+    int fresh_first[3] = {0, 0, 0};
+    hipGraphExec_t graph = nullptr;
+    int bx = 64, by = 8, zchunk = 32;
+    bool profile = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double prof_ms = 0;
+    long prof_steps = 0;
+};
+
+namespace {
+
+int seg_of(int dim) { return dim == 3 ? 0 : (dim == 2 ? 1 : 2); }
+
+void free_inst(Inst* p) {
+    (void)hipFree(p->U[0]);
+    (void)hipFree(p->U[1]);
+    (void)hipFree(p->scratch);
+    delete p;
+}
+
+Inst* find(torch_batch* b, int id) {
+    for (Inst* p : b->inst)
+        if (p->id == id) return p;
+    return nullptr;
+}
+
+void drop_graph(torch_batch* b) {
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    b->graph = nullptr;
+}
+
+// Block counts of each pass for the current tile choice.
+void layout(torch_batch* b, Inst* p) {
+    const fvl::Grid& g = p->grid;
+    const Ops* o = b->ops;
+    if (g.dim == 3) {
+        p->tiles[0] = (g.nx + b->bx - 3) / (b->bx - 2);
+        p->tiles[1] = (g.ny + b->by - 3) / (b->by - 2);
+        p->tiles[2] = (g.nz + b->zchunk - 1) / b->zchunk;
+        p->nblk = p->tiles[0] * p->tiles[1] * p->tiles[2];
+        p->nllf = o->llf3d_blocks(g);
+        p->ncfl = o->cfl_blocks(g);
+    } else if (g.dim == 2) {
+        const int rows = o->strip2d_rows(g);
+        p->tiles[0] = (g.nx + 61) / 62;
+        p->tiles[1] = (g.ny + rows - 1) / rows;
+        p->tiles[2] = rows;
+        p->nblk = p->tiles[0] * p->tiles[1];
+        p->nllf = o->llf2d_blocks(g);
+        p->ncfl = o->cfl_blocks(g);
+    } else {
+        p->tiles[0] = o->step1d_tiles(g);
+        p->tiles[1] = p->tiles[2] = 1;
+        p->nblk = p->tiles[0];  // step and first-order tiles; no partials
+        p->nllf = p->tiles[0];
+        p->ncfl = o->cfl_blocks(g);
+    }
+}
+
+// Appends one block table (count+1 entries) over instances [a, e).
+Table table(std::vector<int>& buf, const std::vector<Inst*>& v, int a, int e, int Inst::*what) {
+    Table t;
+    t.offset = (int)buf.size();
+    t.count = e - a;
+    int acc = 0;
+    buf.push_back(0);
+    for (int i = a; i < e; ++i) {
+        acc += v[i]->*what;
+        buf.push_back(acc);
+    }
+    t.blocks = t.count > 0 ? acc : 0;
+    return t;
+}
+
+int grow(void** p, size_t* cap, size_t bytes) {
+    if (bytes <= *cap && *p) return TORCH_OK;
+    (void)hipFree(*p);
+    *p = nullptr;
+    *cap = 0;
+    size_t want = std::max(bytes, (size_t)4096);
+    want += want / 2;
+    int rc = alloc(p, want);
+    if (!rc) *cap = want;
+    return rc;
+}
+
+hipError_t enqueue_step(torch_batch* b, hipStream_t st) {
+    const Ops* o = b->ops;
+    const int* T = b->tables;
+    fvl::BatchItem* I3 = b->items + b->seg[0];
+    fvl::BatchItem* I2 = b->items + b->seg[1];
+    fvl::BatchItem* I1 = b->items + b->seg[2];
+    hipError_t e;
+    if ((e = o->batch3d(I3, T + b->h3.offset, b->h3.count, b->h3.blocks, b->bx, b->by, st))) return e;
+    if ((e = o->batch_llf3d(I3, T + b->l3.offset, b->l3.count, b->l3.blocks, st))) return e;
+    if ((e = o->batch2d(I2, T + b->h2.offset, b->h2.count, b->h2.blocks, st))) return e;
+    if ((e = o->batch_llf2d(I2, T + b->l2.offset, b->l2.count, b->l2.blocks, st))) return e;
+    if ((e = o->batch1d(I1, T + b->h1.offset, b->h1.count, b->h1.blocks, st))) return e;
+    if ((e = o->batch_llf1d(I1, T + b->h1.offset, b->h1.count, b->h1.blocks, st))) return e;
+    if ((e = o->batch_cfl1d(I1, T + b->c1.offset, b->c1.count, b->c1.blocks, 0, st))) return e;
+    return o->batch_post(b->items, b->ctl, b->seg[3], 0, st);
+}
+
+int capture(torch_batch* b) {
+    drop_graph(b);
+    if (b->seg[3] == 0) return TORCH_OK;
+    hipStream_t st = b->ctx->stream;
+    HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    hipError_t e = enqueue_step(b, st);
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(st, &g);
+    if (e != hipSuccess || e2 != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return fail(TORCH_HIP_ERROR, std::string("graph capture: ") + hipGetErrorString(e ? e : e2));
+    }
+    e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        b->graph = nullptr;
+        return fail(TORCH_HIP_ERROR, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+    return TORCH_OK;
+}
+
+int read_ctl(torch_batch* b) {
+    const int n = b->seg[3];
+    if (n == 0) return TORCH_OK;
+    hipStream_t st = b->ctx->stream;
+    HIP_TRY(hipMemcpyAsync(b->host_ctl, b->ctl, n * sizeof(fvl::BatchCtl), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    for (int i = 0; i < n; ++i) b->inst[i]->ctl = b->host_ctl[i];
+    return TORCH_OK;
+}
+
+// Device tables, items and control records for the current membership and
+// tile choice; derives the first dt of new instances; captures the graph.
+int rebuild(torch_batch* b) {
+    drop_graph(b);
+    std::vector<Inst*>& v = b->inst;
+    std::stable_sort(v.begin(), v.end(), [](const Inst* x, const Inst* y) {
+        int sx = seg_of(x->grid.dim), sy = seg_of(y->grid.dim);
+        if (sx != sy) return sx < sy;
+        return !x->fresh && y->fresh;  // new instances last within their segment
+    });
+    const int n = (int)v.size();
+    for (int s = 0; s < 3; ++s) {
+        int i = 0;
+        while (i < n && seg_of(v[i]->grid.dim) < s) ++i;
+        b->seg[s] = i;
+    }
+    b->seg[3] = n;
+    long slots = 0;
+    for (Inst* p : v) {
+        layout(b, p);
+        p->pslot = slots;
+        slots += (p->grid.dim == 1 ? 0 : p->nblk + p->nllf) + p->ncfl;
+    }
+    for (int s = 0; s < 3; ++s) {
+        int i = b->seg[s];
+        while (i < b->seg[s + 1] && !v[i]->fresh) ++i;
+        b->fresh_first[s] = i;
+    }
+    std::vector<int> tb;
+    b->h3 = table(tb, v, b->seg[0], b->seg[1], &Inst::nblk);
+    b->l3 = table(tb, v, b->seg[0], b->seg[1], &Inst::nllf);
+    b->h2 = table(tb, v, b->seg[1], b->seg[2], &Inst::nblk);
+    b->l2 = table(tb, v, b->seg[1], b->seg[2], &Inst::nllf);
+    b->h1 = table(tb, v, b->seg[2], b->seg[3], &Inst::nblk);
+    b->c1 = table(tb, v, b->seg[2], b->seg[3], &Inst::ncfl);
+    b->p3 = table(tb, v, b->fresh_first[0], b->seg[1], &Inst::ncfl);
+    b->p2 = table(tb, v, b->fresh_first[1], b->seg[2], &Inst::ncfl);
+    b->p1 = table(tb, v, b->fresh_first[2], b->seg[3], &Inst::ncfl);
+    if (n == 0) return TORCH_OK;
+
+    int rc = TORCH_OK;
+    if ((size_t)n > b->cap_n) {
+        size_t cap = std::max((size_t)n, b->cap_n * 2);
+        (void)hipFree(b->items);
+        (void)hipFree(b->ctl);
+        (void)hipFree(b->flags);
+        (void)hipHostFree(b->host_ctl);
+        b->items = nullptr;
+        b->ctl = nullptr;
+        b->flags = nullptr;
+        b->host_ctl = nullptr;
+        b->cap_n = 0;
+        rc = alloc((void**)&b->items, cap * sizeof(fvl::BatchItem));
+        if (!rc) rc = alloc((void**)&b->ctl, cap * sizeof(fvl::BatchCtl));
+        if (!rc) rc = alloc((void**)&b->flags, cap * kFlagStride * sizeof(int));
+        if (!rc && hipHostMalloc((void**)&b->host_ctl, cap * sizeof(fvl::BatchCtl)) != hipSuccess)
+            rc = fail(TORCH_HIP_ERROR, "hipHostMalloc failed");
+        if (rc) return rc;
+        b->cap_n = cap;
+    }
+    if ((rc = grow((void**)&b->partials, &b->cap_partials, (size_t)slots * fvl::NSLOT * sizeof(double))))
+        return rc;
+    if ((rc = grow((void**)&b->tables, &b->cap_tables, tb.size() * sizeof(int)))) return rc;
+
+    std::vector<fvl::BatchItem> items(n);
+    for (int i = 0; i < n; ++i) {
+        Inst* p = v[i];
+        fvl::BatchItem& it = items[i];
+        memset(&it, 0, sizeof(it));
+        fvl::Step& s = it.step;
+        s.U = p->U[p->ctl.cur];
+        s.Uout = p->U[p->ctl.cur ^ 1];
+        s.gamma = p->cfg.gamma;
+        s.flags = b->flags + (long)i * kFlagStride;
+        s.partials = b->partials + p->pslot * fvl::NSLOT;
+        s.zchunk = b->zchunk;
+        s.limiter = p->cfg.limiter;
+        s.want_cfl = p->grid.dim >= 2;
+        s.controller = p->cfg.controller;
+        s.variant = p->cfg.variant;
+        s.wavespeed = p->cfg.wavespeed;
+        s.cfl = p->cfg.cfl;
+        // Device address of this instance's step coefficients.
+        s.coef = (const double*)((const char*)(b->ctl + i) + offsetof(fvl::BatchCtl, coef));
+        it.grid = p->grid;
+        for (int d = 0; d < 3; ++d) it.tiles[d] = p->tiles[d];
+        it.llf_first = p->grid.dim == 1 ? 0 : p->nblk;
+        it.cfl_first = p->grid.dim == 1 ? 0 : p->nblk + p->nllf;
+        it.scratch = p->scratch;
+        p->ctl.nblk = p->nblk;
+        p->ctl.nllf = p->nllf;
+        p->ctl.ncfl = p->ncfl;
+        b->host_ctl[i] = p->ctl;
+    }
+    hipStream_t st = b->ctx->stream;
+    HIP_TRY(hipMemcpyAsync(b->items, items.data(), n * sizeof(fvl::BatchItem), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(b->ctl, b->host_ctl, n * sizeof(fvl::BatchCtl), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(b->tables, tb.data(), tb.size() * sizeof(int), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemsetAsync(b->flags, 0, n * kFlagStride * sizeof(int), st));
+    HIP_TRY(hipStreamSynchronize(st));
+
+    // First dt of new instances: CFL maxima of their initial states.
+    const Ops* o = b->ops;
+    const int* T = b->tables;
+    bool any_fresh = false;
+    for (int s = 0; s < 3; ++s) any_fresh |= b->fresh_first[s] < b->seg[s + 1];
+    if (any_fresh) {
+        HIP_TRY(o->batch_cfl3d(b->items + b->fresh_first[0], T + b->p3.offset, b->p3.count, b->p3.blocks, st));
+        HIP_TRY(o->batch_cfl2d(b->items + b->fresh_first[1], T + b->p2.offset, b->p2.count, b->p2.blocks, st));
+        HIP_TRY(o->batch_cfl1d(b->items + b->fresh_first[2], T + b->p1.offset, b->p1.count, b->p1.blocks, 1, st));
+        HIP_TRY(o->batch_post(b->items, b->ctl, n, 1, st));
+        if ((rc = read_ctl(b))) return rc;
+        for (Inst* p : v) p->fresh = false;
+    }
+    if ((rc = capture(b))) return rc;
+    b->dirty = false;
+    return TORCH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int torch_batch_footprint(const torch_config* config, int precision, size_t* bytes) {
+    if (!config || !bytes) return fail(TORCH_INVALID, "null argument");
+    int rc = check_config(*config);
+    if (rc) return rc;
+    *bytes = state_bytes(*config, precision == TORCH_FP32 ? sizeof(float) : sizeof(double));
+    return TORCH_OK;
+}
+
+int torch_batch_create(torch_ctx* ctx, int precision, size_t hbm_budget, torch_batch** out) {
+    if (!ctx || !out) return fail(TORCH_INVALID, "null argument");
+    if (precision != TORCH_FP64 && precision != TORCH_FP32) return fail(TORCH_INVALID, "unknown precision");
+    torch_batch* b = new torch_batch();
+    b->ctx = ctx;
+    b->precision = precision;
+    b->ops = precision == TORCH_FP64 ? &kOps64 : &kOps32;
+    b->budget = hbm_budget;
+    *out = b;
+    return TORCH_OK;
+}
+
+int torch_batch_destroy(torch_batch* b) {
+    if (!b) return TORCH_OK;
+    std::lock_guard<std::mutex> lock(b->ctx->mu);
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipStreamSynchronize(b->ctx->stream);
+    drop_graph(b);
+    for (Inst* p : b->inst) free_inst(p);
+    (void)hipFree(b->items);
+    (void)hipFree(b->ctl);
+    (void)hipFree(b->flags);
+    (void)hipFree(b->partials);
+    (void)hipFree(b->tables);
+    (void)hipHostFree(b->host_ctl);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    delete b;
+    return TORCH_OK;
+}
+
+int torch_batch_add(torch_batch* b, const torch_config* config, const double* U, double time, double t_end,
+                    int max_steps, int* id) {
+    if (!b || !config || !U || !id) return fail(TORCH_INVALID, "null argument");
+    const torch_config& c = *config;
+    int rc = check_config(c);
+    if (rc) return rc;
+    if (c.reactive || c.gamma_mode) return fail(TORCH_INVALID, "batches hold inert scalar-gamma instances");
+    if (!isfinite(time) || !isfinite(t_end) || !(time < t_end)) return fail(TORCH_INVALID, "require time < t_end");
+    if (max_steps <= 0) return fail(TORCH_INVALID, "max_steps must be positive");
+    const size_t bytes = state_bytes(c, b->ops->elem);
+    std::lock_guard<std::mutex> lock(b->ctx->mu);
+    if (b->budget && b->used + bytes > b->budget) return fail(TORCH_BUDGET, "instance does not fit the HBM budget");
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    Inst* p = new Inst();
+    p->cfg = c;
+    p->cfg.precision = b->precision;
+    make_grid(c, &p->grid, &p->volume);
+    p->ncell = (long)p->grid.nx * p->grid.ny * p->grid.nz;
+    p->nf = c.dim + 2;
+    p->bytes = bytes;
+    const size_t e = b->ops->elem;
+    rc = alloc(&p->U[0], p->nf * p->ncell * e);
+    if (!rc) rc = alloc(&p->U[1], p->nf * p->ncell * e);
+    if (!rc && c.dim == 1) rc = alloc(&p->scratch, b->ops->step1d_scratch(p->grid));
+    if (!rc) rc = upload_as(b->ops, b->ctx->stream, p->U[0], U, p->nf * p->ncell);
+    if (rc) {
+        free_inst(p);
+        return rc;
+    }
+    fvl::BatchCtl& k = p->ctl;
+    memset(&k, 0, sizeof(k));
+    k.t_end = t_end;
+    k.volume = p->volume;
+    k.cfl = c.cfl;
+    for (int d = 0; d < 3; ++d) k.h[d] = p->grid.h[d];
+    k.time = time;
+    k.dim = c.dim;
+    k.max_steps = max_steps;
+    k.active = 2;
+    p->id = b->next_id++;
+    b->inst.push_back(p);
+    b->used += bytes;
+    b->dirty = true;
+    *id = p->id;
+    return TORCH_OK;
+}
+
+int torch_batch_remove(torch_batch* b, int id) {
+    if (!b) return fail(TORCH_INVALID, "null argument");
+    std::lock_guard<std::mutex> lock(b->ctx->mu);
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    for (size_t i = 0; i < b->inst.size(); ++i)
+        if (b->inst[i]->id == id) {
+            HIP_TRY(hipStreamSynchronize(b->ctx->stream));
+            drop_graph(b);
+            b->used -= b->inst[i]->bytes;
+            free_inst(b->inst[i]);
+            b->inst.erase(b->inst.begin() + i);
+            b->dirty = true;
+            return TORCH_OK;
+        }
+    return fail(TORCH_INVALID, "no such batch instance");
+}
+
+int torch_batch_run(torch_batch* b, int steps) {
+    if (!b || steps < 0) return fail(TORCH_INVALID, "invalid argument");
+    std::lock_guard<std::mutex> lock(b->ctx->mu);
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    int rc;
+    if (b->dirty && (rc = rebuild(b))) return rc;
+    if (!b->graph || steps == 0) return TORCH_OK;
+    hipStream_t st = b->ctx->stream;
+    if (b->profile) HIP_TRY(hipEventRecord(b->ev0, st));
+    for (int k = 0; k < steps; ++k) HIP_TRY(hipGraphLaunch(b->graph, st));
+    if (b->profile) HIP_TRY(hipEventRecord(b->ev1, st));
+    if ((rc = read_ctl(b))) return rc;
+    if (b->profile) {
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+        b->prof_ms += ms;
+        b->prof_steps += steps;
+    }
+    return TORCH_OK;
+}
+
+int torch_batch_info(torch_batch* b, int id, torch_batch_progress* info) {
+    if (!b || !info) return fail(TORCH_INVALID, "null argument");
+    std::lock_guard<std::mutex> lock(b->ctx->mu);
+    Inst* p = find(b, id);
+    if (!p) return fail(TORCH_INVALID, "no such batch instance");
+    memset(info, 0, sizeof(*info));
+    const fvl::BatchCtl& k = p->ctl;
+    info->time = k.time;
+    info->dt = k.dt;
+    info->steps = k.steps;
+    info->fallbacks = k.fallbacks;
+    info->status = k.status;
+    info->active = k.active != 0;
+    info->completed = k.time == k.t_end;
+    info->cells = p->ncell;
+    for (int f = 0; f < 5; ++f) info->outward[f] = k.outward[f];
+    for (int m = 0; m < 3; ++m) info->wave_max[m] = k.maxima[m];
+    return TORCH_OK;
+}
+
+int torch_batch_download(torch_batch* b, int id, double* U) {
+    if (!b || !U) return fail(TORCH_INVALID, "null argument");
+    std::lock_guard<std::mutex> lock(b->ctx->mu);
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    Inst* p = find(b, id);
+    if (!p) return fail(TORCH_INVALID, "no such batch instance");
+    return download_as(b->ops, b->ctx->stream, U, p->U[p->ctl.cur], p->nf * p->ncell);
+}
+
+int torch_batch_tune(torch_batch* b, int reps, int* block_x, int* block_y, int* zchunk, double* best_ms) {
+    if (!b || reps <= 0) return fail(TORCH_INVALID, "invalid argument");
+    std::lock_guard<std::mutex> lock(b->ctx->mu);
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    int rc;
+    if (b->dirty && (rc = rebuild(b))) return rc;
+    hipStream_t st = b->ctx->stream;
+    double best = -1;
+    if (b->h3.count > 0) {
+        static const int shapes[4][2] = {{64, 4}, {64, 8}, {32, 8}, {32, 16}};
+        static const int chunks[3] = {16, 32, 64};
+        int bbx = b->bx, bby = b->by, bz = b->zchunk;
+        hipEvent_t e0, e1;
+        HIP_TRY(hipEventCreate(&e0));
+        HIP_TRY(hipEventCreate(&e1));
+        for (auto& sh : shapes)
+            for (int zc : chunks) {
+                b->bx = sh[0];
+                b->by = sh[1];
+                b->zchunk = zc;
+                if ((rc = rebuild(b))) break;
+                // The second-order pass alone, writing only the spare state
+                // buffers and partials: the committed states do not move.
+                const fvl::BatchItem* I3 = b->items + b->seg[0];
+                HIP_TRY(b->ops->batch3d(I3, b->tables + b->h3.offset, b->h3.count, b->h3.blocks, b->bx, b->by, st));
+                HIP_TRY(hipEventRecord(e0, st));
+                for (int r = 0; r < reps; ++r)
+                    HIP_TRY(b->ops->batch3d(I3, b->tables + b->h3.offset, b->h3.count, b->h3.blocks, b->bx, b->by,
+                                            st));
+                HIP_TRY(hipEventRecord(e1, st));
+                HIP_TRY(hipEventSynchronize(e1));
+                float ms = 0;
+                HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+                const double per = ms / reps;
+                if (best < 0 || per < best) {
+                    best = per;
+                    bbx = sh[0];
+                    bby = sh[1];
+                    bz = zc;
+                }
+            }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (rc) return rc;
+        b->bx = bbx;
+        b->by = bby;
+        b->zchunk = bz;
+        if ((rc = rebuild(b))) return rc;  // also clears the flags the trial passes raised
+    }
+    if (block_x) *block_x = b->bx;
+    if (block_y) *block_y = b->by;
+    if (zchunk) *zchunk = b->zchunk;
+    if (best_ms) *best_ms = best;
+    return TORCH_OK;
+}
+
+int torch_batch_set_tiles(torch_batch* b, int block_x, int block_y, int zchunk) {
+    if (!b) return fail(TORCH_INVALID, "null argument");
+    const bool ok = (block_x == 64 && (block_y == 4 || block_y == 8)) || (block_x == 32 && (block_y == 8 || block_y == 16));
+    if (!ok || zchunk < 1) return fail(TORCH_INVALID, "unsupported tile shape");
+    std::lock_guard<std::mutex> lock(b->ctx->mu);
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    HIP_TRY(hipStreamSynchronize(b->ctx->stream));
+    b->bx = block_x;
+    b->by = block_y;
+    b->zchunk = zchunk;
+    b->dirty = true;
+    drop_graph(b);
+    return TORCH_OK;
+}
+
+int torch_batch_profile(torch_batch* b, int enable, double* ms, long* steps) {
+    if (!b) return fail(TORCH_INVALID, "null argument");
+    std::lock_guard<std::mutex> lock(b->ctx->mu);
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    if (enable && !b->ev0) {
+        HIP_TRY(hipEventCreate(&b->ev0));
+        HIP_TRY(hipEventCreate(&b->ev1));
+    }
+    if (ms) *ms = b->prof_ms;
+    if (steps) *steps = b->prof_steps;
+    b->prof_ms = 0;
+    b->prof_steps = 0;
+    b->profile = enable != 0;
+    return TORCH_OK;
+}
+
+}  // extern "C"

@@ -11,10 +11,9 @@
 #include <string>
 #include <vector>
 
-#include "../../include/torch_hip.h"
-#include "fv_launch.h"
+#include "capi_common.h"
 
-namespace {
+namespace tcapi {
 
 thread_local std::string g_error;
 
@@ -23,44 +22,107 @@ int fail(int code, const std::string& msg) {
     return code;
 }
 
-#define HIP_TRY(expr)                                                                       \
-    do {                                                                                    \
-        hipError_t e_ = (expr);                                                             \
-        if (e_ != hipSuccess)                                                               \
-            return fail(TORCH_HIP_ERROR, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-    } while (0)
+int alloc(void** p, size_t bytes) {
+    hipError_t e = hipMalloc(p, bytes ? bytes : 16);
+    if (e != hipSuccess) return fail(TORCH_HIP_ERROR, std::string("hipMalloc: ") + hipGetErrorString(e));
+    return TORCH_OK;
+}
 
-// Kernel entry points of one precision build.
-struct Ops {
-    size_t elem;
-    int (*step3d_blocks)(const fvl::Grid&, const fvl::Launch&, int);
-    hipError_t (*step3d)(const fvl::Grid&, const fvl::Step&, const fvl::Launch&);
-    hipError_t (*llf3d)(const fvl::Grid&, const fvl::Step&, hipStream_t);
-    int (*llf3d_blocks)(const fvl::Grid&);
-    hipError_t (*step2d)(const fvl::Grid&, const fvl::Step&, const fvl::Launch&);
-    int (*step2d_blocks)(const fvl::Grid&, const fvl::Launch&);
-    hipError_t (*llf2d)(const fvl::Grid&, const fvl::Step&, hipStream_t);
-    int (*llf2d_blocks)(const fvl::Grid&);
-    hipError_t (*step1d)(const fvl::Grid&, const fvl::Step&, void*, hipStream_t);
-    size_t (*step1d_scratch)(const fvl::Grid&);
-    int (*step1d_blocks)(const fvl::Grid&);
-    hipError_t (*cfl_cells)(const fvl::Grid&, const void*, const void*, double, double*, int, hipStream_t);
-    int (*cfl_blocks)(const fvl::Grid&);
-    hipError_t (*react)(const fvl::Grid&, const fvl::React&, hipStream_t);
-    int (*react_blocks)(const fvl::Grid&);
-    hipError_t (*hllc_faces)(int, int, long, const void*, const void*, const void*, const void*, void*,
-                             hipStream_t);
-    hipError_t (*totals)(const fvl::Grid&, const void*, double*, hipStream_t);
-};
+int check_config(const torch_config& c) {
+    if (c.dim < 1 || c.dim > 3) return fail(TORCH_INVALID, "Invalid dimension");
+    for (int d = 0; d < c.dim; ++d) {
+        if (c.n[d] < 2) return fail(TORCH_INVALID, "Require cells >= 2");
+        if (!(c.spacing[d] > 0) && (!(c.lengths[d] > 0) || !isfinite(c.lengths[d])))
+            return fail(TORCH_INVALID, "Require length > 0");
+        if (c.bc[d] < 0 || c.bc[d] > 3 || (c.bc[d] == TORCH_INFLOW && !(c.dim == 2 && d == 1)))
+            return fail(TORCH_INVALID, "Unsupported session boundary");
+    }
+    if (c.limiter < 0 || c.limiter > 4) return fail(TORCH_INVALID, "Unknown limiter or chemistry model");
+    if (c.kinetic_model < 0 || c.kinetic_model > 1) return fail(TORCH_INVALID, "Unknown limiter or chemistry model");
+    if (!(c.gamma > 1) || !isfinite(c.gamma)) return fail(TORCH_INVALID, "gamma must be finite and greater than one");
+    if (!(c.cfl > 0) || !isfinite(c.cfl)) return fail(TORCH_INVALID, "CFL must be positive");
+    if (c.precision != TORCH_FP64 && c.precision != TORCH_FP32) return fail(TORCH_INVALID, "unknown precision");
+    if (c.gamma_mode < 0 || c.gamma_mode > 2 || (c.reactive && c.gamma_mode != 1))
+        return fail(TORCH_INVALID, "gamma_mode must be 1 for reactive states, 0 or 2 otherwise");
+    return TORCH_OK;
+}
+
+void make_grid(const torch_config& c, fvl::Grid* out, double* volume) {
+    fvl::Grid& g = *out;
+    memset(&g, 0, sizeof(g));
+    g.dim = c.dim;
+    g.wavespeed = c.wavespeed;
+    g.nx = c.n[0];
+    g.ny = c.dim >= 2 ? c.n[1] : 1;
+    g.nz = c.dim >= 3 ? c.n[2] : 1;
+    for (int d = 0; d < 3; ++d) {
+        g.bc[d] = d < c.dim ? c.bc[d] : TORCH_OPEN;
+        g.h[d] = d < c.dim ? (c.spacing[d] > 0 ? c.spacing[d] : c.lengths[d] / (double)c.n[d]) : 1.0;
+    }
+    // Volume as np.prod(spacing): ((hx*hy)*hz) over the active axes.
+    double v = g.h[0];
+    if (c.dim >= 2) v = v * g.h[1];
+    if (c.dim >= 3) v = v * g.h[2];
+    *volume = v;
+    if (c.dim == 2) {
+        // Inlet conservative state at gamma 1.4 with the reference's floors
+        // (torchSolver2D.py:61-82, 243).
+        const double* p = c.inlet_primitive;
+        double rho = fmax(p[0], 1e-12), pr = fmax(p[3], 1e-12);
+        double ke = 0.5 * rho * (p[1] * p[1] + p[2] * p[2]);
+        g.inlet[0] = rho;
+        g.inlet[1] = rho * p[1];
+        g.inlet[2] = rho * p[2];
+        g.inlet[3] = pr / (1.4 - 1.0) + ke;
+    }
+}
+
+int upload_as(const Ops* ops, hipStream_t st, void* dst, const double* src, long n) {
+    if (ops->elem == sizeof(double)) {
+        HIP_TRY(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyHostToDevice, st));
+    } else {
+        std::vector<float> tmp(n);
+        for (long i = 0; i < n; ++i) tmp[i] = (float)src[i];
+        HIP_TRY(hipMemcpyAsync(dst, tmp.data(), n * sizeof(float), hipMemcpyHostToDevice, st));
+        HIP_TRY(hipStreamSynchronize(st));
+    }
+    HIP_TRY(hipStreamSynchronize(st));
+    return TORCH_OK;
+}
+
+int download_as(const Ops* ops, hipStream_t st, double* dst, const void* src, long n) {
+    if (ops->elem == sizeof(double)) {
+        HIP_TRY(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+    } else {
+        std::vector<float> tmp(n);
+        HIP_TRY(hipMemcpyAsync(tmp.data(), src, n * sizeof(float), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        for (long i = 0; i < n; ++i) dst[i] = tmp[i];
+    }
+    return TORCH_OK;
+}
 
 #define OPS_OF(NS, T)                                                                                    \
-    Ops{sizeof(T),        NS::step3d_blocks, NS::step3d,     NS::llf3d,        NS::llf3d_blocks,          \
-        NS::step2d,       NS::step2d_blocks, NS::llf2d,      NS::llf2d_blocks,                            \
+    Ops{sizeof(T),        NS::step3d_blocks, NS::step3d,      NS::llf3d,        NS::llf3d_blocks,         \
+        NS::step2d,       NS::step2d_blocks, NS::llf2d,       NS::llf2d_blocks,                           \
         NS::step1d,       NS::step1d_scratch, NS::step1d_blocks,                                         \
-        NS::cfl_cells,    NS::cfl_blocks,    NS::react,      NS::react_blocks, NS::hllc_faces, NS::totals}
+        NS::cfl_cells,    NS::cfl_blocks,    NS::react,       NS::react_blocks, NS::hllc_faces, NS::totals, \
+        NS::convert_in,   NS::convert_out,                                                               \
+        NS::batch3d,      NS::batch_llf3d,   NS::batch_cfl3d, NS::batch2d,      NS::batch_llf2d,          \
+        NS::batch_cfl2d,  NS::batch1d,       NS::batch_llf1d, NS::batch_cfl1d,  NS::batch_post,           \
+        NS::strip2d_rows, NS::step1d_tiles}
 
-const Ops kOps64 = OPS_OF(fv64, double);
-const Ops kOps32 = OPS_OF(fv32, float);
+// Not const: a const table would be promoted to device memory, where the
+// host launchers it points to do not exist.
+Ops kOps64 = OPS_OF(fv64, double);
+Ops kOps32 = OPS_OF(fv32, float);
+
+}  // namespace tcapi
+
+using namespace tcapi;
+
+namespace {
 
 // Fold per-block partials in a fixed order (deterministic sums).
 __global__ void k_finalize(const double* partials, int nblocks, double* out, const int* flags, int mask) {
@@ -108,12 +170,6 @@ hipError_t finalize_if(const double* partials, int nblocks, double* out, const i
 }
 }  // namespace fvl
 
-struct torch_ctx {
-    int device;
-    hipStream_t stream;
-    std::mutex mu;
-};
-
 // Device buffers of one session. Slots of 4 state buffers rotate so the
 // committed state is never overwritten by a step that may still fail.
 struct torch_state {
@@ -150,11 +206,6 @@ long face_count(const fvl::Grid& g, int d) {
     return nx * ny * nz;
 }
 
-int alloc(void** p, size_t bytes) {
-    hipError_t e = hipMalloc(p, bytes ? bytes : 16);
-    if (e != hipSuccess) return fail(TORCH_HIP_ERROR, std::string("hipMalloc: ") + hipGetErrorString(e));
-    return TORCH_OK;
-}
 
 fvl::Launch launch_of(const torch_state* s) {
     fvl::Launch l;
@@ -410,21 +461,8 @@ int torch_state_destroy(torch_state* s) {
 int torch_state_create(torch_ctx* ctx, const torch_config* cfg, torch_state** out) {
     if (!ctx || !cfg || !out) return fail(TORCH_INVALID, "null argument");
     const torch_config& c = *cfg;
-    if (c.dim < 1 || c.dim > 3) return fail(TORCH_INVALID, "Invalid dimension");
-    for (int d = 0; d < c.dim; ++d) {
-        if (c.n[d] < 2) return fail(TORCH_INVALID, "Require cells >= 2");
-        if (!(c.spacing[d] > 0) && (!(c.lengths[d] > 0) || !isfinite(c.lengths[d])))
-            return fail(TORCH_INVALID, "Require length > 0");
-        if (c.bc[d] < 0 || c.bc[d] > 3 || (c.bc[d] == TORCH_INFLOW && !(c.dim == 2 && d == 1)))
-            return fail(TORCH_INVALID, "Unsupported session boundary");
-    }
-    if (c.limiter < 0 || c.limiter > 4) return fail(TORCH_INVALID, "Unknown limiter or chemistry model");
-    if (c.kinetic_model < 0 || c.kinetic_model > 1) return fail(TORCH_INVALID, "Unknown limiter or chemistry model");
-    if (!(c.gamma > 1) || !isfinite(c.gamma)) return fail(TORCH_INVALID, "gamma must be finite and greater than one");
-    if (!(c.cfl > 0) || !isfinite(c.cfl)) return fail(TORCH_INVALID, "CFL must be positive");
-    if (c.precision != TORCH_FP64 && c.precision != TORCH_FP32) return fail(TORCH_INVALID, "unknown precision");
-    if (c.gamma_mode < 0 || c.gamma_mode > 2 || (c.reactive && c.gamma_mode != 1))
-        return fail(TORCH_INVALID, "gamma_mode must be 1 for reactive states, 0 or 2 otherwise");
+    int rc0 = check_config(c);
+    if (rc0) return rc0;
 
     std::lock_guard<std::mutex> lock(ctx->mu);
     HIP_TRY(hipSetDevice(ctx->device));
@@ -438,31 +476,7 @@ int torch_state_create(torch_ctx* ctx, const torch_config* cfg, torch_state** ou
     }
     if (s->cfg.zchunk <= 0) s->cfg.zchunk = 32;
     fvl::Grid& g = s->grid;
-    memset(&g, 0, sizeof(g));
-    g.dim = c.dim;
-    g.wavespeed = c.wavespeed;
-    g.nx = c.n[0];
-    g.ny = c.dim >= 2 ? c.n[1] : 1;
-    g.nz = c.dim >= 3 ? c.n[2] : 1;
-    for (int d = 0; d < 3; ++d) {
-        g.bc[d] = d < c.dim ? c.bc[d] : TORCH_OPEN;
-        g.h[d] = d < c.dim ? (c.spacing[d] > 0 ? c.spacing[d] : c.lengths[d] / (double)c.n[d]) : 1.0;
-    }
-    // Volume as np.prod(spacing): ((hx*hy)*hz) over the active axes.
-    s->volume = g.h[0];
-    if (c.dim >= 2) s->volume = s->volume * g.h[1];
-    if (c.dim >= 3) s->volume = s->volume * g.h[2];
-    if (c.dim == 2) {
-        // Inlet conservative state at gamma 1.4 with the reference's floors
-        // (torchSolver2D.py:61-82, 243).
-        const double* p = c.inlet_primitive;
-        double rho = fmax(p[0], 1e-12), pr = fmax(p[3], 1e-12);
-        double ke = 0.5 * rho * (p[1] * p[1] + p[2] * p[2]);
-        g.inlet[0] = rho;
-        g.inlet[1] = rho * p[1];
-        g.inlet[2] = rho * p[2];
-        g.inlet[3] = pr / (1.4 - 1.0) + ke;
-    }
+    make_grid(c, &g, &s->volume);
     s->ncell = (long)g.nx * g.ny * g.nz;
     s->nf = c.dim + 2;
     const size_t e = s->ops->elem;
@@ -500,29 +514,11 @@ int torch_state_create(torch_ctx* ctx, const torch_config* cfg, torch_state** ou
 }
 
 static int upload_field(torch_state* s, void* dst, const double* src, long n) {
-    if (s->ops->elem == sizeof(double)) {
-        HIP_TRY(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyHostToDevice, s->ctx->stream));
-        HIP_TRY(hipStreamSynchronize(s->ctx->stream));
-    } else {
-        std::vector<float> tmp(n);
-        for (long i = 0; i < n; ++i) tmp[i] = (float)src[i];
-        HIP_TRY(hipMemcpyAsync(dst, tmp.data(), n * sizeof(float), hipMemcpyHostToDevice, s->ctx->stream));
-        HIP_TRY(hipStreamSynchronize(s->ctx->stream));
-    }
-    return TORCH_OK;
+    return upload_as(s->ops, s->ctx->stream, dst, src, n);
 }
 
 static int download_field(torch_state* s, double* dst, const void* src, long n) {
-    if (s->ops->elem == sizeof(double)) {
-        HIP_TRY(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToHost, s->ctx->stream));
-        HIP_TRY(hipStreamSynchronize(s->ctx->stream));
-    } else {
-        std::vector<float> tmp(n);
-        HIP_TRY(hipMemcpyAsync(tmp.data(), src, n * sizeof(float), hipMemcpyDeviceToHost, s->ctx->stream));
-        HIP_TRY(hipStreamSynchronize(s->ctx->stream));
-        for (long i = 0; i < n; ++i) dst[i] = tmp[i];
-    }
-    return TORCH_OK;
+    return download_as(s->ops, s->ctx->stream, dst, src, n);
 }
 
 int torch_state_upload(torch_state* s, const double* U, const double* lam, const double* beta) {

@@ -117,42 +117,18 @@ __device__ __forceinline__ real slope1(real delta, real dup, real ddown, real w)
     return dup * ddown > 0 ? sgn * nmin(fabs(sl), bound) : (real)0;
 }
 
-struct Scratch1 {
-    real* S;   // 3 x (n+1) wave speeds
-    real* SL;  // 3 x n left sub-cell states
-    real* SR;  // 3 x n right sub-cell states
-    real* F;   // 3 x (n+1) interface fluxes
-};
 
-static Scratch1 split(void* scratch, int n) {
-    real* p = (real*)scratch;
-    Scratch1 s;
-    s.S = p;
-    s.SL = s.S + 3 * (n + 1);
-    s.SR = s.SL + 3 * n;
-    s.F = s.SR + 3 * n;
-    return s;
-}
-
-__global__ void k_waves1(Step s, int n, int bc, Scratch1 w) {
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f > n) return;
-    real L[3], R[3], S[3];
-    cell1((const real*)s.U, n, bc, f, L);
-    cell1((const real*)s.U, n, bc, f + 1, R);
-    waves1(L, R, gamma1(s, n, bc, f), gamma1(s, n, bc, f + 1), s.wavespeed, S);
-    for (int c = 0; c < 3; ++c) w.S[c * (n + 1) + f] = S[c];
-}
-
+// Sub-cell states (left, right) of cell `src` from the cell averages:
+// limited central slope, half-step predictor, states that fail the
+// positivity test replaced by the cell average (torchSolver.py:447-491).
 template <int LIM>
-__global__ void k_recon1(Step s, int n, int bc, Scratch1 w) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__device__ __forceinline__ void recon1(const Step& s, int n, int bc, int src, real hc, real* sl, real* sr) {
+    const real* U = (const real*)s.U;
     real lo[3], in[3], hi[3];
-    cell1((const real*)s.U, n, bc, i, lo);
-    cell1((const real*)s.U, n, bc, i + 1, in);
-    cell1((const real*)s.U, n, bc, i + 2, hi);
-    const real g = gamma1(s, n, bc, i + 1);
+    cell1(U, n, bc, src, lo);
+    cell1(U, n, bc, src + 1, in);
+    cell1(U, n, bc, src + 2, hi);
+    const real g = gamma1(s, n, bc, src + 1);
     real wv = 0;
     if (s.controller == 0) {
         P1 q = prim1(in, g);
@@ -160,6 +136,7 @@ __global__ void k_recon1(Step s, int n, int bc, Scratch1 w) {
         wv = nmin(nmax(((real)1 - mach) / ((real)1 + mach), (real)-1), (real)1);
     }
     real ul[3], ur[3], fl[3], fr[3];
+#pragma unroll
     for (int c = 0; c < 3; ++c) {
         real d = slope1<LIM>((real)0.5 * (hi[c] - lo[c]), in[c] - lo[c], hi[c] - in[c], wv);
         ul[c] = in[c] - (real)0.5 * d;
@@ -168,15 +145,18 @@ __global__ void k_recon1(Step s, int n, int bc, Scratch1 w) {
     eflux1(ur, g, fr);
     eflux1(ul, g, fl);
     // -(dt / (2 dx)); 0.5*dt/dx rounds to the same double.
-    const real k = -(real)s.hc[0];
+    const real k = -hc;
+#pragma unroll
     for (int side = 0; side < 2; ++side) {
         real st[3];
+#pragma unroll
         for (int c = 0; c < 3; ++c) st[c] = (side == 0 ? ul[c] : ur[c]) + k * (fr[c] - fl[c]);
         real p = (g - (real)1) * (st[2] - ((real)0.5 * (st[1] * st[1])) / rmax(st[0], (real)1e-12));
         bool bad = st[0] <= (real)1e-12 || p <= (real)1e-12 || !isfinite(st[0]) || !isfinite(st[1]) ||
                    !isfinite(st[2]);
-        real* dst = side == 0 ? w.SL : w.SR;
-        for (int c = 0; c < 3; ++c) dst[c * n + i] = bad ? in[c] : st[c];
+        real* dst = side == 0 ? sl : sr;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) dst[c] = bad ? in[c] : st[c];
     }
 }
 
@@ -230,90 +210,120 @@ __device__ void hllc1(const real* ul, const real* ur, real gl, real gr, const re
     }
 }
 
-__global__ void k_flux1(Step s, int n, int bc, Scratch1 w) {
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f > n) return;
-    real ul[3], ur[3], S[3], F[3];
-    // Sub-cell ghosts (bc_creator applied to the sub-cell array).
-    if (f == 0) {
-        const bool wrap = bc == BC_PERIODIC;
-        for (int c = 0; c < 3; ++c) ul[c] = wrap ? w.SR[c * n + n - 1] : w.SL[c * n];
-        if (bc == BC_REFLECTIVE) ul[1] = -ul[1];
-    } else {
-        for (int c = 0; c < 3; ++c) ul[c] = w.SR[c * n + f - 1];
+// One block of 256 threads updates T1 cells [i0, i0+T1): thread t builds the
+// sub-cell pair of padded cell i0-1+t, then the interface flux of face i0+t,
+// then the update of cell i0+t. The interface fluxes also go to the
+// instance's flux array (3 x (n+1)), read afterwards by the species and
+// boundary-integral passes.
+constexpr int T1 = 254;
+
+struct Lds1 {
+    real sl[3][256], sr[3][256], f[3][256];
+};
+
+__device__ __forceinline__ void store_face(const Step& s, real* Fs, int n, int f, const real* F, Lds1& m, int t) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        m.f[c][t] = F[c];
+        Fs[c * (n + 1) + f] = F[c];
     }
-    if (f == n) {
-        const bool wrap = bc == BC_PERIODIC;
-        for (int c = 0; c < 3; ++c) ur[c] = wrap ? w.SL[c * n] : w.SR[c * n + n - 1];
-        if (bc == BC_REFLECTIVE) ur[1] = -ur[1];
-    } else {
-        for (int c = 0; c < 3; ++c) ur[c] = w.SL[c * n + f];
-    }
-    for (int c = 0; c < 3; ++c) S[c] = w.S[c * (n + 1) + f];
-    hllc1(ul, ur, gamma1(s, n, bc, f), gamma1(s, n, bc, f + 1), S, s.variant, F);
-    for (int c = 0; c < 3; ++c) w.F[c * (n + 1) + f] = F[c];
+    if (s.F[0])
+        for (int c = 0; c < 3; ++c) ((real*)s.F[0])[c * (n + 1) + f] = F[c];
 }
 
-// LLF interface flux on cell averages, run only after a failed update.
-__global__ void k_llf_flux1(Step s, int n, int bc, Scratch1 w) {
-    if (!(*s.flags & fvl::F_UPD_BAD)) return;
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f > n) return;
-    real L[3], R[3], fl[3], fr[3];
-    cell1((const real*)s.U, n, bc, f, L);
-    cell1((const real*)s.U, n, bc, f + 1, R);
-    real gl = gamma1(s, n, bc, f), gr = gamma1(s, n, bc, f + 1);
-    P1 a = prim1(L, gl), b = prim1(R, gr);
-    real al = nmax(fabs(a.v) + sound1(a, gl), fabs(b.v) + sound1(b, gr));
-    eflux1(L, gl, fl);
-    eflux1(R, gr, fr);
-    for (int c = 0; c < 3; ++c) w.F[c * (n + 1) + f] = (real)0.5 * (fl[c] + fr[c] - al * (R[c] - L[c]));
-}
-
-// Conservative update; `second` selects the pass (second order or LLF).
-__global__ void k_update1(Step s, int n, int bc, Scratch1 w, int second) {
-    const int flag_in = *s.flags;
-    if (!second && !(flag_in & fvl::F_UPD_BAD)) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+// Conservative update of cell i0+t from the block's face fluxes; `pass`
+// names the flag a nonphysical result raises (conservative_update_1d,
+// torchSolver.py:1054-1071).
+__device__ __forceinline__ void update1(const Step& s, int n, int bc, int i0, real c0, const Lds1& m, int pass) {
+    const int t = threadIdx.x, i = i0 + t;
+    if (t >= T1 || i >= n) return;
     real in[3], o[3];
     cell1((const real*)s.U, n, bc, i + 1, in);
-    const real k = -(real)s.c[0];
-    for (int c = 0; c < 3; ++c) o[c] = in[c] + k * (w.F[c * (n + 1) + i + 1] - w.F[c * (n + 1) + i]);
+    const real k = -c0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) o[c] = in[c] + k * (m.f[c][t + 1] - m.f[c][t]);
+#pragma unroll
     for (int c = 0; c < 3; ++c) ((real*)s.Uout)[c * (long)n + i] = o[c];
     bool ok = isfinite(o[0]) && isfinite(o[1]) && isfinite(o[2]) && o[0] > (real)1e-12 &&
               o[2] - ((real)0.5 * (o[1] * o[1])) / o[0] > (real)1e-12;
-    if (!ok) atomicOr(s.flags, second ? fvl::F_UPD_BAD : fvl::F_LLF_BAD);
+    if (!ok) atomicOr(s.flags, pass);
 }
 
-__global__ void k_species1(Step s, int n, int bc, Scratch1 w) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const real* U = (const real*)s.U;
-    const real rn = rmax(((const real*)s.Uout)[i], (real)1e-12);
-    for (int j = 0; j < s.nq; ++j) {
-        const real* q = (const real*)s.q[j];
-        real qm = q[ghost_src(i - 1, n, bc == BC_PERIODIC ? BC_PERIODIC : BC_OPEN)], qc = q[i];
-        real qp = q[ghost_src(i + 1, n, bc == BC_PERIODIC ? BC_PERIODIC : BC_OPEN)];
-        real f0 = w.F[i], f1 = w.F[i + 1];
-        real sl = f0 * (f0 >= 0 ? qm : qc), sh = f1 * (f1 >= 0 ? qc : qp);
-        real rq = U[i] * qc - (real)s.c[0] * (sh - sl);
-        ((real*)s.qout[j])[i] = rmin(rmax(rq / rn, (real)0), (real)1);
+template <int LIM>
+__device__ void step1d_tile(const Step& s, int n, int bc, int tile, real* Fs, Lds1& m) {
+    const fvl::Coef K = fvl::coef_of(s);
+    const int t = threadIdx.x, i0 = tile * T1, j = i0 - 1 + t;
+    if (j <= n) {
+        // Sub-cell ghosts are bc_creator applied to the sub-cell arrays:
+        // periodic wraps, open copies the adjacent sub-cell, reflective copies
+        // it with the momentum negated.
+        const bool wrap = bc == BC_PERIODIC, mirror = bc == BC_REFLECTIVE;
+        const int src = j < 0 ? (wrap ? n - 1 : 0) : (j >= n ? (wrap ? 0 : n - 1) : j);
+        real a[3], b[3];
+        recon1<LIM>(s, n, bc, src, (real)K.hc[0], a, b);
+        if (j < 0 && !wrap) {
+            for (int c = 0; c < 3; ++c) b[c] = a[c];
+            if (mirror) b[1] = -b[1];
+        }
+        if (j >= n && !wrap) {
+            for (int c = 0; c < 3; ++c) a[c] = b[c];
+            if (mirror) a[1] = -a[1];
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            m.sl[c][t] = a[c];
+            m.sr[c][t] = b[c];
+        }
     }
+    __syncthreads();
+    const int f = i0 + t;
+    if (t <= T1 && f <= n) {
+        const real* U = (const real*)s.U;
+        real ul[3], ur[3], L[3], R[3], S[3], F[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            ul[c] = m.sr[c][t];
+            ur[c] = m.sl[c][t + 1];
+        }
+        cell1(U, n, bc, f, L);
+        cell1(U, n, bc, f + 1, R);
+        const real gl = gamma1(s, n, bc, f), gr = gamma1(s, n, bc, f + 1);
+        waves1(L, R, gl, gr, s.wavespeed, S);
+        hllc1(ul, ur, gl, gr, S, s.variant, F);
+        store_face(s, Fs, n, f, F, m, t);
+    }
+    __syncthreads();
+    update1(s, n, bc, i0, (real)K.c[0], m, fvl::F_UPD_BAD);
 }
 
-__global__ void k_bound1(Step s, int n, Scratch1 w) {
-    if (threadIdx.x != 0) return;
-    double* p = s.partials;
-    for (int k = 0; k < fvl::NSLOT; ++k) p[k] = 0.0;
-    for (int c = 0; c < 3; ++c)
-        p[fvl::S_OUT + c] = s.w[0] * (double)(w.F[c * (n + 1) + n] - w.F[c * (n + 1)]);
+// LLF pass on cell averages, run only after a failed second-order update.
+__device__ void llf1d_tile(const Step& s, int n, int bc, int tile, real* Fs, Lds1& m) {
+    if (!(*s.flags & fvl::F_UPD_BAD)) return;  // the same answer for every thread of the block
+    const fvl::Coef K = fvl::coef_of(s);
+    const int t = threadIdx.x, i0 = tile * T1, f = i0 + t;
+    if (t <= T1 && f <= n) {
+        const real* U = (const real*)s.U;
+        real L[3], R[3], fl[3], fr[3], F[3];
+        cell1(U, n, bc, f, L);
+        cell1(U, n, bc, f + 1, R);
+        real gl = gamma1(s, n, bc, f), gr = gamma1(s, n, bc, f + 1);
+        P1 a = prim1(L, gl), b = prim1(R, gr);
+        real al = nmax(fabs(a.v) + sound1(a, gl), fabs(b.v) + sound1(b, gr));
+        eflux1(L, gl, fl);
+        eflux1(R, gr, fr);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) F[c] = (real)0.5 * (fl[c] + fr[c] - al * (R[c] - L[c]));
+        store_face(s, Fs, n, f, F, m, t);
+    }
+    __syncthreads();
+    update1(s, n, bc, i0, (real)K.c[0], m, fvl::F_LLF_BAD);
 }
 
-// CFL maxima: max |S| over faces (slot 0), max |u|+a over padded cells (slot 1).
-__global__ void k_cfl1(Step s, int n, int bc, double* partials) {
+// CFL maxima of faces [f0, f0+256): max |S| (slot 0) and max |u|+a over the
+// padded cells (slot 1).
+__device__ void cfl1d_faces(const Step& s, int n, int bc, int f0, double* partials, long pblk) {
     __shared__ double red[4][2];
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    const int f = f0 + threadIdx.x;
     double m0 = 0, m1 = 0;
     if (f <= n) {
         real L[3], R[3], S[3];
@@ -337,7 +347,7 @@ __global__ void k_cfl1(Step s, int n, int bc, double* partials) {
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        double* p = partials + blockIdx.x * (long)fvl::NSLOT;
+        double* p = partials + pblk * fvl::NSLOT;
         for (int k = 0; k < fvl::NSLOT; ++k) p[k] = 0.0;
         for (int k = 0; k < 4; ++k) {
             p[0] = fmax(p[0], red[k][0]);
@@ -346,33 +356,148 @@ __global__ void k_cfl1(Step s, int n, int bc, double* partials) {
     }
 }
 
-size_t step1d_scratch(const Grid& g) { return sizeof(real) * (size_t)(3 * (g.nx + 1) * 2 + 6 * g.nx + 64); }
+template <int LIM>
+__global__ void __launch_bounds__(256) k_step1d(Step s, int n, int bc, real* Fs) {
+    __shared__ Lds1 m;
+    step1d_tile<LIM>(s, n, bc, blockIdx.x, Fs, m);
+}
+
+__global__ void __launch_bounds__(256) k_llf1d(Step s, int n, int bc, real* Fs) {
+    __shared__ Lds1 m;
+    llf1d_tile(s, n, bc, blockIdx.x, Fs, m);
+}
+
+__global__ void __launch_bounds__(256) k_cfl1(Step s, int n, int bc, double* partials) {
+    cfl1d_faces(s, n, bc, blockIdx.x * 256, partials, blockIdx.x);
+}
+
+__global__ void k_species1(Step s, int n, int bc, const real* Fs) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const fvl::Coef K = fvl::coef_of(s);
+    const real* U = (const real*)s.U;
+    const real rn = rmax(((const real*)s.Uout)[i], (real)1e-12);
+    const int sbc = bc == BC_PERIODIC ? BC_PERIODIC : BC_OPEN;  // scalar_bc has no mirror
+    for (int j = 0; j < s.nq; ++j) {
+        const real* q = (const real*)s.q[j];
+        real qm = q[ghost_src(i - 1, n, sbc)], qc = q[i], qp = q[ghost_src(i + 1, n, sbc)];
+        real f0 = Fs[i], f1 = Fs[i + 1];
+        real sl = f0 * (f0 >= 0 ? qm : qc), sh = f1 * (f1 >= 0 ? qc : qp);
+        real rq = U[i] * qc - (real)K.c[0] * (sh - sl);
+        ((real*)s.qout[j])[i] = rmin(rmax(rq / rn, (real)0), (real)1);
+    }
+}
+
+// Boundary integral dt*vol/dx * (F_last - F_first) of the final fluxes.
+__device__ __forceinline__ void bound1(double w, const real* Fs, int n, double* out) {
+    for (int c = 0; c < 3; ++c) out[c] = w * (double)(Fs[c * (n + 1) + n] - Fs[c * (n + 1)]);
+}
+
+__global__ void k_bound1(Step s, int n, const real* Fs) {
+    if (threadIdx.x != 0) return;
+    const fvl::Coef K = fvl::coef_of(s);
+    double* p = s.partials;
+    for (int k = 0; k < fvl::NSLOT; ++k) p[k] = 0.0;
+    bound1(K.w[0], Fs, n, p + fvl::S_OUT);
+}
+
+// Work-list forms: instance i owns blocks first[i] .. first[i+1]-1, one
+// tile of T1 cells (or 256 CFL faces) per block.
+__device__ __forceinline__ int owner1(const int* first, int n, int b) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) / 2;
+        if (first[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ void __launch_bounds__(256) k_batch1d(const fvl::BatchItem* items, const int* first, int n) {
+    __shared__ Lds1 m;
+    const int i = owner1(first, n, blockIdx.x);
+    const fvl::BatchItem& it = items[i];
+    const Step& s = it.step;
+    if (s.coef && s.coef[9] == 0.0) return;
+    const int local = blockIdx.x - first[i], nx = it.grid.nx, bc = it.grid.bc[0];
+    real* Fs = (real*)it.scratch;
+    switch (s.limiter) {
+        case VANLEER: step1d_tile<VANLEER>(s, nx, bc, local, Fs, m); break;
+        case ALBADA: step1d_tile<ALBADA>(s, nx, bc, local, Fs, m); break;
+        case MINBEE: step1d_tile<MINBEE>(s, nx, bc, local, Fs, m); break;
+        case SUPERBEE: step1d_tile<SUPERBEE>(s, nx, bc, local, Fs, m); break;
+        default: step1d_tile<UNOS>(s, nx, bc, local, Fs, m); break;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_batch_llf1d(const fvl::BatchItem* items, const int* first, int n) {
+    __shared__ Lds1 m;
+    const int i = owner1(first, n, blockIdx.x);
+    const fvl::BatchItem& it = items[i];
+    if (it.step.coef && it.step.coef[9] == 0.0) return;
+    llf1d_tile(it.step, it.grid.nx, it.grid.bc[0], blockIdx.x - first[i], (real*)it.scratch, m);
+}
+
+// CFL maxima of each 1D instance's output state (after the step) or, with
+// `input` set, of its input state (the first step of a new instance).
+__global__ void __launch_bounds__(256) k_batch_cfl1d(const fvl::BatchItem* items, const int* first, int n,
+                                                     int input) {
+    const int i = owner1(first, n, blockIdx.x);
+    const fvl::BatchItem& it = items[i];
+    if (!input && it.step.coef && it.step.coef[9] == 0.0) return;
+    Step s = it.step;
+    if (!input) s.U = s.Uout;
+    const int local = blockIdx.x - first[i];
+    cfl1d_faces(s, it.grid.nx, it.grid.bc[0], local * 256, s.partials, it.cfl_first + local);
+}
+
+size_t step1d_scratch(const Grid& g) { return sizeof(real) * (size_t)(3 * (g.nx + 1) + 64); }
 int step1d_blocks(const Grid&) { return 1; }
+int step1d_tiles(const Grid& g) { return (g.nx + T1 - 1) / T1; }
 int cfl1d_blocks(const Grid& g) { return (g.nx + 1 + 255) / 256; }
+
+template <int LIM>
+static void launch1(const Grid& g, const Step& s, real* Fs, hipStream_t st) {
+    k_step1d<LIM><<<step1d_tiles(g), 256, 0, st>>>(s, g.nx, g.bc[0], Fs);
+}
 
 hipError_t step1d(const Grid& g, const Step& s, void* scratch, hipStream_t st) {
     const int n = g.nx, bc = g.bc[0];
-    Scratch1 w = split(scratch, n);
-    const unsigned bf = (n + 1 + 255) / 256, bcn = (n + 255) / 256;
-    k_waves1<<<bf, 256, 0, st>>>(s, n, bc, w);
+    real* Fs = (real*)scratch;
     switch (s.limiter) {
-        case VANLEER: k_recon1<VANLEER><<<bcn, 256, 0, st>>>(s, n, bc, w); break;
-        case ALBADA: k_recon1<ALBADA><<<bcn, 256, 0, st>>>(s, n, bc, w); break;
-        case MINBEE: k_recon1<MINBEE><<<bcn, 256, 0, st>>>(s, n, bc, w); break;
-        case SUPERBEE: k_recon1<SUPERBEE><<<bcn, 256, 0, st>>>(s, n, bc, w); break;
-        default: k_recon1<UNOS><<<bcn, 256, 0, st>>>(s, n, bc, w); break;
+        case VANLEER: launch1<VANLEER>(g, s, Fs, st); break;
+        case ALBADA: launch1<ALBADA>(g, s, Fs, st); break;
+        case MINBEE: launch1<MINBEE>(g, s, Fs, st); break;
+        case SUPERBEE: launch1<SUPERBEE>(g, s, Fs, st); break;
+        default: launch1<UNOS>(g, s, Fs, st); break;
     }
-    k_flux1<<<bf, 256, 0, st>>>(s, n, bc, w);
-    k_update1<<<bcn, 256, 0, st>>>(s, n, bc, w, 1);
-    k_llf_flux1<<<bf, 256, 0, st>>>(s, n, bc, w);
-    k_update1<<<bcn, 256, 0, st>>>(s, n, bc, w, 0);
-    if (s.nq > 0) k_species1<<<bcn, 256, 0, st>>>(s, n, bc, w);
-    k_bound1<<<1, 64, 0, st>>>(s, n, w);
+    k_llf1d<<<step1d_tiles(g), 256, 0, st>>>(s, n, bc, Fs);
+    if (s.nq > 0) k_species1<<<(n + 255) / 256, 256, 0, st>>>(s, n, bc, Fs);
+    k_bound1<<<1, 64, 0, st>>>(s, n, Fs);
     return hipGetLastError();
 }
 
 hipError_t cfl1d(const Grid& g, const Step& s, hipStream_t st) {
     k_cfl1<<<cfl1d_blocks(g), 256, 0, st>>>(s, g.nx, g.bc[0], s.partials);
+    return hipGetLastError();
+}
+
+hipError_t batch1d(const fvl::BatchItem* items, const int* first, int n, int blocks, hipStream_t st) {
+    if (blocks <= 0) return hipSuccess;
+    k_batch1d<<<blocks, 256, 0, st>>>(items, first, n);
+    return hipGetLastError();
+}
+
+hipError_t batch_llf1d(const fvl::BatchItem* items, const int* first, int n, int blocks, hipStream_t st) {
+    if (blocks <= 0) return hipSuccess;
+    k_batch_llf1d<<<blocks, 256, 0, st>>>(items, first, n);
+    return hipGetLastError();
+}
+
+hipError_t batch_cfl1d(const fvl::BatchItem* items, const int* first, int n, int blocks, int input,
+                       hipStream_t st) {
+    if (blocks <= 0) return hipSuccess;
+    k_batch_cfl1d<<<blocks, 256, 0, st>>>(items, first, n, input);
     return hipGetLastError();
 }
 

@@ -66,7 +66,8 @@ __device__ __forceinline__ void flux2f(const real* u, real g, int dir, real* f) 
 }
 
 template <int NT>
-__device__ void partials2(double (&acc)[5], double (&mx)[3], double* partials, int* flags, int bad) {
+__device__ void partials2(double (&acc)[5], double (&mx)[3], double* partials, int* flags, int bad,
+                          long block = -1) {
     __shared__ double red[NT / 64][8];
     __shared__ int sbad;
     const int tid = threadIdx.x;
@@ -80,7 +81,7 @@ __device__ void partials2(double (&acc)[5], double (&mx)[3], double* partials, i
         for (int i = 0; i < 8; ++i) red[tid / 64][i] = v[i];
     __syncthreads();
     if (tid == 0) {
-        double* p = partials + (blockIdx.x + (long)gridDim.x * blockIdx.y) * fvl::NSLOT;
+        double* p = partials + (block >= 0 ? block : blockIdx.x + (long)gridDim.x * blockIdx.y) * fvl::NSLOT;
         for (int i = 0; i < 8; ++i) {
             double r = red[0][i];
             for (int k = 1; k < NT / 64; ++k) r = i < 3 ? fmax(r, red[k][i]) : r + red[k][i];
@@ -91,20 +92,26 @@ __device__ void partials2(double (&acc)[5], double (&mx)[3], double* partials, i
     }
 }
 
+// |u|+a and |v|+a of a 2D state with the reference's floors (the CFL rule of
+// compute_cfl_dt_2d, torchSolver2D.py:512-530), accumulated as maxima.
+__device__ __forceinline__ void wave2(const real* u, real g, double (&mx)[3]);
+
+// One 62-column strip (tbx) of one row chunk (tby); partial slot pblk.
 template <int LIM, bool SPEC>
-__global__ void __launch_bounds__(64) k_step2d(Step s, G2 g, int ychunk) {
+__device__ __forceinline__ void step2d_strip(const Step& s, const G2& g, int ychunk, int tbx, int tby, long pblk) {
     constexpr int TX = 62;
     const int lane = threadIdx.x, lx = lane - 1;
-    const int x = blockIdx.x * TX + lx;
+    const int x = tbx * TX + lx;
     const int nx = g.nx, ny = g.ny;
     const long N = (long)nx * ny;
-    const int y0 = blockIdx.y * ychunk, y1 = min(y0 + ychunk, ny);
+    const int y0 = tby * ychunk, y1 = min(y0 + ychunk, ny);
     const real* __restrict__ U = (const real*)s.U;
     real* __restrict__ Uo = (real*)s.Uout;
     const int nq = SPEC ? s.nq : 0;
     const bool owns = lx >= 0 && lx < TX && x < nx;
     const bool xface = lx >= 0 && x <= nx;
-    const real c0 = (real)s.c[0], c1 = (real)s.c[1], h0 = (real)s.hc[0], h1 = (real)s.hc[1];
+    const fvl::Coef K = fvl::coef_of(s);
+    const real c0 = (real)K.c[0], c1 = (real)K.c[1], h0 = (real)K.hc[0], h1 = (real)K.hc[1];
 
     real Ap[4], Ac[4], An[4], gc, gn;
     real qc[2] = {0, 0}, qp[2] = {0, 0}, qn[2] = {0, 0};
@@ -197,6 +204,7 @@ __global__ void __launch_bounds__(64) k_step2d(Step s, G2 g, int ychunk) {
 #pragma unroll
             for (int c = 0; c < 4; ++c) Uo[c * N + i] = o[c];
             if (!physical4(o)) bad |= fvl::F_UPD_BAD;
+            if (s.want_cfl) wave2(o, gp, mx);  // gp: gamma of row k-1
             for (int j = 0; j < nq; ++j) {
                 real v = rq[j] - c1 * (sfy[j] - sfyp[j]);
                 ((real*)s.qout[j])[i] = rmin(rmax(v / rmax(o[0], (real)1e-12), (real)0), (real)1);
@@ -209,7 +217,7 @@ __global__ void __launch_bounds__(64) k_step2d(Step s, G2 g, int ychunk) {
         }
         if (row_out && xface) {
             if (x == 0 || x == nx) {
-                double sg = x == 0 ? -s.w[0] : s.w[0];
+                double sg = x == 0 ? -K.w[0] : K.w[0];
                 for (int c = 0; c < 4; ++c) acc[c] += sg * (double)Fx[c];
             }
             if (s.F[0]) {
@@ -219,7 +227,7 @@ __global__ void __launch_bounds__(64) k_step2d(Step s, G2 g, int ychunk) {
         }
         if (yface) {
             if (k == 0 || k == ny) {
-                double sg = k == 0 ? -s.w[1] : s.w[1];
+                double sg = k == 0 ? -K.w[1] : K.w[1];
                 for (int c = 0; c < 4; ++c) acc[c] += sg * (double)Fy[c];
             }
             if (s.F[1]) {
@@ -242,23 +250,61 @@ __global__ void __launch_bounds__(64) k_step2d(Step s, G2 g, int ychunk) {
             sfyp[j] = sfy[j];
         }
     }
-    partials2<64>(acc, mx, s.partials, s.flags, bad);
+    partials2<64>(acc, mx, s.partials, s.flags, bad, pblk);
 }
 
-// Floored primitives, flux and |u|+a of a padded 2D cell (torchSolver2D.py:
-// 38-58, 85-91, 420-447).
-struct Prim2 {
-    real rho, ux, vy, p, a;
-};
-__device__ __forceinline__ Prim2 prim2(const real* u, real g) {
-    Prim2 q;
-    q.rho = rmax(u[0], (real)1e-12);
-    q.ux = u[1] / q.rho;
-    q.vy = u[2] / q.rho;
-    real ke = ((real)0.5 * q.rho) * (q.ux * q.ux + q.vy * q.vy);
-    q.p = rmax((g - (real)1) * (u[3] - ke), (real)1e-12);
-    q.a = sqrt((g * rmax(q.p, (real)1e-12)) / rmax(q.rho, (real)1e-12));
+template <int LIM, bool SPEC>
+__global__ void __launch_bounds__(64) k_step2d(Step s, G2 g, int ychunk) {
+    step2d_strip<LIM, SPEC>(s, g, ychunk, blockIdx.x, blockIdx.y, blockIdx.x + (long)gridDim.x * blockIdx.y);
+}
+
+__device__ __forceinline__ G2 g2_of(const Grid& g) {
+    G2 q;
+    q.nx = g.nx;
+    q.ny = g.ny;
+    q.bcx = g.bc[0];
+    q.bcy = g.bc[1];
+    for (int c = 0; c < 4; ++c) q.inlet[c] = (real)g.inlet[c];
     return q;
+}
+
+__device__ __forceinline__ int owner_of(const int* first, int n, int b) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) / 2;
+        if (first[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Work-list form over 2D instances (strips x row chunks per instance).
+__global__ void __launch_bounds__(64) k_batch2d(const fvl::BatchItem* items, const int* first, int n) {
+    const int i = owner_of(first, n, blockIdx.x);
+    const fvl::BatchItem& it = items[i];
+    const int local = blockIdx.x - first[i];
+    const G2 g = g2_of(it.grid);
+    const Step& s = it.step;
+    if (s.coef && s.coef[9] == 0.0) {
+        double acc[5] = {0, 0, 0, 0, 0}, mx[3] = {0, 0, 0};
+        partials2<64>(acc, mx, s.partials, s.flags, 0, local);
+        return;
+    }
+    const int tbx = local % it.tiles[0], tby = local / it.tiles[0], ch = it.tiles[2];
+    switch (s.limiter) {
+        case VANLEER: step2d_strip<VANLEER, false>(s, g, ch, tbx, tby, local); break;
+        case ALBADA: step2d_strip<ALBADA, false>(s, g, ch, tbx, tby, local); break;
+        case MINBEE: step2d_strip<MINBEE, false>(s, g, ch, tbx, tby, local); break;
+        case SUPERBEE: step2d_strip<SUPERBEE, false>(s, g, ch, tbx, tby, local); break;
+        default: step2d_strip<UNOS, false>(s, g, ch, tbx, tby, local); break;
+    }
+}
+
+__device__ __forceinline__ void wave2(const real* u, real g, double (&mx)[3]) {
+    const Prim2 p = prim2(u, g);
+    const double vx = (double)(fabs(p.ux) + p.a), vy = (double)(fabs(p.vy) + p.a);
+    mx[0] = (vx != vx || vx > mx[0]) ? vx : mx[0];
+    mx[1] = (vy != vy || vy > mx[1]) ? vy : mx[1];
 }
 
 __device__ __forceinline__ void llf2_face(const real* L, const real* R, real gl, real gr, int dir, real* F) {
@@ -278,14 +324,17 @@ __device__ __forceinline__ void llf2_face(const real* L, const real* R, real gl,
     for (int c = 0; c < 4; ++c) F[c] = (real)0.5 * (fl[c] + fr[c] - sp * (R[c] - L[c]));
 }
 
-__global__ void __launch_bounds__(256) k_llf2d(Step s, G2 g) {
+// First-order pass over cells start, start+stride, ...; partial slot pblk.
+__device__ __forceinline__ void llf2d_cells(const Step& s, const G2& g, long start, long stride, long pblk) {
     double acc[5] = {0, 0, 0, 0, 0}, mx[3] = {0, 0, 0};
     int bad = 0;
     const bool need = (*s.flags & fvl::F_UPD_BAD) != 0;
+    if (!need) return;  // partials of the second-order pass stay in use
     const int nx = g.nx, ny = g.ny;
     const long N = (long)nx * ny;
-    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (need && i < N) {
+    const fvl::Coef K = fvl::coef_of(s);
+    if (!K.active) return;
+    for (long i = start; i < N; i += stride) {
         const int x = (int)(i % nx), y = (int)(i / nx);
         const real* U = (const real*)s.U;
         real Ac[4], o[4], rqv[2], qc[2] = {0, 0};
@@ -303,7 +352,7 @@ __global__ void __launch_bounds__(256) k_llf2d(Step s, G2 g) {
             load4(U, N, g, x + dx, y + dy, Hi);
             llf2_face(Lo, Ac, gamma2(s, g, x - dx, y - dy), gcen, m, Fl);
             llf2_face(Ac, Hi, gcen, gamma2(s, g, x + dx, y + dy), m, Fh);
-            const real cm = (real)s.c[m];
+            const real cm = (real)K.c[m];
             for (int c = 0; c < 4; ++c) o[c] = o[c] - cm * (Fh[c] - Fl[c]);
             for (int j = 0; j < s.nq; ++j) {
                 const real* q = (const real*)s.q[j];
@@ -314,9 +363,9 @@ __global__ void __launch_bounds__(256) k_llf2d(Step s, G2 g) {
             }
             const int pos = m == 0 ? x : y, n = m == 0 ? nx : ny;
             if (pos == 0)
-                for (int c = 0; c < 4; ++c) acc[c] -= s.w[m] * (double)Fl[c];
+                for (int c = 0; c < 4; ++c) acc[c] -= K.w[m] * (double)Fl[c];
             if (pos == n - 1)
-                for (int c = 0; c < 4; ++c) acc[c] += s.w[m] * (double)Fh[c];
+                for (int c = 0; c < 4; ++c) acc[c] += K.w[m] * (double)Fh[c];
             if (s.F[m]) {
                 const long M = (long)(ny + dy) * (nx + dx);
                 long f = (long)y * (nx + dx) + x, fh = (long)(y + dy) * (nx + dx) + (x + dx);
@@ -327,11 +376,39 @@ __global__ void __launch_bounds__(256) k_llf2d(Step s, G2 g) {
         }
         for (int c = 0; c < 4; ++c) ((real*)s.Uout)[c * N + i] = o[c];
         if (!physical4(o)) bad |= fvl::F_LLF_BAD;
+        if (s.want_cfl) wave2(o, gcen, mx);
         for (int j = 0; j < s.nq; ++j)
             ((real*)s.qout[j])[i] = rmin(rmax(rqv[j] / rmax(o[0], (real)1e-12), (real)0), (real)1);
     }
-    if (!need) return;  // partials of the second-order pass stay in place
-    partials2<256>(acc, mx, s.partials, s.flags, bad);
+    partials2<256>(acc, mx, s.partials, s.flags, bad, pblk);
+}
+
+__global__ void __launch_bounds__(256) k_llf2d(Step s, G2 g) {
+    llf2d_cells(s, g, blockIdx.x * (long)blockDim.x + threadIdx.x, (long)gridDim.x * blockDim.x, blockIdx.x);
+}
+
+__global__ void __launch_bounds__(256) k_batch_llf2d(const fvl::BatchItem* items, const int* first, int n) {
+    const int i = owner_of(first, n, blockIdx.x);
+    const fvl::BatchItem& it = items[i];
+    const int local = blockIdx.x - first[i], nblk = first[i + 1] - first[i];
+    llf2d_cells(it.step, g2_of(it.grid), local * 256L + threadIdx.x, nblk * 256L, it.llf_first + local);
+}
+
+// CFL maxima of each 2D instance's state (interior; the inlet ghost state of
+// an Inflow boundary is added by the batch post kernel).
+__global__ void __launch_bounds__(256) k_batch_cfl2d(const fvl::BatchItem* items, const int* first, int n) {
+    const int i = owner_of(first, n, blockIdx.x);
+    const fvl::BatchItem& it = items[i];
+    const int local = blockIdx.x - first[i], nblk = first[i + 1] - first[i];
+    const long N = (long)it.grid.nx * it.grid.ny;
+    const real* U = (const real*)it.step.U;
+    double acc[5] = {0, 0, 0, 0, 0}, mx[3] = {0, 0, 0};
+    for (long c = local * 256L + threadIdx.x; c < N; c += nblk * 256L) {
+        real u[4];
+        for (int k = 0; k < 4; ++k) u[k] = U[k * N + c];
+        wave2(u, (real)it.step.gamma, mx);
+    }
+    partials2<256>(acc, mx, it.step.partials, nullptr, 0, it.cfl_first + local);
 }
 
 // CFL maxima over the padded 2D state: interior cells and, with an Inflow
@@ -378,6 +455,26 @@ static int ychunk_of(const Grid& g) {
 int step2d_blocks(const Grid& g, const fvl::Launch&) {
     int ch = ychunk_of(g);
     return ((g.nx + 61) / 62) * ((g.ny + ch - 1) / ch);
+}
+
+int strip2d_rows(const Grid& g) { return ychunk_of(g); }
+
+hipError_t batch2d(const fvl::BatchItem* items, const int* first, int n, int blocks, hipStream_t st) {
+    if (blocks <= 0) return hipSuccess;
+    k_batch2d<<<blocks, 64, 0, st>>>(items, first, n);
+    return hipGetLastError();
+}
+
+hipError_t batch_llf2d(const fvl::BatchItem* items, const int* first, int n, int blocks, hipStream_t st) {
+    if (blocks <= 0) return hipSuccess;
+    k_batch_llf2d<<<blocks, 256, 0, st>>>(items, first, n);
+    return hipGetLastError();
+}
+
+hipError_t batch_cfl2d(const fvl::BatchItem* items, const int* first, int n, int blocks, hipStream_t st) {
+    if (blocks <= 0) return hipSuccess;
+    k_batch_cfl2d<<<blocks, 256, 0, st>>>(items, first, n);
+    return hipGetLastError();
 }
 
 template <bool SPEC>

@@ -54,7 +54,7 @@ __device__ __forceinline__ real load1(const real* __restrict__ q, const G3& g, i
 // threads, written to partials[block * NSLOT + slot] by thread 0.
 template <int NT>
 __device__ void block_partials(double (&acc)[5], double (&mx)[3], double src, double* partials, int* flags,
-                               int bad) {
+                               int bad, long block = -1) {
     __shared__ double red[NT / 64 > 0 ? NT / 64 : 1][9];
     __shared__ int sbad;
     const int tid = threadIdx.x + blockDim.x * threadIdx.y;
@@ -74,7 +74,8 @@ __device__ void block_partials(double (&acc)[5], double (&mx)[3], double src, do
     }
     __syncthreads();
     if (tid == 0) {
-        const long b = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
+        const long b =
+            block >= 0 ? block : blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
         double* p = partials + b * fvl::NSLOT;
         for (int i = 0; i < 9; ++i) {
             double r = red[0][i];
@@ -112,27 +113,43 @@ __device__ __forceinline__ void wave3(const real* o, double g, double (&mx)[3]) 
 #endif
 }
 
+// LDS footprint of one tile: the A plane with a two-cell halo (reused for
+// the y fluxes) and the y face states.
+template <int BX, int BY, bool SPEC>
+struct Tile3 {
+    static constexpr int NQ = SPEC ? 2 : 0;
+    static constexpr int PW = BX + 2, PH = BY + 2;  // plane: x in [-2, TX+1], y in [-2, TY+1]
+    static constexpr int NYS = 6 + NQ;              // y-face payload: state, gamma, fractions
+    static constexpr int NFY = 5 + NQ;
+    static constexpr int PLANE = 5 * PW * PH;
+    static constexpr int FLUX = NFY * BX * BY;
+    static constexpr int BUF = PLANE > FLUX ? PLANE : FLUX;
+    static constexpr int YS = NYS * BY * BX;
+};
+
+using fvl::Coef;
+using fvl::coef_of;
+
+// One tile (tbx, tby) of one z chunk tbz; `pblk` is the index of this
+// block's slot in s.partials.
 template <int LIM, int BX, int BY, bool SPEC>
-__global__ void __launch_bounds__(BX* BY) k_step3d(Step s, G3 g) {
+__device__ __forceinline__ void step3d_tile(const Step& s, const G3& g, int tbx, int tby, int tbz, long pblk,
+                                            real* sbuf, real* sybuf) {
+    using T = Tile3<BX, BY, SPEC>;
     constexpr int TX = BX - 2, TY = BY - 2;
-    constexpr int NQ = SPEC ? 2 : 0;
-    constexpr int PW = BX + 2, PH = BY + 2;  // LDS plane: x in [-2, TX+1], y in [-2, TY+1]
-    constexpr int NYS = 6 + NQ;              // y-face payload: state, gamma, fractions
-    constexpr int NFY = 5 + NQ;
-    constexpr int PLANE = 5 * PW * PH;
-    constexpr int FLUX = NFY * BX * BY;
-    __shared__ real sbuf[PLANE > FLUX ? PLANE : FLUX];  // A plane, later the y fluxes
-    __shared__ real sY[NYS][BY][BX];
+    constexpr int NQ = T::NQ;
+    constexpr int PW = T::PW, PH = T::PH;
     real(*sA)[PH][PW] = reinterpret_cast<real(*)[PH][PW]>(sbuf);
     real(*sF)[BY][BX] = reinterpret_cast<real(*)[BY][BX]>(sbuf);
+    real(*sY)[BY][BX] = reinterpret_cast<real(*)[BY][BX]>(sybuf);
 
     const int lane = threadIdx.x, row = threadIdx.y;
     const int lx = lane - 1, ly = row - 1;
-    const int bx0 = blockIdx.x * TX, by0 = blockIdx.y * TY;
+    const int bx0 = tbx * TX, by0 = tby * TY;
     const int x = bx0 + lx, y = by0 + ly;
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const long N = (long)nx * ny * nz;
-    const int z0 = blockIdx.z * s.zchunk, z1 = min(z0 + s.zchunk, nz);
+    const int z0 = tbz * s.zchunk, z1 = min(z0 + s.zchunk, nz);
     const real* __restrict__ U = (const real*)s.U;
     real* __restrict__ Uo = (real*)s.Uout;
     const real* __restrict__ lamp = (const real*)s.lam;
@@ -142,8 +159,10 @@ __global__ void __launch_bounds__(BX* BY) k_step3d(Step s, G3 g) {
     const bool owns = lx >= 0 && lx < TX && ly >= 0 && ly < TY && x < nx && y < ny;
     const bool xface = lx >= 0 && ly >= 0 && ly < TY && x <= nx && y < ny;
     const bool yface = lx >= 0 && lx < TX && ly >= 0 && x < nx && y <= ny;
-    const real c0 = (real)s.c[0], c1 = (real)s.c[1], c2 = (real)s.c[2];
-    const real hc[3] = {(real)s.hc[0], (real)s.hc[1], (real)s.hc[2]};
+    const Coef K = coef_of(s);
+    const real c0 = (real)K.c[0], c1 = (real)K.c[1], c2 = (real)K.c[2];
+    const real hc[3] = {(real)K.hc[0], (real)K.hc[1], (real)K.hc[2]};
+    const double w0 = K.w[0], w1 = K.w[1], w2 = K.w[2];
     const real gconst = (real)s.gamma;
 
     // Ring of the LDS plane not covered by the block's own columns.
@@ -314,7 +333,7 @@ __global__ void __launch_bounds__(BX* BY) k_step3d(Step s, G3 g) {
         // Outward boundary fluxes and optional face dumps.
         if (plane_out && xface) {
             if (x == 0 || x == nx) {
-                double sgn = x == 0 ? -s.w[0] : s.w[0];
+                double sgn = x == 0 ? -w0 : w0;
 #pragma unroll
                 for (int c = 0; c < 5; ++c) acc[c] += sgn * (double)Fx[c];
             }
@@ -326,7 +345,7 @@ __global__ void __launch_bounds__(BX* BY) k_step3d(Step s, G3 g) {
         }
         if (plane_out && yface) {
             if (y == 0 || y == ny) {
-                double sgn = y == 0 ? -s.w[1] : s.w[1];
+                double sgn = y == 0 ? -w1 : w1;
 #pragma unroll
                 for (int c = 0; c < 5; ++c) acc[c] += sgn * (double)Fy[c];
             }
@@ -338,7 +357,7 @@ __global__ void __launch_bounds__(BX* BY) k_step3d(Step s, G3 g) {
         }
         if (zface) {
             if (k == 0 || k == nz) {
-                double sgn = k == 0 ? -s.w[2] : s.w[2];
+                double sgn = k == 0 ? -w2 : w2;
 #pragma unroll
                 for (int c = 0; c < 5; ++c) acc[c] += sgn * (double)Fz[c];
             }
@@ -363,7 +382,53 @@ __global__ void __launch_bounds__(BX* BY) k_step3d(Step s, G3 g) {
             sfzp[j] = sfz[j];
         }
     }
-    block_partials<BX * BY>(acc, mx, 0.0, s.partials, s.flags, bad);
+    block_partials<BX * BY>(acc, mx, 0.0, s.partials, s.flags, bad, pblk);
+}
+
+template <int LIM, int BX, int BY, bool SPEC>
+__global__ void __launch_bounds__(BX* BY) k_step3d(Step s, G3 g) {
+    using T = Tile3<BX, BY, SPEC>;
+    __shared__ real sbuf[T::BUF];
+    __shared__ real sy[T::YS];
+    const long b = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
+    step3d_tile<LIM, BX, BY, SPEC>(s, g, blockIdx.x, blockIdx.y, blockIdx.z, b, sbuf, sy);
+}
+
+// Work-list form: blocks of many instances of different sizes, limiters
+// and boundaries in one launch. Block b belongs to the instance i with
+// first[i] <= b < first[i+1]; its tile follows from that instance's tile
+// counts. Instances that are idle this step (coef active flag clear) only
+// publish empty partials.
+template <int BX, int BY>
+__global__ void __launch_bounds__(BX* BY) k_batch3d(const fvl::BatchItem* items, const int* first, int n) {
+    using T = Tile3<BX, BY, false>;
+    __shared__ real sbuf[T::BUF];
+    __shared__ real sy[T::YS];
+    const int b = blockIdx.x;
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) / 2;
+        if (first[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    const fvl::BatchItem& it = items[lo];
+    const Step& s = it.step;
+    const G3 g{it.grid.nx, it.grid.ny, it.grid.nz, it.grid.bc[0], it.grid.bc[1], it.grid.bc[2]};
+    const int local = b - first[lo];
+    const int tbx = local % it.tiles[0], tby = (local / it.tiles[0]) % it.tiles[1];
+    const int tbz = local / (it.tiles[0] * it.tiles[1]);
+    if (s.coef && s.coef[9] == 0.0) {
+        double acc[5] = {0, 0, 0, 0, 0}, mx[3] = {0, 0, 0};
+        block_partials<BX * BY>(acc, mx, 0.0, s.partials, s.flags, 0, local);
+        return;
+    }
+    switch (s.limiter) {
+        case VANLEER: step3d_tile<VANLEER, BX, BY, false>(s, g, tbx, tby, tbz, local, sbuf, sy); break;
+        case ALBADA: step3d_tile<ALBADA, BX, BY, false>(s, g, tbx, tby, tbz, local, sbuf, sy); break;
+        case MINBEE: step3d_tile<MINBEE, BX, BY, false>(s, g, tbx, tby, tbz, local, sbuf, sy); break;
+        case SUPERBEE: step3d_tile<SUPERBEE, BX, BY, false>(s, g, tbx, tby, tbz, local, sbuf, sy); break;
+        default: step3d_tile<UNOS, BX, BY, false>(s, g, tbx, tby, tbz, local, sbuf, sy); break;
+    }
 }
 
 // First-order LLF pass over the whole grid, run only when the second-order
@@ -380,14 +445,17 @@ __device__ __forceinline__ void llf_face(const real* L, const real* R, real gl, 
     for (int c = 0; c < 5; ++c) F[c] = (real)0.5 * (fl[c] + fr[c] - speed * (R[c] - L[c]));
 }
 
-__global__ void __launch_bounds__(256) k_llf3d(Step s, G3 g) {
+// Cells start, start+stride, ... of one instance; partial slot pblk.
+__device__ __forceinline__ void llf3d_cells(const Step& s, const G3& g, long start, long stride, long pblk) {
     double acc[5] = {0, 0, 0, 0, 0}, mx[3] = {0, 0, 0};
     int bad = 0;
     const int need = *s.flags & (fvl::F_PRED_BAD | fvl::F_UPD_BAD);
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const long N = (long)nx * ny * nz;
     if (!need) return;  // the host reads these partials only after a fallback
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
+    const Coef K = coef_of(s);
+    if (!K.active) return;
+    for (long i = start; i < N; i += stride) {
         const int x = (int)(i % nx), y = (int)((i / nx) % ny), z = (int)(i / ((long)nx * ny));
         const real* U = (const real*)s.U;
         const real* lamp = (const real*)s.lam;
@@ -411,7 +479,7 @@ __global__ void __launch_bounds__(256) k_llf3d(Step s, G3 g) {
             real ghi = lamp ? gamma_from(load1(lamp, g, x + dx, y + dy, z + dz), direct) : (real)s.gamma;
             llf_face(Lo, Ac, glo, gcen, m + 1, Fl);
             llf_face(Ac, Hi, gcen, ghi, m + 1, Fh);
-            const real cm = (real)s.c[m];
+            const real cm = (real)K.c[m];
 #pragma unroll
             for (int c = 0; c < 5; ++c) o[c] = o[c] - cm * (Fh[c] - Fl[c]);
             for (int j = 0; j < s.nq; ++j) {
@@ -423,9 +491,9 @@ __global__ void __launch_bounds__(256) k_llf3d(Step s, G3 g) {
             }
             const int pos = m == 0 ? x : (m == 1 ? y : z), n = m == 0 ? nx : (m == 1 ? ny : nz);
             if (pos == 0)
-                for (int c = 0; c < 5; ++c) acc[c] -= s.w[m] * (double)Fl[c];
+                for (int c = 0; c < 5; ++c) acc[c] -= K.w[m] * (double)Fl[c];
             if (pos == n - 1)
-                for (int c = 0; c < 5; ++c) acc[c] += s.w[m] * (double)Fh[c];
+                for (int c = 0; c < 5; ++c) acc[c] += K.w[m] * (double)Fh[c];
             if (s.F[m]) {
                 const int ex = dx, ey = dy, ez = dz;
                 const long M = (long)(nz + ez) * (ny + ey) * (nx + ex);
@@ -446,7 +514,26 @@ __global__ void __launch_bounds__(256) k_llf3d(Step s, G3 g) {
             ((real*)s.qout[j])[i] = rmin(rmax(f, (real)0), (real)1);
         }
     }
-    block_partials<256>(acc, mx, 0.0, s.partials, s.flags, bad);
+    block_partials<256>(acc, mx, 0.0, s.partials, s.flags, bad, pblk);
+}
+
+__global__ void __launch_bounds__(256) k_llf3d(Step s, G3 g) {
+    llf3d_cells(s, g, blockIdx.x * (long)blockDim.x + threadIdx.x, (long)gridDim.x * blockDim.x, blockIdx.x);
+}
+
+// Work-list form: instance i owns blocks first[i] .. first[i+1]-1.
+__global__ void __launch_bounds__(256) k_batch_llf3d(const fvl::BatchItem* items, const int* first, int n) {
+    const int b = blockIdx.x;
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) / 2;
+        if (first[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    const fvl::BatchItem& it = items[lo];
+    const G3 g{it.grid.nx, it.grid.ny, it.grid.nz, it.grid.bc[0], it.grid.bc[1], it.grid.bc[2]};
+    const int local = b - first[lo], nblk = first[lo + 1] - first[lo];
+    llf3d_cells(it.step, g, local * 256L + threadIdx.x, nblk * 256L, it.llf_first + local);
 }
 
 // ---------------------------------------------------------------------------
@@ -633,6 +720,51 @@ int llf3d_blocks(const Grid& g) {
 
 hipError_t llf3d(const Grid& g, const Step& s, hipStream_t st) {
     k_llf3d<<<llf3d_blocks(g), 256, 0, st>>>(s, g3(g));
+    return hipGetLastError();
+}
+
+hipError_t batch3d(const fvl::BatchItem* items, const int* first, int n, int blocks, int bx, int by,
+                   hipStream_t st) {
+    if (blocks <= 0) return hipSuccess;
+    if (bx == 64 && by == 4) k_batch3d<64, 4><<<blocks, dim3(64, 4), 0, st>>>(items, first, n);
+    else if (bx == 64 && by == 8) k_batch3d<64, 8><<<blocks, dim3(64, 8), 0, st>>>(items, first, n);
+    else if (bx == 32 && by == 8) k_batch3d<32, 8><<<blocks, dim3(32, 8), 0, st>>>(items, first, n);
+    else if (bx == 32 && by == 16) k_batch3d<32, 16><<<blocks, dim3(32, 16), 0, st>>>(items, first, n);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t batch_llf3d(const fvl::BatchItem* items, const int* first, int n, int blocks, hipStream_t st) {
+    if (blocks <= 0) return hipSuccess;
+    k_batch_llf3d<<<blocks, 256, 0, st>>>(items, first, n);
+    return hipGetLastError();
+}
+
+// CFL maxima of each instance's current state (the batch's first step).
+__global__ void __launch_bounds__(256) k_batch_cfl3d(const fvl::BatchItem* items, const int* first, int n) {
+    const int b = blockIdx.x;
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) / 2;
+        if (first[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    const fvl::BatchItem& it = items[lo];
+    const int local = b - first[lo], nblk = first[lo + 1] - first[lo];
+    const long N = (long)it.grid.nx * it.grid.ny * it.grid.nz;
+    const real* U = (const real*)it.step.U;
+    double acc[5] = {0, 0, 0, 0, 0}, mx[3] = {0, 0, 0};
+    for (long i = local * 256L + threadIdx.x; i < N; i += nblk * 256L) {
+        real o[5];
+        for (int c = 0; c < 5; ++c) o[c] = U[c * N + i];
+        wave3(o, it.step.gamma, mx);
+    }
+    block_partials<256>(acc, mx, 0.0, it.step.partials, nullptr, 0, it.cfl_first + local);
+}
+
+hipError_t batch_cfl3d(const fvl::BatchItem* items, const int* first, int n, int blocks, hipStream_t st) {
+    if (blocks <= 0) return hipSuccess;
+    k_batch_cfl3d<<<blocks, 256, 0, st>>>(items, first, n);
     return hipGetLastError();
 }
 

@@ -297,6 +297,22 @@ __device__ __forceinline__ real gamma_of(real lam) { return ((real)4.5 - lam) / 
 // taken as stored when the array already holds gamma.
 __device__ __forceinline__ real gamma_from(real v, bool direct) { return direct ? v : gamma_of(v); }
 
+// Floored primitives and sound speed of a 2D cell (torchSolver2D.py:38-58,
+// 85-91, 420-447): the 2D CFL, LLF and inlet-state arithmetic.
+struct Prim2 {
+    real rho, ux, vy, p, a;
+};
+__device__ __forceinline__ Prim2 prim2(const real* u, real g) {
+    Prim2 q;
+    q.rho = rmax(u[0], (real)1e-12);
+    q.ux = u[1] / q.rho;
+    q.vy = u[2] / q.rho;
+    real ke = ((real)0.5 * q.rho) * (q.ux * q.ux + q.vy * q.vy);
+    q.p = rmax((g - (real)1) * (u[3] - ke), (real)1e-12);
+    q.a = sqrt((g * rmax(q.p, (real)1e-12)) / rmax(q.rho, (real)1e-12));
+    return q;
+}
+
 // 64-bit-safe lane shuffles for one wave (width 64).
 __device__ __forceinline__ real shfl_up1(real v) { return __shfl_up(v, 1, 64); }
 __device__ __forceinline__ real shfl_down1(real v) { return __shfl_down(v, 1, 64); }

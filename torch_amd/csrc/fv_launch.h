@@ -61,6 +61,64 @@ struct Step {
     // 1D options
     int controller, variant, wavespeed;
     double cfl;
+    // When set, the step coefficients come from this device array instead
+    // of c/hc/w: [dt/h x3, 0.5*dt/h x3, dt*vol/h x3, active flag].
+    const double* coef;
+};
+
+// Step coefficients: from the host (Step.c/hc/w) or, for graph-replayed
+// batches, from the device array Step.coef written by the batch post kernel.
+struct Coef {
+    double c[3], hc[3], w[3];
+    bool active;
+};
+__device__ __forceinline__ Coef coef_of(const Step& s) {
+    Coef k;
+    if (s.coef) {
+        for (int d = 0; d < 3; ++d) {
+            k.c[d] = s.coef[d];
+            k.hc[d] = s.coef[3 + d];
+            k.w[d] = s.coef[6 + d];
+        }
+        k.active = s.coef[9] != 0.0;
+    } else {
+        for (int d = 0; d < 3; ++d) {
+            k.c[d] = s.c[d];
+            k.hc[d] = s.hc[d];
+            k.w[d] = s.w[d];
+        }
+        k.active = true;
+    }
+    return k;
+}
+
+// One instance of a batch (device memory, one array per buffer parity).
+// Partial slots of an instance: [second-order | first-order | CFL pass].
+struct BatchItem {
+    Step step;
+    Grid grid;
+    int tiles[3];      // 3D: tiles in x, y and z chunks; 2D: strips in x, row chunks
+    int llf_first;     // first slot of the first-order pass
+    int cfl_first;     // first slot of the CFL pass
+    void* scratch;     // 1D work arrays
+};
+
+// Per-instance control record of a batch (device memory). The host fills
+// the constants at admission; the batch post kernel advances the rest after
+// every step, as Simulation.step does on the host (torchSession.py:374-522).
+struct BatchCtl {
+    double t_end, volume, cfl;
+    double h[3];
+    double time, dt;    // current time; dt of the step in flight
+    double maxima[3];    // wave maxima of the committed state
+    double outward[5];   // accumulated boundary integral
+    double coef[10];     // Step.coef of the step in flight
+    int dim, max_steps;
+    int nblk, nllf, ncfl;  // partial slots of the second-order, first-order and CFL passes
+    int steps, fallbacks;
+    int status;          // 0, or the TORCH_* code that stopped the instance
+    int active;          // 1 stepping, 2 waiting for its first dt, 0 stopped
+    int cur;             // which of the two state buffers holds the committed state
 };
 
 struct React {
@@ -115,6 +173,29 @@ struct Launch {
     hipError_t totals(const fvl::Grid& g, const void* U, double* partials, hipStream_t st);\
     hipError_t convert_in(const double* src, void* dst, long n, hipStream_t st);          \
     hipError_t convert_out(const void* src, double* dst, long n, hipStream_t st);         \
+    /* batches: items/first in device memory; first[n] = total blocks */                 \
+    hipError_t batch3d(const fvl::BatchItem* items, const int* first, int n, int blocks,   \
+                       int bx, int by, hipStream_t st);                                    \
+    hipError_t batch_llf3d(const fvl::BatchItem* items, const int* first, int n,          \
+                           int blocks, hipStream_t st);                                    \
+    hipError_t batch_cfl3d(const fvl::BatchItem* items, const int* first, int n,          \
+                           int blocks, hipStream_t st);                                    \
+    hipError_t batch2d(const fvl::BatchItem* items, const int* first, int n, int blocks,   \
+                       hipStream_t st);                                                    \
+    hipError_t batch_llf2d(const fvl::BatchItem* items, const int* first, int n,          \
+                           int blocks, hipStream_t st);                                    \
+    hipError_t batch_cfl2d(const fvl::BatchItem* items, const int* first, int n,          \
+                           int blocks, hipStream_t st);                                    \
+    hipError_t batch1d(const fvl::BatchItem* items, const int* first, int n, int blocks,   \
+                       hipStream_t st);                                                    \
+    hipError_t batch_llf1d(const fvl::BatchItem* items, const int* first, int n,          \
+                           int blocks, hipStream_t st);                                    \
+    hipError_t batch_cfl1d(const fvl::BatchItem* items, const int* first, int n,          \
+                           int blocks, int input, hipStream_t st);                         \
+    hipError_t batch_post(fvl::BatchItem* items, fvl::BatchCtl* ctl, int n, int prime,    \
+                          hipStream_t st);                                                 \
+    int strip2d_rows(const fvl::Grid& g);                                                 \
+    int step1d_tiles(const fvl::Grid& g);                                                 \
     }
 
 FVL_DECLARE(fv64)

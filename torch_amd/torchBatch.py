@@ -1,0 +1,127 @@
+"""Batches of sessions advanced together on one device.
+
+The reference steps every ``Simulation`` on its own
+(python/cfdSolver/torchSession.py:374-522); a set of problems is a Python
+loop over sessions. ``Batch`` keeps many inert sessions resident on the
+device and advances all of them with one graph replay per step. Each
+instance keeps the single-session arithmetic and bookkeeping (its own dt,
+time, step count and boundary integral), so a session stepped in a batch
+ends where it ends stepped alone.
+
+Typical use with existing sessions::
+
+    batch = Batch(precision="fp32")
+    ids = [batch.add(sim) for sim in sessions]
+    batch.run(100)
+    for sim, i in zip(sessions, ids):
+        batch.commit(i, sim)        # state, time, steps, integrals back
+"""
+
+import numpy as np
+
+from . import _abi
+from .torchExecution import DeviceExecution
+
+
+class Batch:
+    """Inert sessions of any dimension and size stepped together.
+
+    ``hbm_budget`` caps the device bytes the batch may hold (0: no cap);
+    ``add`` returns None when an instance does not fit, which is the
+    admission test of a serving loop.
+    """
+
+    def __init__(self, precision="fp32", device=0, hbm_budget=0):
+        self.precision = str(precision).lower()
+        self._execution = DeviceExecution(1, device=device, precision=self.precision)
+        self._batch = _abi.DeviceBatch(self._execution.context, _abi.precision_code(self.precision), hbm_budget)
+        self._meta = {}
+
+    # -- admission -----------------------------------------------------------
+    def add(self, sim, t_end=None, max_steps=None):
+        """Admit a Simulation's committed state; returns the instance id (or None).
+
+        The instance runs from ``sim.time`` to ``t_end`` (default: the
+        session's) with at most ``max_steps`` further steps (default: what the
+        session has left).
+        """
+        c = sim.config
+        if c["reactive"]:
+            raise ValueError("batches hold inert sessions")
+        cfg = sim._make_config()
+        cfg.precision = _abi.precision_code(self.precision)
+        t_end = c["t_end"] if t_end is None else float(t_end)
+        max_steps = c["max_steps"] - sim.steps if max_steps is None else int(max_steps)
+        ident = self._batch.add(cfg, sim.state, sim.time, t_end, max_steps)
+        if ident is not None:
+            self._meta[ident] = (c["dimension"] + 2, sim.steps)
+        return ident
+
+    def footprint(self, sim):
+        cfg = sim._make_config()
+        return _abi.DeviceBatch.footprint(cfg, _abi.precision_code(self.precision))
+
+    def remove(self, ident):
+        self._batch.remove(ident)
+        self._meta.pop(ident, None)
+
+    # -- stepping ------------------------------------------------------------
+    def run(self, steps):
+        """Advance every active instance by up to ``steps`` steps."""
+        self._batch.run(steps)
+        return self
+
+    def tune(self, reps=3):
+        """Pick the 3D tile shape and z chunk by timing the current instances."""
+        return self._batch.tune(reps)
+
+    def set_tiles(self, block_x, block_y, zchunk):
+        self._batch.set_tiles(block_x, block_y, zchunk)
+
+    def profile(self, enable=True):
+        return self._batch.profile(enable)
+
+    # -- results -------------------------------------------------------------
+    def info(self, ident):
+        p = self._batch.info(ident)
+        nf = self._meta[ident][0]
+        return dict(time=p.time, dt=p.dt, steps=p.steps, fallbacks=p.fallbacks, status=p.status,
+                    active=bool(p.active), completed=bool(p.completed), cells=p.cells,
+                    boundary_integral=np.array(p.outward[:nf]), wave_max=np.array(p.wave_max[:]))
+
+    def state(self, ident):
+        return self._batch.download(ident)
+
+    def error(self, ident):
+        """Raise the exception Simulation.step would have raised, if any."""
+        status = self._batch.info(ident).status
+        if status == _abi.NONPHYSICAL:
+            raise RuntimeError("Hydrodynamic update produced a nonphysical state")
+        if status == _abi.STALLED:
+            raise RuntimeError("Integration stalled before requested final time")
+        if status != _abi.OK:
+            raise RuntimeError(f"batch instance stopped with status {status}")
+
+    def commit(self, ident, sim):
+        """Write an instance's progress back into its Simulation."""
+        info = self.info(ident)
+        start_steps = self._meta[ident][1]
+        sim.state = self.state(ident)
+        sim.time = info["time"]
+        sim.steps = start_steps + info["steps"]
+        sim.fallbacks += info["fallbacks"]
+        sim.boundary_integral = sim.boundary_integral + info["boundary_integral"]
+        return sim
+
+    def close(self):
+        if getattr(self, "_batch", None) is not None:
+            self._batch.close()
+            self._batch = None
+        if getattr(self, "_execution", None) is not None:
+            self._execution.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
