@@ -1,20 +1,25 @@
-"""Benchmark: 3D Cartesian Sod tube on one MI355X, FP32 form and FP64 plain.
+"""Benchmark on one MI355X: the 3D Sod tube (FP32 batched form and FP64 plain
+method) and the batched uneven mix.
 
-Workload (one step = one full Simulation.step on device-resident state):
-3D Sod tube along x on a 256^3 uniform grid, gamma 1.4, (rho,u,p) = (1,0,1)
-for x <= 0.5 and (0.125,0,0.1) beyond, uniform in y and z, MINBEE limiter,
-open x ends, reflective y/z walls, CFL 0.8, synthetic initial data. Each
-precision runs W untimed warmup steps and K timed steps (default 200, the
+Headline workload (one step = one full Simulation.step of device-resident
+state): 3D Sod tube along x on a 256^3 uniform grid, gamma 1.4,
+(rho,u,p) = (1,0,1) for x <= 0.5 and (0.125,0,0.1) beyond, uniform in y and
+z, MINBEE limiter, open x ends, reflective y/z walls, CFL 0.8, synthetic
+initial data; W untimed warmup steps and K timed steps (default 200, the
 workload's length). The JSON line reports:
 
-- value: whole-job cell-updates/s of the FP32 form (FP32 bulk arithmetic,
-  FP64 CFL/balance reductions), with the FP64 plain method beside it;
-- roofline: the fused step kernel's algorithmic HBM bytes (40 B per FP32
-  cell-update, 80 B per FP64 one: one read and one write of the five
-  conserved fields) over its average launch time measured with HIP events
-  on the kernel's own stream, against MI355X's 8 TB/s;
+- value: whole-job cell-updates/s of the FP32 batched form (FP32 bulk
+  arithmetic, FP64 CFL/balance reductions; the instance stepped by graph
+  replays of the batch engine, tile picked by the autotuner);
+- fp64_plain: the FP64 plain method through the session step loop;
+- roofline: the 3D step kernel's algorithmic HBM bytes (40 B per FP32
+  cell-update: one read and one write of the five conserved fields) over
+  its average launch time measured with HIP events on the batch's stream,
+  against MI355X's 8 TB/s; traffic from rocprof counter runs when given;
 - fp32_l1_rel: per-field relative L1 error of the FP32 form against the FP64
   plain method driven with the same dt sequence;
+- batched: the uneven mix (256 seeded 1D/2D/3D two-state problems, 100
+  steps each, FP32 and FP64 batches) with its throughput and FP32 error;
 - cpu_baseline: the NumPy oracle (a restatement of the reference's CPU path)
   timed on this host on a bounded sample of the same workload.
 
@@ -73,6 +78,85 @@ def run_precision(n, precision, steps, warmup, tile=None):
         kernel_ms=kernel_ms / max(launches, 1), launches=launches, fallbacks=fallbacks, dts=dts,
         state=state,
     )
+
+
+def run_batched_sod(n, steps, warmup, precision="fp32", tile=None):
+    """The Sod instance as a batch of one: graph-replayed steps, tuned tile."""
+    from torch_amd.torchBatch import Batch
+
+    batch = Batch(precision=precision)
+    ident = batch.add_problem(sod_primitive(n), [1.0, 1.0, 1.0], t_end=OPTIONS["t_end"], cfl=OPTIONS["cfl"],
+                              limiter=OPTIONS["limiter"], boundaries=OPTIONS["boundaries"],
+                              max_steps=warmup + steps)
+    if tile:
+        batch.set_tiles(*tile)
+    else:
+        tile, _ = batch.tune(reps=3)
+    batch.run(warmup)
+    kernel_ms = batch.time_pass(reps=10)  # while the instance is still stepping
+    t0 = time.perf_counter()
+    batch.run(steps)
+    wall = time.perf_counter() - t0
+    info = batch.info(ident)
+    batch.close()
+    if info["steps"] != warmup + steps or info["status"] != 0:
+        raise RuntimeError(f"batched Sod instance stopped early: {info}")
+    return dict(wall=wall, ms_per_step=1e3 * wall / steps, value=n ** 3 * steps / wall, kernel_ms=kernel_ms,
+                tile=tile, fallbacks=info["fallbacks"])
+
+
+def run_uneven(seed, count, steps, precisions=("fp32", "fp64"), tile=None):
+    """The batched uneven mix in each precision; FP32 error against FP64."""
+    from torch_amd.torchBatch import Batch
+    from torch_amd.workloads import uneven_problems
+
+    out, batches, ids, cells = {}, {}, {}, {}
+    for precision in precisions:
+        t0 = time.perf_counter()
+        batch = Batch(precision=precision)
+        ids[precision] = []
+        by_dim = {1: 0, 2: 0, 3: 0}
+        for i, prim, opts in uneven_problems(seed, count):
+            ident = batch.add_problem(prim, opts["lengths"], t_end=100.0, cfl=opts["cfl"], gamma=opts["gamma"],
+                                      limiter=opts["limiter"], boundaries=opts["boundaries"], max_steps=steps)
+            ids[precision].append(ident)
+            ncell = int(np.prod(prim.shape[1:]))
+            cells[i] = ncell
+            by_dim[prim.ndim - 1] += ncell
+        if tile:
+            batch.set_tiles(*tile)
+            used = tile
+        else:
+            used, _ = batch.tune(reps=2)
+        batch.run(0)  # admission: first dt of every instance, graph capture
+        setup = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        batch.run(steps)
+        wall = time.perf_counter() - t0
+        done = [batch.info(i) for i in ids[precision]]
+        bad = [d for d in done if d["status"] != 0 or d["steps"] != steps]
+        total = sum(d["cells"] * d["steps"] for d in done)
+        out[precision] = dict(value=total / wall, ms_per_step=1e3 * wall / steps, wall=wall, setup_s=setup,
+                              tile=list(used), stopped_early=len(bad),
+                              fallback_steps=int(sum(d["fallbacks"] for d in done)),
+                              cells_by_dim={str(k): v for k, v in by_dim.items()})
+        batches[precision] = batch
+        print(f"# uneven {precision}: {out[precision]['ms_per_step']:.3f} ms/step, "
+              f"{out[precision]['value']:.4g} cell-updates/s, setup {setup:.1f} s", file=sys.stderr, flush=True)
+    err = None
+    if "fp32" in batches and "fp64" in batches:
+        worst = np.zeros(5)
+        per = []
+        for a, b in zip(ids["fp32"], ids["fp64"]):
+            x, y = batches["fp32"].state(a), batches["fp64"].state(b)
+            e = [float(np.sum(np.abs(u - v)) / max(np.sum(np.abs(v)), 1e-300)) for u, v in zip(x, y)]
+            per.append(max(e))
+            worst[: len(e)] = np.maximum(worst[: len(e)], e)
+        err = dict(max_l1_rel_per_field=[float(f"{e:.3e}") for e in worst],
+                   median_instance_l1_rel=float(f"{np.median(per):.3e}"))
+    for b in batches.values():
+        b.close()
+    return out, err, sum(cells.values())
 
 
 def roofline(n, bytes_per_cell, kernel_ms):
@@ -145,6 +229,18 @@ def copy_calibration(mib=1024, reps=3):
     print(json.dumps({"copy_mib": mib, "reps": reps, "GBps_rw": round(2 * n * reps / wall / 1e9, 1)}))
 
 
+LEGS = ("fp64", "fp32", "batched", "error", "uneven", "cpu")
+
+
+def _tile(text):
+    if not text:
+        return None
+    parts = tuple(int(v) for v in text.split(","))
+    if len(parts) != 3:
+        raise argparse.ArgumentTypeError("tile is BX,BY,ZCHUNK")
+    return parts
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -152,7 +248,14 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--legs", default=",".join(LEGS),
+                    help="comma list of legs to run (profiling runs pick one): " + ",".join(LEGS))
+    ap.add_argument("--tile", type=_tile, default=None, help="BX,BY,ZCHUNK of the batched Sod (skips the tuner)")
     ap.add_argument("--error-steps", type=int, default=50, help="steps of the FP32-vs-FP64 error pass")
+    ap.add_argument("--batch-count", type=int, default=256, help="instances of the uneven mix")
+    ap.add_argument("--batch-steps", type=int, default=100)
+    ap.add_argument("--batch-seed", type=int, default=508)
+    ap.add_argument("--batch-tile", type=_tile, default=None, help="BX,BY,ZCHUNK of the mix (skips the tuner)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device copies of MIB MiB (counter calibration) and exit")
     args = ap.parse_args(argv)
@@ -160,35 +263,70 @@ def main(argv=None):
         copy_calibration(args.copy_calibration)
         return 0
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (the workload is the "
-              "single-GPU 3D Sod tube; multi-GPU sharding is described, not run).", file=sys.stderr)
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (the workloads are "
+              "single-GPU; how they would shard is described, not run).", file=sys.stderr)
         return 2
+    legs = set(args.legs.split(","))
+    if args.no_cpu:
+        legs.discard("cpu")
+    unknown = legs - set(LEGS)
+    if unknown:
+        ap.error(f"unknown legs {sorted(unknown)}")
     n = args.n
-    print(f"# 3D Sod {n}^3: FP64 plain, {args.warmup}+{args.steps} steps", file=sys.stderr, flush=True)
-    r64 = run_precision(n, "fp64", args.steps, args.warmup)
-    print(f"# fp64: {r64['ms_per_step']:.3f} ms/step, kernel {r64['kernel_ms']:.3f} ms", file=sys.stderr,
-          flush=True)
-    r32 = run_precision(n, "fp32", args.steps, args.warmup)
-    print(f"# fp32: {r32['ms_per_step']:.3f} ms/step, kernel {r32['kernel_ms']:.3f} ms", file=sys.stderr,
-          flush=True)
-    err = fp32_error(n, args.warmup, args.error_steps, r64["dts"][: args.error_steps])
-    print(f"# fp32 L1 rel vs fp64 after {args.error_steps} steps: {err}", file=sys.stderr, flush=True)
-    cpu = None if args.no_cpu else cpu_baseline()
+    say = lambda msg: print(msg, file=sys.stderr, flush=True)  # noqa: E731
+    r64 = r32s = r32 = err = uneven = cpu = None
+    if "fp64" in legs or "error" in legs:
+        say(f"# 3D Sod {n}^3: FP64 plain, {args.warmup}+{args.steps} steps")
+        r64 = run_precision(n, "fp64", args.steps, args.warmup)
+        say(f"# fp64 plain: {r64['ms_per_step']:.3f} ms/step, kernel {r64['kernel_ms']:.3f} ms")
+    if "fp32" in legs:
+        r32s = run_precision(n, "fp32", args.steps, args.warmup)
+        say(f"# fp32 session loop: {r32s['ms_per_step']:.3f} ms/step, kernel {r32s['kernel_ms']:.3f} ms")
+    if "batched" in legs:
+        r32 = run_batched_sod(n, args.steps, args.warmup, tile=args.tile)
+        say(f"# fp32 batched: {r32['ms_per_step']:.3f} ms/step, kernel {r32['kernel_ms']:.3f} ms, tile {r32['tile']}")
+    if "error" in legs:
+        err = fp32_error(n, args.warmup, args.error_steps, r64["dts"][: args.error_steps])
+        say(f"# fp32 L1 rel vs fp64 after {args.error_steps} steps: {err}")
+    if "uneven" in legs:
+        res, uerr, ucells = run_uneven(args.batch_seed, args.batch_count, args.batch_steps, tile=args.batch_tile)
+        uneven = {
+            "workload": f"{args.batch_count} seeded (rng {args.batch_seed}) two-state problems, dimension "
+                        "cycling 1,2,3: 1D N in 256..8192, 2D 64^2..512^2, 3D 32^3..160^3, random limiter, "
+                        "gamma 1.4/1.67, open boundaries, CFL 0.4, one graph-replayed batch",
+            "instances": args.batch_count, "cells": ucells, "steps": args.batch_steps,
+            "unit": "cell-updates/s",
+            "fp32": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in res["fp32"].items()},
+            "fp64": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in res["fp64"].items()},
+            "fp32_vs_fp64": uerr,
+        }
+        if r64:
+            uneven["fp32_over_fp64_plain_sod"] = round(res["fp32"]["value"] / r64["value"], 3)
+    if "cpu" in legs:
+        cpu = cpu_baseline()
+    head = r32 or r32s
     line = {
-        "metric": METRIC, "value": round(r32["value"], 1), "unit": "cell-updates/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(r32["ms_per_step"], 4),
+        "metric": METRIC, "value": round(head["value"], 1) if head else None, "unit": "cell-updates/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(head["ms_per_step"], 4) if head else None,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"3D Cartesian Sod tube {n}^3, MINBEE, CFL 0.8, open x / reflective y,z, "
-                               "gamma 1.4, full Simulation.step (FP32 form: f32 bulk, f64 reductions)",
-                   "cells": n ** 3, "parallelism": "1 GPU"},
-        "roofline": roofline(n, 40, r32["kernel_ms"]),
-        "fp64_plain": {"value": round(r64["value"], 1), "ms_per_step": round(r64["ms_per_step"], 4),
-                       "dtype": "f64", "kernel_ms": round(r64["kernel_ms"], 4),
-                       "roofline": roofline(n, 80, r64["kernel_ms"]), "fallbacks": r64["fallbacks"]},
-        "fp32_kernel_ms": round(r32["kernel_ms"], 4),
-        "fp32_over_fp64": round(r32["value"] / r64["value"], 3),
-        "fp32_l1_rel": [float(f"{e:.3e}") for e in err],
+                               "gamma 1.4, full Simulation.step; FP32 form (f32 bulk, f64 reductions) as a "
+                               "graph-replayed batch of one",
+                   "cells": n ** 3, "tile": list(r32["tile"]) if r32 else None, "parallelism": "1 GPU"},
+        "roofline": roofline(n, 40, head["kernel_ms"]) if head else None,
+        "fp64_plain": None if not r64 else {
+            "value": round(r64["value"], 1), "ms_per_step": round(r64["ms_per_step"], 4), "dtype": "f64",
+            "kernel_ms": round(r64["kernel_ms"], 4), "roofline": roofline(n, 80, r64["kernel_ms"]),
+            "fallbacks": r64["fallbacks"]},
+        "fp32_session_loop": None if not r32s else {
+            "value": round(r32s["value"], 1), "ms_per_step": round(r32s["ms_per_step"], 4),
+            "kernel_ms": round(r32s["kernel_ms"], 4)},
+        "fp32_kernel_ms": round(head["kernel_ms"], 4) if head else None,
+        "fp32_over_fp64": round(head["value"] / r64["value"], 3) if head and r64 else None,
+        "fp32_l1_rel": [float(f"{e:.3e}") for e in err] if err else None,
+        "batched": uneven,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -180,6 +180,10 @@ int torch_batch_download(torch_batch* batch, int id, double* U);
  * instances for every supported tile shape and z chunk, keeps the fastest. */
 int torch_batch_tune(torch_batch* batch, int reps, int* block_x, int* block_y, int* zchunk, double* best_ms);
 int torch_batch_set_tiles(torch_batch* batch, int block_x, int block_y, int zchunk);
+/* Measurement hook: average milliseconds of `reps` launches of the 3D
+ * second-order work list on the current states (HIP events on the batch's
+ * stream); writes only the spare state buffers, so nothing advances. */
+int torch_batch_time_pass(torch_batch* batch, int reps, double* ms);
 /* Measurement hook: HIP events around the graph replays of each run. Returns
  * the accumulated milliseconds and steps since the last call, then resets. */
 int torch_batch_profile(torch_batch* batch, int enable, double* ms, long* steps);

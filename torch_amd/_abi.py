@@ -31,7 +31,7 @@ EXPORTS = (
     "torch_state_totals", "torch_state_profile", "torch_hllc_faces",
     "torch_batch_footprint", "torch_batch_create", "torch_batch_destroy", "torch_batch_add",
     "torch_batch_remove", "torch_batch_run", "torch_batch_info", "torch_batch_download",
-    "torch_batch_tune", "torch_batch_set_tiles", "torch_batch_profile",
+    "torch_batch_tune", "torch_batch_set_tiles", "torch_batch_profile", "torch_batch_time_pass",
 )
 
 
@@ -119,6 +119,7 @@ def load():
                                                     _dp]),
                 "torch_batch_set_tiles": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
                 "torch_batch_profile": (ctypes.c_int, [vp, ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_long)]),
+                "torch_batch_time_pass": (ctypes.c_int, [vp, ctypes.c_int, _dp]),
             }
             for name, (res, args) in sig.items():
                 fn = getattr(lib, name)
@@ -338,6 +339,12 @@ class DeviceBatch:
 
     def set_tiles(self, block_x, block_y, zchunk):
         check(load().torch_batch_set_tiles(self.handle, int(block_x), int(block_y), int(zchunk)))
+
+    def time_pass(self, reps=5):
+        """Average ms of the 3D second-order work list (nothing advances)."""
+        ms = ctypes.c_double()
+        check(load().torch_batch_time_pass(self.handle, int(reps), ctypes.byref(ms)))
+        return ms.value
 
     def profile(self, enable=True):
         """Enable replay timing; returns (ms, steps) since the last call."""

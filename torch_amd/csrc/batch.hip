@@ -70,18 +70,21 @@ struct torch_batch {
     int next_id = 1;
     bool dirty = true;
     int seg[4] = {0, 0, 0, 0};  // [3D | 2D | 1D] segment starts, seg[3] = n
+    int lim[3][6] = {};         // limiter sub-segment starts within each dimension segment
     fvl::BatchItem* items = nullptr;
+    fvl::BatchItem* pitems = nullptr;  // copies of the items of new instances (first dt)
     fvl::BatchCtl* ctl = nullptr;
     int* flags = nullptr;
     double* partials = nullptr;
     int* tables = nullptr;
     fvl::BatchCtl* host_ctl = nullptr;  // pinned
-    size_t cap_n = 0, cap_partials = 0, cap_tables = 0;
-    Table h3, l3, h2, l2, h1, c1;  // step tables
-    Table p3, p2, p1;              // prime tables (fresh suffix of each segment)
-    int fresh_first[3] = {0, 0, 0};
+    size_t cap_n = 0, cap_partials = 0, cap_tables = 0, cap_pitems = 0;
+    Table hyd[3][5];          // second-order tables per dimension and limiter
+    Table l3, l2, l1, c1;     // first-order tables per dimension, 1D CFL
+    Table p3, p2, p1;         // first-dt tables over pitems
+    int pseg[4] = {0, 0, 0, 0};
     hipGraphExec_t graph = nullptr;
-    int bx = 64, by = 8, zchunk = 32;
+    int bx = 32, by = 16, zchunk = 16;  // the tuned single-instance default
     bool profile = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double prof_ms = 0;
@@ -165,6 +168,17 @@ int grow(void** p, size_t* cap, size_t bytes) {
     return rc;
 }
 
+// The second-order pass of every 3D instance (one launch per limiter).
+hipError_t enqueue_3d(torch_batch* b, hipStream_t st) {
+    for (int L = 0; L < 5; ++L) {
+        const Table& t = b->hyd[0][L];
+        hipError_t e = b->ops->batch3d(b->items + b->lim[0][L], b->tables + t.offset, t.count, t.blocks, b->bx, b->by,
+                                       L, 0, st);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
 hipError_t enqueue_step(torch_batch* b, hipStream_t st) {
     const Ops* o = b->ops;
     const int* T = b->tables;
@@ -172,12 +186,18 @@ hipError_t enqueue_step(torch_batch* b, hipStream_t st) {
     fvl::BatchItem* I2 = b->items + b->seg[1];
     fvl::BatchItem* I1 = b->items + b->seg[2];
     hipError_t e;
-    if ((e = o->batch3d(I3, T + b->h3.offset, b->h3.count, b->h3.blocks, b->bx, b->by, st))) return e;
+    if ((e = enqueue_3d(b, st))) return e;
     if ((e = o->batch_llf3d(I3, T + b->l3.offset, b->l3.count, b->l3.blocks, st))) return e;
-    if ((e = o->batch2d(I2, T + b->h2.offset, b->h2.count, b->h2.blocks, st))) return e;
+    for (int L = 0; L < 5; ++L) {
+        const Table& t = b->hyd[1][L];
+        if ((e = o->batch2d(b->items + b->lim[1][L], T + t.offset, t.count, t.blocks, L, st))) return e;
+    }
     if ((e = o->batch_llf2d(I2, T + b->l2.offset, b->l2.count, b->l2.blocks, st))) return e;
-    if ((e = o->batch1d(I1, T + b->h1.offset, b->h1.count, b->h1.blocks, st))) return e;
-    if ((e = o->batch_llf1d(I1, T + b->h1.offset, b->h1.count, b->h1.blocks, st))) return e;
+    for (int L = 0; L < 5; ++L) {
+        const Table& t = b->hyd[2][L];
+        if ((e = o->batch1d(b->items + b->lim[2][L], T + t.offset, t.count, t.blocks, L, st))) return e;
+    }
+    if ((e = o->batch_llf1d(I1, T + b->l1.offset, b->l1.count, b->l1.blocks, st))) return e;
     if ((e = o->batch_cfl1d(I1, T + b->c1.offset, b->c1.count, b->c1.blocks, 0, st))) return e;
     return o->batch_post(b->items, b->ctl, b->seg[3], 0, st);
 }
@@ -221,7 +241,7 @@ int rebuild(torch_batch* b) {
     std::stable_sort(v.begin(), v.end(), [](const Inst* x, const Inst* y) {
         int sx = seg_of(x->grid.dim), sy = seg_of(y->grid.dim);
         if (sx != sy) return sx < sy;
-        return !x->fresh && y->fresh;  // new instances last within their segment
+        return x->cfg.limiter < y->cfg.limiter;
     });
     const int n = (int)v.size();
     for (int s = 0; s < 3; ++s) {
@@ -230,27 +250,38 @@ int rebuild(torch_batch* b) {
         b->seg[s] = i;
     }
     b->seg[3] = n;
+    for (int s = 0; s < 3; ++s) {
+        for (int L = 0; L <= 5; ++L) {
+            int i = b->seg[s];
+            while (i < b->seg[s + 1] && v[i]->cfg.limiter < L) ++i;
+            b->lim[s][L] = i;
+        }
+    }
     long slots = 0;
     for (Inst* p : v) {
         layout(b, p);
         p->pslot = slots;
         slots += (p->grid.dim == 1 ? 0 : p->nblk + p->nllf) + p->ncfl;
     }
+    std::vector<Inst*> fresh;
+    for (Inst* p : v)
+        if (p->fresh) fresh.push_back(p);
     for (int s = 0; s < 3; ++s) {
-        int i = b->seg[s];
-        while (i < b->seg[s + 1] && !v[i]->fresh) ++i;
-        b->fresh_first[s] = i;
+        int i = 0;
+        while (i < (int)fresh.size() && seg_of(fresh[i]->grid.dim) < s) ++i;
+        b->pseg[s] = i;
     }
+    b->pseg[3] = (int)fresh.size();
     std::vector<int> tb;
-    b->h3 = table(tb, v, b->seg[0], b->seg[1], &Inst::nblk);
+    for (int s = 0; s < 3; ++s)
+        for (int L = 0; L < 5; ++L) b->hyd[s][L] = table(tb, v, b->lim[s][L], b->lim[s][L + 1], &Inst::nblk);
     b->l3 = table(tb, v, b->seg[0], b->seg[1], &Inst::nllf);
-    b->h2 = table(tb, v, b->seg[1], b->seg[2], &Inst::nblk);
     b->l2 = table(tb, v, b->seg[1], b->seg[2], &Inst::nllf);
-    b->h1 = table(tb, v, b->seg[2], b->seg[3], &Inst::nblk);
+    b->l1 = table(tb, v, b->seg[2], b->seg[3], &Inst::nllf);
     b->c1 = table(tb, v, b->seg[2], b->seg[3], &Inst::ncfl);
-    b->p3 = table(tb, v, b->fresh_first[0], b->seg[1], &Inst::ncfl);
-    b->p2 = table(tb, v, b->fresh_first[1], b->seg[2], &Inst::ncfl);
-    b->p1 = table(tb, v, b->fresh_first[2], b->seg[3], &Inst::ncfl);
+    b->p3 = table(tb, fresh, b->pseg[0], b->pseg[1], &Inst::ncfl);
+    b->p2 = table(tb, fresh, b->pseg[1], b->pseg[2], &Inst::ncfl);
+    b->p1 = table(tb, fresh, b->pseg[2], b->pseg[3], &Inst::ncfl);
     if (n == 0) return TORCH_OK;
 
     int rc = TORCH_OK;
@@ -276,6 +307,9 @@ int rebuild(torch_batch* b) {
     if ((rc = grow((void**)&b->partials, &b->cap_partials, (size_t)slots * fvl::NSLOT * sizeof(double))))
         return rc;
     if ((rc = grow((void**)&b->tables, &b->cap_tables, tb.size() * sizeof(int)))) return rc;
+    if (!fresh.empty() &&
+        (rc = grow((void**)&b->pitems, &b->cap_pitems, fresh.size() * sizeof(fvl::BatchItem))))
+        return rc;
 
     std::vector<fvl::BatchItem> items(n);
     for (int i = 0; i < n; ++i) {
@@ -314,15 +348,20 @@ int rebuild(torch_batch* b) {
     HIP_TRY(hipMemsetAsync(b->flags, 0, n * kFlagStride * sizeof(int), st));
     HIP_TRY(hipStreamSynchronize(st));
 
-    // First dt of new instances: CFL maxima of their initial states.
-    const Ops* o = b->ops;
-    const int* T = b->tables;
-    bool any_fresh = false;
-    for (int s = 0; s < 3; ++s) any_fresh |= b->fresh_first[s] < b->seg[s + 1];
-    if (any_fresh) {
-        HIP_TRY(o->batch_cfl3d(b->items + b->fresh_first[0], T + b->p3.offset, b->p3.count, b->p3.blocks, st));
-        HIP_TRY(o->batch_cfl2d(b->items + b->fresh_first[1], T + b->p2.offset, b->p2.count, b->p2.blocks, st));
-        HIP_TRY(o->batch_cfl1d(b->items + b->fresh_first[2], T + b->p1.offset, b->p1.count, b->p1.blocks, 1, st));
+    // First dt of new instances: CFL maxima of their initial states, from
+    // copies of their items (same buffers and partial slots).
+    if (!fresh.empty()) {
+        std::vector<fvl::BatchItem> pit;
+        for (Inst* p : fresh)
+            for (int i = 0; i < n; ++i)
+                if (v[i] == p) pit.push_back(items[i]);
+        HIP_TRY(hipMemcpyAsync(b->pitems, pit.data(), pit.size() * sizeof(fvl::BatchItem), hipMemcpyHostToDevice,
+                               st));
+        const Ops* o = b->ops;
+        const int* T = b->tables;
+        HIP_TRY(o->batch_cfl3d(b->pitems + b->pseg[0], T + b->p3.offset, b->p3.count, b->p3.blocks, st));
+        HIP_TRY(o->batch_cfl2d(b->pitems + b->pseg[1], T + b->p2.offset, b->p2.count, b->p2.blocks, st));
+        HIP_TRY(o->batch_cfl1d(b->pitems + b->pseg[2], T + b->p1.offset, b->p1.count, b->p1.blocks, 1, st));
         HIP_TRY(o->batch_post(b->items, b->ctl, n, 1, st));
         if ((rc = read_ctl(b))) return rc;
         for (Inst* p : v) p->fresh = false;
@@ -368,6 +407,7 @@ int torch_batch_destroy(torch_batch* b) {
     (void)hipFree(b->flags);
     (void)hipFree(b->partials);
     (void)hipFree(b->tables);
+    (void)hipFree(b->pitems);
     (void)hipHostFree(b->host_ctl);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
@@ -497,9 +537,9 @@ int torch_batch_tune(torch_batch* b, int reps, int* block_x, int* block_y, int* 
     if (b->dirty && (rc = rebuild(b))) return rc;
     hipStream_t st = b->ctx->stream;
     double best = -1;
-    if (b->h3.count > 0) {
+    if (b->seg[1] > b->seg[0]) {
         static const int shapes[4][2] = {{64, 4}, {64, 8}, {32, 8}, {32, 16}};
-        static const int chunks[3] = {16, 32, 64};
+        static const int chunks[4] = {8, 16, 32, 64};
         int bbx = b->bx, bby = b->by, bz = b->zchunk;
         hipEvent_t e0, e1;
         HIP_TRY(hipEventCreate(&e0));
@@ -512,12 +552,9 @@ int torch_batch_tune(torch_batch* b, int reps, int* block_x, int* block_y, int* 
                 if ((rc = rebuild(b))) break;
                 // The second-order pass alone, writing only the spare state
                 // buffers and partials: the committed states do not move.
-                const fvl::BatchItem* I3 = b->items + b->seg[0];
-                HIP_TRY(b->ops->batch3d(I3, b->tables + b->h3.offset, b->h3.count, b->h3.blocks, b->bx, b->by, st));
+                HIP_TRY(enqueue_3d(b, st));
                 HIP_TRY(hipEventRecord(e0, st));
-                for (int r = 0; r < reps; ++r)
-                    HIP_TRY(b->ops->batch3d(I3, b->tables + b->h3.offset, b->h3.count, b->h3.blocks, b->bx, b->by,
-                                            st));
+                for (int r = 0; r < reps; ++r) HIP_TRY(enqueue_3d(b, st));
                 HIP_TRY(hipEventRecord(e1, st));
                 HIP_TRY(hipEventSynchronize(e1));
                 float ms = 0;
@@ -542,6 +579,35 @@ int torch_batch_tune(torch_batch* b, int reps, int* block_x, int* block_y, int* 
     if (block_y) *block_y = b->by;
     if (zchunk) *zchunk = b->zchunk;
     if (best_ms) *best_ms = best;
+    return TORCH_OK;
+}
+
+int torch_batch_time_pass(torch_batch* b, int reps, double* ms) {
+    if (!b || reps <= 0 || !ms) return fail(TORCH_INVALID, "invalid argument");
+    std::lock_guard<std::mutex> lock(b->ctx->mu);
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    int rc;
+    if (b->dirty && (rc = rebuild(b))) return rc;
+    *ms = 0;
+    if (b->seg[1] == b->seg[0]) return TORCH_OK;
+    hipStream_t st = b->ctx->stream;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    hipError_t e = hipEventRecord(e0, st);
+    for (int r = 0; r < reps && e == hipSuccess; ++r) e = enqueue_3d(b, st);
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float t = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (e != hipSuccess) return fail(TORCH_HIP_ERROR, hipGetErrorString(e));
+    *ms = t / reps;
+    // The trial passes may have raised flags the next real step raises anyway;
+    // start it from clear ones as the post kernel leaves them.
+    HIP_TRY(hipMemsetAsync(b->flags, 0, b->seg[3] * kFlagStride * sizeof(int), st));
+    HIP_TRY(hipStreamSynchronize(st));
     return TORCH_OK;
 }
 

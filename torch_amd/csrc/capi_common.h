@@ -56,13 +56,13 @@ struct Ops {
     hipError_t (*convert_in)(const double*, void*, long, hipStream_t);
     hipError_t (*convert_out)(const void*, double*, long, hipStream_t);
     // batches
-    hipError_t (*batch3d)(const fvl::BatchItem*, const int*, int, int, int, int, hipStream_t);
+    hipError_t (*batch3d)(const fvl::BatchItem*, const int*, int, int, int, int, int, int, hipStream_t);
     hipError_t (*batch_llf3d)(const fvl::BatchItem*, const int*, int, int, hipStream_t);
     hipError_t (*batch_cfl3d)(const fvl::BatchItem*, const int*, int, int, hipStream_t);
-    hipError_t (*batch2d)(const fvl::BatchItem*, const int*, int, int, hipStream_t);
+    hipError_t (*batch2d)(const fvl::BatchItem*, const int*, int, int, int, hipStream_t);
     hipError_t (*batch_llf2d)(const fvl::BatchItem*, const int*, int, int, hipStream_t);
     hipError_t (*batch_cfl2d)(const fvl::BatchItem*, const int*, int, int, hipStream_t);
-    hipError_t (*batch1d)(const fvl::BatchItem*, const int*, int, int, hipStream_t);
+    hipError_t (*batch1d)(const fvl::BatchItem*, const int*, int, int, int, hipStream_t);
     hipError_t (*batch_llf1d)(const fvl::BatchItem*, const int*, int, int, hipStream_t);
     hipError_t (*batch_cfl1d)(const fvl::BatchItem*, const int*, int, int, int, hipStream_t);
     hipError_t (*batch_post)(fvl::BatchItem*, fvl::BatchCtl*, int, int, hipStream_t);

@@ -402,38 +402,21 @@ __global__ void k_bound1(Step s, int n, const real* Fs) {
 }
 
 // Work-list forms: instance i owns blocks first[i] .. first[i+1]-1, one
-// tile of T1 cells (or 256 CFL faces) per block.
-__device__ __forceinline__ int owner1(const int* first, int n, int b) {
-    int lo = 0, hi = n - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) / 2;
-        if (first[mid] <= b) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
-}
-
+// tile of T1 cells (or 256 CFL faces) per block; one limiter per launch.
+template <int LIM>
 __global__ void __launch_bounds__(256) k_batch1d(const fvl::BatchItem* items, const int* first, int n) {
     __shared__ Lds1 m;
-    const int i = owner1(first, n, blockIdx.x);
-    const fvl::BatchItem& it = items[i];
+    const int i = fvl::owner(first, n, blockIdx.x);
+    const fvl::BatchItem& it = fvl::item_at(items, i);
     const Step& s = it.step;
     if (s.coef && s.coef[9] == 0.0) return;
-    const int local = blockIdx.x - first[i], nx = it.grid.nx, bc = it.grid.bc[0];
-    real* Fs = (real*)it.scratch;
-    switch (s.limiter) {
-        case VANLEER: step1d_tile<VANLEER>(s, nx, bc, local, Fs, m); break;
-        case ALBADA: step1d_tile<ALBADA>(s, nx, bc, local, Fs, m); break;
-        case MINBEE: step1d_tile<MINBEE>(s, nx, bc, local, Fs, m); break;
-        case SUPERBEE: step1d_tile<SUPERBEE>(s, nx, bc, local, Fs, m); break;
-        default: step1d_tile<UNOS>(s, nx, bc, local, Fs, m); break;
-    }
+    step1d_tile<LIM>(s, it.grid.nx, it.grid.bc[0], blockIdx.x - first[i], (real*)it.scratch, m);
 }
 
 __global__ void __launch_bounds__(256) k_batch_llf1d(const fvl::BatchItem* items, const int* first, int n) {
     __shared__ Lds1 m;
-    const int i = owner1(first, n, blockIdx.x);
-    const fvl::BatchItem& it = items[i];
+    const int i = fvl::owner(first, n, blockIdx.x);
+    const fvl::BatchItem& it = fvl::item_at(items, i);
     if (it.step.coef && it.step.coef[9] == 0.0) return;
     llf1d_tile(it.step, it.grid.nx, it.grid.bc[0], blockIdx.x - first[i], (real*)it.scratch, m);
 }
@@ -442,8 +425,8 @@ __global__ void __launch_bounds__(256) k_batch_llf1d(const fvl::BatchItem* items
 // `input` set, of its input state (the first step of a new instance).
 __global__ void __launch_bounds__(256) k_batch_cfl1d(const fvl::BatchItem* items, const int* first, int n,
                                                      int input) {
-    const int i = owner1(first, n, blockIdx.x);
-    const fvl::BatchItem& it = items[i];
+    const int i = fvl::owner(first, n, blockIdx.x);
+    const fvl::BatchItem& it = fvl::item_at(items, i);
     if (!input && it.step.coef && it.step.coef[9] == 0.0) return;
     Step s = it.step;
     if (!input) s.U = s.Uout;
@@ -482,9 +465,15 @@ hipError_t cfl1d(const Grid& g, const Step& s, hipStream_t st) {
     return hipGetLastError();
 }
 
-hipError_t batch1d(const fvl::BatchItem* items, const int* first, int n, int blocks, hipStream_t st) {
+hipError_t batch1d(const fvl::BatchItem* items, const int* first, int n, int blocks, int limiter, hipStream_t st) {
     if (blocks <= 0) return hipSuccess;
-    k_batch1d<<<blocks, 256, 0, st>>>(items, first, n);
+    switch (limiter) {
+        case VANLEER: k_batch1d<VANLEER><<<blocks, 256, 0, st>>>(items, first, n); break;
+        case ALBADA: k_batch1d<ALBADA><<<blocks, 256, 0, st>>>(items, first, n); break;
+        case MINBEE: k_batch1d<MINBEE><<<blocks, 256, 0, st>>>(items, first, n); break;
+        case SUPERBEE: k_batch1d<SUPERBEE><<<blocks, 256, 0, st>>>(items, first, n); break;
+        default: k_batch1d<UNOS><<<blocks, 256, 0, st>>>(items, first, n); break;
+    }
     return hipGetLastError();
 }
 

@@ -268,20 +268,12 @@ __device__ __forceinline__ G2 g2_of(const Grid& g) {
     return q;
 }
 
-__device__ __forceinline__ int owner_of(const int* first, int n, int b) {
-    int lo = 0, hi = n - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) / 2;
-        if (first[mid] <= b) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
-}
-
-// Work-list form over 2D instances (strips x row chunks per instance).
+// Work-list form over 2D instances (strips x row chunks per instance), one
+// limiter per launch.
+template <int LIM>
 __global__ void __launch_bounds__(64) k_batch2d(const fvl::BatchItem* items, const int* first, int n) {
-    const int i = owner_of(first, n, blockIdx.x);
-    const fvl::BatchItem& it = items[i];
+    const int i = fvl::owner(first, n, blockIdx.x);
+    const fvl::BatchItem& it = fvl::item_at(items, i);
     const int local = blockIdx.x - first[i];
     const G2 g = g2_of(it.grid);
     const Step& s = it.step;
@@ -290,14 +282,7 @@ __global__ void __launch_bounds__(64) k_batch2d(const fvl::BatchItem* items, con
         partials2<64>(acc, mx, s.partials, s.flags, 0, local);
         return;
     }
-    const int tbx = local % it.tiles[0], tby = local / it.tiles[0], ch = it.tiles[2];
-    switch (s.limiter) {
-        case VANLEER: step2d_strip<VANLEER, false>(s, g, ch, tbx, tby, local); break;
-        case ALBADA: step2d_strip<ALBADA, false>(s, g, ch, tbx, tby, local); break;
-        case MINBEE: step2d_strip<MINBEE, false>(s, g, ch, tbx, tby, local); break;
-        case SUPERBEE: step2d_strip<SUPERBEE, false>(s, g, ch, tbx, tby, local); break;
-        default: step2d_strip<UNOS, false>(s, g, ch, tbx, tby, local); break;
-    }
+    step2d_strip<LIM, false>(s, g, it.tiles[2], local % it.tiles[0], local / it.tiles[0], local);
 }
 
 __device__ __forceinline__ void wave2(const real* u, real g, double (&mx)[3]) {
@@ -388,8 +373,8 @@ __global__ void __launch_bounds__(256) k_llf2d(Step s, G2 g) {
 }
 
 __global__ void __launch_bounds__(256) k_batch_llf2d(const fvl::BatchItem* items, const int* first, int n) {
-    const int i = owner_of(first, n, blockIdx.x);
-    const fvl::BatchItem& it = items[i];
+    const int i = fvl::owner(first, n, blockIdx.x);
+    const fvl::BatchItem& it = fvl::item_at(items, i);
     const int local = blockIdx.x - first[i], nblk = first[i + 1] - first[i];
     llf2d_cells(it.step, g2_of(it.grid), local * 256L + threadIdx.x, nblk * 256L, it.llf_first + local);
 }
@@ -397,8 +382,8 @@ __global__ void __launch_bounds__(256) k_batch_llf2d(const fvl::BatchItem* items
 // CFL maxima of each 2D instance's state (interior; the inlet ghost state of
 // an Inflow boundary is added by the batch post kernel).
 __global__ void __launch_bounds__(256) k_batch_cfl2d(const fvl::BatchItem* items, const int* first, int n) {
-    const int i = owner_of(first, n, blockIdx.x);
-    const fvl::BatchItem& it = items[i];
+    const int i = fvl::owner(first, n, blockIdx.x);
+    const fvl::BatchItem& it = fvl::item_at(items, i);
     const int local = blockIdx.x - first[i], nblk = first[i + 1] - first[i];
     const long N = (long)it.grid.nx * it.grid.ny;
     const real* U = (const real*)it.step.U;
@@ -459,9 +444,15 @@ int step2d_blocks(const Grid& g, const fvl::Launch&) {
 
 int strip2d_rows(const Grid& g) { return ychunk_of(g); }
 
-hipError_t batch2d(const fvl::BatchItem* items, const int* first, int n, int blocks, hipStream_t st) {
+hipError_t batch2d(const fvl::BatchItem* items, const int* first, int n, int blocks, int limiter, hipStream_t st) {
     if (blocks <= 0) return hipSuccess;
-    k_batch2d<<<blocks, 64, 0, st>>>(items, first, n);
+    switch (limiter) {
+        case VANLEER: k_batch2d<VANLEER><<<blocks, 64, 0, st>>>(items, first, n); break;
+        case ALBADA: k_batch2d<ALBADA><<<blocks, 64, 0, st>>>(items, first, n); break;
+        case MINBEE: k_batch2d<MINBEE><<<blocks, 64, 0, st>>>(items, first, n); break;
+        case SUPERBEE: k_batch2d<SUPERBEE><<<blocks, 64, 0, st>>>(items, first, n); break;
+        default: k_batch2d<UNOS><<<blocks, 64, 0, st>>>(items, first, n); break;
+    }
     return hipGetLastError();
 }
 

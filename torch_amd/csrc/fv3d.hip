@@ -394,24 +394,19 @@ __global__ void __launch_bounds__(BX* BY) k_step3d(Step s, G3 g) {
     step3d_tile<LIM, BX, BY, SPEC>(s, g, blockIdx.x, blockIdx.y, blockIdx.z, b, sbuf, sy);
 }
 
-// Work-list form: blocks of many instances of different sizes, limiters
-// and boundaries in one launch. Block b belongs to the instance i with
-// first[i] <= b < first[i+1]; its tile follows from that instance's tile
-// counts. Instances that are idle this step (coef active flag clear) only
-// publish empty partials.
-template <int BX, int BY>
+// Work-list form: blocks of many instances of different sizes and
+// boundaries (one limiter per launch) in one launch. Block b belongs to the
+// instance i with first[i] <= b < first[i+1]; its tile follows from that
+// instance's tile counts. Instances that are idle this step (coef active flag
+// clear) only publish empty partials.
+template <int LIM, int BX, int BY, bool SPEC>
 __global__ void __launch_bounds__(BX* BY) k_batch3d(const fvl::BatchItem* items, const int* first, int n) {
-    using T = Tile3<BX, BY, false>;
+    using T = Tile3<BX, BY, SPEC>;
     __shared__ real sbuf[T::BUF];
     __shared__ real sy[T::YS];
     const int b = blockIdx.x;
-    int lo = 0, hi = n - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) / 2;
-        if (first[mid] <= b) lo = mid;
-        else hi = mid - 1;
-    }
-    const fvl::BatchItem& it = items[lo];
+    const int lo = fvl::owner(first, n, b);
+    const fvl::BatchItem& it = fvl::item_at(items, lo);
     const Step& s = it.step;
     const G3 g{it.grid.nx, it.grid.ny, it.grid.nz, it.grid.bc[0], it.grid.bc[1], it.grid.bc[2]};
     const int local = b - first[lo];
@@ -422,13 +417,7 @@ __global__ void __launch_bounds__(BX* BY) k_batch3d(const fvl::BatchItem* items,
         block_partials<BX * BY>(acc, mx, 0.0, s.partials, s.flags, 0, local);
         return;
     }
-    switch (s.limiter) {
-        case VANLEER: step3d_tile<VANLEER, BX, BY, false>(s, g, tbx, tby, tbz, local, sbuf, sy); break;
-        case ALBADA: step3d_tile<ALBADA, BX, BY, false>(s, g, tbx, tby, tbz, local, sbuf, sy); break;
-        case MINBEE: step3d_tile<MINBEE, BX, BY, false>(s, g, tbx, tby, tbz, local, sbuf, sy); break;
-        case SUPERBEE: step3d_tile<SUPERBEE, BX, BY, false>(s, g, tbx, tby, tbz, local, sbuf, sy); break;
-        default: step3d_tile<UNOS, BX, BY, false>(s, g, tbx, tby, tbz, local, sbuf, sy); break;
-    }
+    step3d_tile<LIM, BX, BY, SPEC>(s, g, tbx, tby, tbz, local, sbuf, sy);
 }
 
 // First-order LLF pass over the whole grid, run only when the second-order
@@ -524,13 +513,8 @@ __global__ void __launch_bounds__(256) k_llf3d(Step s, G3 g) {
 // Work-list form: instance i owns blocks first[i] .. first[i+1]-1.
 __global__ void __launch_bounds__(256) k_batch_llf3d(const fvl::BatchItem* items, const int* first, int n) {
     const int b = blockIdx.x;
-    int lo = 0, hi = n - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) / 2;
-        if (first[mid] <= b) lo = mid;
-        else hi = mid - 1;
-    }
-    const fvl::BatchItem& it = items[lo];
+    const int lo = fvl::owner(first, n, b);
+    const fvl::BatchItem& it = fvl::item_at(items, lo);
     const G3 g{it.grid.nx, it.grid.ny, it.grid.nz, it.grid.bc[0], it.grid.bc[1], it.grid.bc[2]};
     const int local = b - first[lo], nblk = first[lo + 1] - first[lo];
     llf3d_cells(it.step, g, local * 256L + threadIdx.x, nblk * 256L, it.llf_first + local);
@@ -723,15 +707,35 @@ hipError_t llf3d(const Grid& g, const Step& s, hipStream_t st) {
     return hipGetLastError();
 }
 
-hipError_t batch3d(const fvl::BatchItem* items, const int* first, int n, int blocks, int bx, int by,
-                   hipStream_t st) {
-    if (blocks <= 0) return hipSuccess;
-    if (bx == 64 && by == 4) k_batch3d<64, 4><<<blocks, dim3(64, 4), 0, st>>>(items, first, n);
-    else if (bx == 64 && by == 8) k_batch3d<64, 8><<<blocks, dim3(64, 8), 0, st>>>(items, first, n);
-    else if (bx == 32 && by == 8) k_batch3d<32, 8><<<blocks, dim3(32, 8), 0, st>>>(items, first, n);
-    else if (bx == 32 && by == 16) k_batch3d<32, 16><<<blocks, dim3(32, 16), 0, st>>>(items, first, n);
+template <int BX, int BY, bool SPEC>
+static void launch_batch3d(const fvl::BatchItem* items, const int* first, int n, int blocks, int limiter,
+                           hipStream_t st) {
+    const dim3 block(BX, BY);
+    switch (limiter) {
+        case VANLEER: k_batch3d<VANLEER, BX, BY, SPEC><<<blocks, block, 0, st>>>(items, first, n); break;
+        case ALBADA: k_batch3d<ALBADA, BX, BY, SPEC><<<blocks, block, 0, st>>>(items, first, n); break;
+        case MINBEE: k_batch3d<MINBEE, BX, BY, SPEC><<<blocks, block, 0, st>>>(items, first, n); break;
+        case SUPERBEE: k_batch3d<SUPERBEE, BX, BY, SPEC><<<blocks, block, 0, st>>>(items, first, n); break;
+        default: k_batch3d<UNOS, BX, BY, SPEC><<<blocks, block, 0, st>>>(items, first, n); break;
+    }
+}
+
+template <bool SPEC>
+static hipError_t batch_shape(const fvl::BatchItem* items, const int* first, int n, int blocks, int bx, int by,
+                              int limiter, hipStream_t st) {
+    if (bx == 64 && by == 4) launch_batch3d<64, 4, SPEC>(items, first, n, blocks, limiter, st);
+    else if (bx == 64 && by == 8) launch_batch3d<64, 8, SPEC>(items, first, n, blocks, limiter, st);
+    else if (bx == 32 && by == 8) launch_batch3d<32, 8, SPEC>(items, first, n, blocks, limiter, st);
+    else if (bx == 32 && by == 16) launch_batch3d<32, 16, SPEC>(items, first, n, blocks, limiter, st);
     else return hipErrorInvalidValue;
     return hipGetLastError();
+}
+
+hipError_t batch3d(const fvl::BatchItem* items, const int* first, int n, int blocks, int bx, int by, int limiter,
+                   int spec, hipStream_t st) {
+    if (blocks <= 0) return hipSuccess;
+    return spec ? batch_shape<true>(items, first, n, blocks, bx, by, limiter, st)
+                : batch_shape<false>(items, first, n, blocks, bx, by, limiter, st);
 }
 
 hipError_t batch_llf3d(const fvl::BatchItem* items, const int* first, int n, int blocks, hipStream_t st) {
@@ -743,13 +747,8 @@ hipError_t batch_llf3d(const fvl::BatchItem* items, const int* first, int n, int
 // CFL maxima of each instance's current state (the batch's first step).
 __global__ void __launch_bounds__(256) k_batch_cfl3d(const fvl::BatchItem* items, const int* first, int n) {
     const int b = blockIdx.x;
-    int lo = 0, hi = n - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) / 2;
-        if (first[mid] <= b) lo = mid;
-        else hi = mid - 1;
-    }
-    const fvl::BatchItem& it = items[lo];
+    const int lo = fvl::owner(first, n, b);
+    const fvl::BatchItem& it = fvl::item_at(items, lo);
     const int local = b - first[lo], nblk = first[lo + 1] - first[lo];
     const long N = (long)it.grid.nx * it.grid.ny * it.grid.nz;
     const real* U = (const real*)it.step.U;

@@ -121,6 +121,26 @@ struct BatchCtl {
     int cur;             // which of the two state buffers holds the committed state
 };
 
+// Instance of block b in a work list: first[i] <= b < first[i+1].
+__device__ __forceinline__ int owner(const int* first, int n, int b) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) / 2;
+        if (first[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Work-list kernels read their item through the constant address space:
+// items change only between launches (host uploads, the post kernel), and
+// constant loads may be issued again where needed instead of holding every
+// field of the item in registers for the whole kernel.
+__device__ __forceinline__ const BatchItem& item_at(const BatchItem* items, int i) {
+    typedef const __attribute__((address_space(4))) BatchItem CItem;
+    return *(const BatchItem*)((const CItem*)items + i);
+}
+
 struct React {
     const void* U;     // input state
     void* Uout;
@@ -175,19 +195,19 @@ struct Launch {
     hipError_t convert_out(const void* src, double* dst, long n, hipStream_t st);         \
     /* batches: items/first in device memory; first[n] = total blocks */                 \
     hipError_t batch3d(const fvl::BatchItem* items, const int* first, int n, int blocks,   \
-                       int bx, int by, hipStream_t st);                                    \
+                       int bx, int by, int limiter, int spec, hipStream_t st);             \
     hipError_t batch_llf3d(const fvl::BatchItem* items, const int* first, int n,          \
                            int blocks, hipStream_t st);                                    \
     hipError_t batch_cfl3d(const fvl::BatchItem* items, const int* first, int n,          \
                            int blocks, hipStream_t st);                                    \
     hipError_t batch2d(const fvl::BatchItem* items, const int* first, int n, int blocks,   \
-                       hipStream_t st);                                                    \
+                       int limiter, hipStream_t st);                                       \
     hipError_t batch_llf2d(const fvl::BatchItem* items, const int* first, int n,          \
                            int blocks, hipStream_t st);                                    \
     hipError_t batch_cfl2d(const fvl::BatchItem* items, const int* first, int n,          \
                            int blocks, hipStream_t st);                                    \
     hipError_t batch1d(const fvl::BatchItem* items, const int* first, int n, int blocks,   \
-                       hipStream_t st);                                                    \
+                       int limiter, hipStream_t st);                                       \
     hipError_t batch_llf1d(const fvl::BatchItem* items, const int* first, int n,          \
                            int blocks, hipStream_t st);                                    \
     hipError_t batch_cfl1d(const fvl::BatchItem* items, const int* first, int n,          \

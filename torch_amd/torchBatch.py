@@ -21,6 +21,7 @@ import numpy as np
 
 from . import _abi
 from .torchExecution import DeviceExecution
+from .torchSolver3D import config as _config
 
 
 class Batch:
@@ -57,6 +58,33 @@ class Batch:
             self._meta[ident] = (c["dimension"] + 2, sim.steps)
         return ident
 
+    def add_problem(self, primitive, lengths, *, t_end, cfl=0.4, gamma=1.4, boundaries=None, limiter="VanLeer",
+                    max_steps=100000, time=0.0, inlet_primitive=None, controller="Sound Speed Controlled",
+                    variant="Variante 1 - Original", wavespeed="Pressure Based"):
+        """Admit an inert problem from primitives without building a session.
+
+        Takes the arguments of ``Simulation`` (torchSession.py:24-158) and
+        converts the primitives to the conservative state the same way.
+        """
+        primitive = np.asarray(primitive, dtype=float)
+        dim = primitive.ndim - 1
+        if dim not in (1, 2, 3) or primitive.shape[0] != dim + 2:
+            raise ValueError("Require primitives with shape (dimension+2, ... reversed spatial axes)")
+        counts = list(primitive.shape[1:][::-1])
+        lengths = list(np.atleast_1d(lengths))
+        bcs = list(boundaries) if boundaries is not None else ["Open"] * dim
+        cfg = _config(counts + [1] * (3 - dim), None, lengths + [1.0] * (3 - dim), dim=dim, gamma=gamma,
+                      limiter=limiter, bcs=bcs + ["Open"] * (3 - dim), precision=self.precision, cfl=cfl,
+                      inlet=inlet_primitive, controller=_abi.CONTROLLERS.index(controller),
+                      variant=_abi.VARIANTS.index(variant), wavespeed=_abi.WAVESPEEDS.index(wavespeed))
+        state = primitive.copy()
+        state[1:-1] *= primitive[0]
+        state[-1] = primitive[-1] / (gamma - 1) + 0.5 * primitive[0] * np.sum(primitive[1:-1] ** 2, axis=0)
+        ident = self._batch.add(cfg, state, time, t_end, max_steps)
+        if ident is not None:
+            self._meta[ident] = (dim + 2, 0)
+        return ident
+
     def footprint(self, sim):
         cfg = sim._make_config()
         return _abi.DeviceBatch.footprint(cfg, _abi.precision_code(self.precision))
@@ -79,7 +107,12 @@ class Batch:
         self._batch.set_tiles(block_x, block_y, zchunk)
 
     def profile(self, enable=True):
+        """Enable replay timing; returns (ms, steps) since the last call."""
         return self._batch.profile(enable)
+
+    def time_pass(self, reps=5):
+        """Average ms of one 3D second-order work-list launch (nothing advances)."""
+        return self._batch.time_pass(reps)
 
     # -- results -------------------------------------------------------------
     def info(self, ident):
