@@ -20,6 +20,9 @@ workload's length). The JSON line reports:
   plain method driven with the same dt sequence;
 - batched: the uneven mix (256 seeded 1D/2D/3D two-state problems, 100
   steps each, FP32 and FP64 batches) with its throughput and FP32 error;
+- served: seeded 3D ignition requests through the continuous-batching
+  server (closed loop for capacity, then Poisson arrivals at 80% of it):
+  p50/p99 time-to-solution and aggregate cell-updates/s;
 - cpu_baseline: the NumPy oracle (a restatement of the reference's CPU path)
   timed on this host on a bounded sample of the same workload.
 
@@ -159,6 +162,38 @@ def run_uneven(seed, count, steps, precisions=("fp32", "fp64"), tile=None):
     return out, err, sum(cells.values())
 
 
+def run_served(seed, count, t_run, budget_gib, round_steps, precision="fp32"):
+    """Served reactive ignition solves: a closed-loop run (all requests at
+    t=0) measures capacity, then Poisson arrivals at 80% of it give the
+    p50/p99 time-to-solution."""
+    from torch_amd.torchServe import Server, ignition_request, summary
+    from torch_amd.workloads import served_requests
+
+    specs = served_requests(seed=seed, count=count, t_run=t_run)
+    budget = int(budget_gib * (1 << 30))
+    out = {}
+    server = Server(precision=precision, hbm_budget=budget, round_steps=round_steps)
+    closed = summary(server.serve([ignition_request(s, 0.0) for s in specs]))
+    server.close()
+    capacity = closed["completed"] / closed["makespan_s"]
+    print(f"# served closed loop: {closed['makespan_s']:.2f} s for {count} requests, "
+          f"{closed['cell_updates_per_s']:.4g} cell-updates/s", file=sys.stderr, flush=True)
+    rng = np.random.default_rng(seed + 1)
+    rate = 0.8 * capacity
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=count))
+    arrivals -= arrivals[0]
+    server = Server(precision=precision, hbm_budget=budget, round_steps=round_steps)
+    opened = summary(server.serve([ignition_request(s, a) for s, a in zip(specs, arrivals)]))
+    server.close()
+    print(f"# served open loop at {rate:.2f} req/s: p50 {opened['p50_s']:.3f} s, p99 {opened['p99_s']:.3f} s",
+          file=sys.stderr, flush=True)
+    out["closed_loop"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in closed.items()}
+    out["open_loop"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in opened.items()}
+    out["arrival_rate_per_s"] = round(rate, 3)
+    out["capacity_per_s"] = round(capacity, 3)
+    return out
+
+
 def roofline(n, bytes_per_cell, kernel_ms):
     achieved = bytes_per_cell * n ** 3 / (kernel_ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -229,7 +264,7 @@ def copy_calibration(mib=1024, reps=3):
     print(json.dumps({"copy_mib": mib, "reps": reps, "GBps_rw": round(2 * n * reps / wall / 1e9, 1)}))
 
 
-LEGS = ("fp64", "fp32", "batched", "error", "uneven", "cpu")
+LEGS = ("fp64", "fp32", "batched", "error", "uneven", "served", "cpu")
 
 
 def _tile(text):
@@ -256,6 +291,11 @@ def main(argv=None):
     ap.add_argument("--batch-steps", type=int, default=100)
     ap.add_argument("--batch-seed", type=int, default=508)
     ap.add_argument("--batch-tile", type=_tile, default=None, help="BX,BY,ZCHUNK of the mix (skips the tuner)")
+    ap.add_argument("--serve-count", type=int, default=64, help="served ignition requests")
+    ap.add_argument("--serve-seed", type=int, default=355)
+    ap.add_argument("--serve-t-run", type=float, default=0.1, help="run length of each served solve")
+    ap.add_argument("--serve-budget-gib", type=float, default=4.0, help="HBM budget of the served batch")
+    ap.add_argument("--serve-round", type=int, default=4, help="steps between admission checks")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device copies of MIB MiB (counter calibration) and exit")
     args = ap.parse_args(argv)
@@ -302,6 +342,15 @@ def main(argv=None):
         }
         if r64:
             uneven["fp32_over_fp64_plain_sod"] = round(res["fp32"]["value"] / r64["value"], 3)
+    served = None
+    if "served" in legs:
+        served = run_served(args.serve_seed, args.serve_count, args.serve_t_run, args.serve_budget_gib,
+                            args.serve_round)
+        served["workload"] = (
+            f"{args.serve_count} seeded (rng {args.serve_seed}) 3D ignition solves, n^3 with n in 32..160, "
+            f"Arrhenius (ZND for ~10%), Ea/RT0 in [20,30], Q/RT0 in [10,50], Gaussian hot spot T/T0 in "
+            f"[1.5,3], run length {args.serve_t_run} (induction ~0.1 of it), VanLeer, CFL 0.4, open "
+            f"boundaries; FP32 batch, HBM budget {args.serve_budget_gib} GiB, {args.serve_round} steps/round")
     if "cpu" in legs:
         cpu = cpu_baseline()
     head = r32 or r32s
@@ -327,6 +376,7 @@ def main(argv=None):
         "fp32_over_fp64": round(head["value"] / r64["value"], 3) if head and r64 else None,
         "fp32_l1_rel": [float(f"{e:.3e}") for e in err] if err else None,
         "batched": uneven,
+        "served": served,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

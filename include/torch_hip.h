@@ -135,13 +135,15 @@ int torch_hllc_faces(torch_ctx* ctx, int precision, int nv, int direction, long 
 /* ---------------------------------------------------------------------------
  * Batches. No counterpart in the reference, where every Simulation steps on
  * its own (torchSession.py:374-522) and a batch of problems is a Python loop
- * over sessions. Here many inert instances (any dimension, size, limiter,
- * boundaries, gamma, CFL and end time) live on the device together and one
- * torch_batch_run step advances all of them: a fixed sequence of work-list
- * launches, captured once in a HIP graph and replayed. Each instance keeps
- * the single-session arithmetic and its own dt, time, step count and
- * boundary integral (the bookkeeping of Simulation.step, done on the
- * device), so its trajectory is the one torch_state_step gives alone.
+ * over sessions. Here many instances (inert of any dimension, reactive 3D;
+ * any size, limiter, boundaries, gamma, CFL and end time) live on the device
+ * together and one torch_batch_run step advances all of them: a fixed
+ * sequence of work-list launches, captured once in a HIP graph and replayed.
+ * Each instance keeps the single-session arithmetic and its own dt, time,
+ * step count, boundary and source integrals and reactive dt halvings (the
+ * bookkeeping of Simulation.step, done on the device), so its trajectory is
+ * the one torch_state_step gives alone. A reactive step whose first
+ * chemistry half-step fails the CFL test takes one more replay per halving.
  * Instances may be added and removed between runs (continuous batching);
  * the batch caps its device memory at hbm_budget bytes (0: no cap).
  * ------------------------------------------------------------------------- */
@@ -156,7 +158,9 @@ typedef struct torch_batch_progress {
     int active;          /* still stepping */
     int completed;       /* time == t_end */
     long cells;          /* cells of the instance */
+    int retries;         /* reactive dt halvings so far */
     double outward[5];   /* accumulated boundary integral */
+    double source[5];    /* accumulated source integral (reactive) */
     double wave_max[3];  /* CFL maxima of the committed state */
 } torch_batch_progress;
 
@@ -164,18 +168,21 @@ typedef struct torch_batch_progress {
 int torch_batch_footprint(const torch_config* config, int precision, size_t* bytes);
 int torch_batch_create(torch_ctx* ctx, int precision, size_t hbm_budget, torch_batch** out);
 int torch_batch_destroy(torch_batch* batch);
-/* Admits an inert instance (Simulation(...) followed by run(t_end) with at
- * most max_steps steps, torchSession.py:19-183, 563-600) with state U in the
- * reference layout at `time`. TORCH_BUDGET when it does not fit now. */
-int torch_batch_add(torch_batch* batch, const torch_config* config, const double* U, double time, double t_end,
-                    int max_steps, int* id);
+/* Admits an instance (Simulation(...) followed by run(t_end) with at most
+ * max_steps steps, torchSession.py:19-183, 563-600) with state U in the
+ * reference layout at `time`: inert with a scalar gamma in 1D, 2D or 3D, or
+ * reactive in 3D (gamma_mode 1; lam and, for ZND, beta of the cell shape;
+ * beta NULL means ones). TORCH_BUDGET when it does not fit now. */
+int torch_batch_add(torch_batch* batch, const torch_config* config, const double* U, const double* lam,
+                    const double* beta, double time, double t_end, int max_steps, int* id);
 int torch_batch_remove(torch_batch* batch, int id);
 /* Advances every active instance by up to `steps` steps, then waits. An
  * instance stops at t_end, at max_steps, or on the error its status names
  * (its committed state is then the last good one). */
 int torch_batch_run(torch_batch* batch, int steps);
 int torch_batch_info(torch_batch* batch, int id, torch_batch_progress* info);
-int torch_batch_download(torch_batch* batch, int id, double* U);
+/* Committed state (and, reactive, lambda and beta; NULL skips) of an instance. */
+int torch_batch_download(torch_batch* batch, int id, double* U, double* lam, double* beta);
 /* Autotuner: times the 3D second-order work list over the batch's current
  * instances for every supported tile shape and z chunk, keeps the fastest. */
 int torch_batch_tune(torch_batch* batch, int reps, int* block_x, int* block_y, int* zchunk, double* best_ms);

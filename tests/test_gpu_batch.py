@@ -236,3 +236,74 @@ def test_batch_commit_into_session():
         ref.step()
         assert np.array_equal(s.state, ref.state)
     batch.close()
+
+
+IGNITION = {"q_heat": 3e9, "pre_exponential": 1e6}
+
+
+def _ignition(precision, model, shape=(8, 10, 12), constants=IGNITION, bcs=("Open", "Periodic", "Reflective"),
+              hot=1e8):
+    """Hot, fast-reacting mixture with a large heat release: the heated
+    half-step fails the CFL test every step (one dt halving each)."""
+    from torch_amd.torchSession import Simulation
+    nz, ny, nx = shape
+    z, y, x = np.meshgrid((np.arange(nz) + 0.5) / nz, (np.arange(ny) + 0.5) / ny, (np.arange(nx) + 0.5) / nx,
+                          indexing="ij")
+    prim = np.zeros((5,) + shape)
+    prim[0] = 1.0
+    prim[4] = hot * (1 + 0.5 * np.exp(-((x - 0.3) ** 2 + (y - 0.5) ** 2 + (z - 0.5) ** 2) / 0.05))
+    lam0 = 1.0 - 0.2 * x
+    return Simulation(prim, [1.0, 1.0, 1.0], t_end=1e-5, reactive=True, kinetic_model=model, lambda_init=lam0,
+                      boundaries=list(bcs), precision=precision, constants=constants)
+
+
+def _check_reactive(batch, ident, ref):
+    info = batch.info(ident)
+    assert info["status"] == 0
+    assert info["steps"] == ref.steps and info["time"] == ref.time
+    assert np.array_equal(batch.state(ident), ref.state)
+    lam, beta = batch.fractions(ident)
+    assert np.array_equal(lam, ref.lam)
+    if ref.beta is not None:
+        assert np.array_equal(beta, ref.beta)
+    assert np.array_equal(info["boundary_integral"], ref.boundary_integral)
+    assert np.array_equal(info["source_integral"], ref.source_integral)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_reactive_batch_matches_sessions(precision):
+    """Arrhenius and ZND ignition (with dt halvings) mixed with inert work."""
+    from torch_amd.torchBatch import Batch
+    sims = [_ignition(precision, "arrhenius"), _ignition(precision, "znd", shape=(6, 9, 7)),
+            _ignition(precision, "arrhenius", shape=(5, 6, 40), bcs=("Reflective", "Open", "Open"))]
+    inert = _sessions(precision, seed=14, specs=SPECS[5:6] + SPECS[9:10])
+    steps = 5
+    refs = [_alone(s, steps) for s in sims + inert]
+    batch = Batch(precision=precision)
+    ids = [batch.add(s, max_steps=steps) for s in sims + inert]
+    batch.run(steps + 40)  # rejected chemistry attempts take a replay each
+    retries = 0
+    for s, i, ref in zip(sims + inert, ids, refs):
+        if s.config["reactive"]:
+            _check_reactive(batch, i, ref)
+            retries += batch.info(i)["retries"]
+        else:
+            _check_same(batch, i, ref)
+    assert retries > 0  # the dt-halving path ran inside the batch
+    batch.close()
+
+
+def test_reactive_batch_commit_and_continue():
+    from torch_amd.torchBatch import Batch
+    sim = _ignition("fp64", "znd")
+    ref = _alone(sim, 6)
+    batch = Batch(precision="fp64")
+    i = batch.add(sim, max_steps=3)
+    batch.run(30)
+    batch.commit(i, sim)
+    for _ in range(3):
+        sim.step()
+    assert sim.steps == ref.steps and sim.time == ref.time
+    assert np.array_equal(sim.state, ref.state) and np.array_equal(sim.lam, ref.lam)
+    assert np.array_equal(sim.source_integral, ref.source_integral)
+    batch.close()

@@ -61,8 +61,8 @@ class BatchProgress(ctypes.Structure):
     _fields_ = [
         ("time", ctypes.c_double), ("dt", ctypes.c_double), ("steps", ctypes.c_int),
         ("fallbacks", ctypes.c_int), ("status", ctypes.c_int), ("active", ctypes.c_int),
-        ("completed", ctypes.c_int), ("cells", ctypes.c_long), ("outward", ctypes.c_double * 5),
-        ("wave_max", ctypes.c_double * 3),
+        ("completed", ctypes.c_int), ("cells", ctypes.c_long), ("retries", ctypes.c_int),
+        ("outward", ctypes.c_double * 5), ("source", ctypes.c_double * 5), ("wave_max", ctypes.c_double * 3),
     ]
 
 
@@ -108,12 +108,12 @@ def load():
                                                          ctypes.POINTER(ctypes.c_size_t)]),
                 "torch_batch_create": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_size_t, ctypes.POINTER(vp)]),
                 "torch_batch_destroy": (ctypes.c_int, [vp]),
-                "torch_batch_add": (ctypes.c_int, [vp, ctypes.POINTER(Config), _dp, ctypes.c_double,
+                "torch_batch_add": (ctypes.c_int, [vp, ctypes.POINTER(Config), _dp, _dp, _dp, ctypes.c_double,
                                                    ctypes.c_double, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
                 "torch_batch_remove": (ctypes.c_int, [vp, ctypes.c_int]),
                 "torch_batch_run": (ctypes.c_int, [vp, ctypes.c_int]),
                 "torch_batch_info": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(BatchProgress)]),
-                "torch_batch_download": (ctypes.c_int, [vp, ctypes.c_int, _dp]),
+                "torch_batch_download": (ctypes.c_int, [vp, ctypes.c_int, _dp, _dp, _dp]),
                 "torch_batch_tune": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                                     ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                                     _dp]),
@@ -298,20 +298,25 @@ class DeviceBatch:
         check(load().torch_batch_footprint(ctypes.byref(config), int(precision), ctypes.byref(out)))
         return out.value
 
-    def add(self, config, state, time, t_end, max_steps):
+    def add(self, config, state, time, t_end, max_steps, lam=None, beta=None):
         """Admit an instance; returns its id, or None when the HBM budget is full."""
         state = np.ascontiguousarray(state, dtype=np.float64)
         counts = [config.n[d] for d in range(config.dim)]
         shape = (config.dim + 2,) + tuple(counts[::-1])
         if state.shape != shape:
             raise ValueError(f"state must have shape {shape}")
+        lam = None if lam is None else np.ascontiguousarray(lam, dtype=np.float64)
+        beta = None if beta is None else np.ascontiguousarray(beta, dtype=np.float64)
+        for a in (lam, beta):
+            if a is not None and a.shape != shape[1:]:
+                raise ValueError(f"lambda/beta must have shape {shape[1:]}")
         out = ctypes.c_int()
-        code = load().torch_batch_add(self.handle, ctypes.byref(config), ptr(state), float(time), float(t_end),
-                                      int(max_steps), ctypes.byref(out))
+        code = load().torch_batch_add(self.handle, ctypes.byref(config), ptr(state), ptr(lam), ptr(beta),
+                                      float(time), float(t_end), int(max_steps), ctypes.byref(out))
         if code == BUDGET:
             return None
         check(code)
-        self.shapes[out.value] = shape
+        self.shapes[out.value] = (shape, bool(config.reactive), config.kinetic_model == 1)
         return out.value
 
     def remove(self, ident):
@@ -327,9 +332,13 @@ class DeviceBatch:
         return out
 
     def download(self, ident):
-        state = np.empty(self.shapes[ident])
-        check(load().torch_batch_download(self.handle, int(ident), ptr(state)))
-        return state
+        """(state, lam, beta) of an instance; lam/beta None unless reactive (ZND)."""
+        shape, reactive, znd = self.shapes[ident]
+        state = np.empty(shape)
+        lam = np.empty(shape[1:]) if reactive else None
+        beta = np.empty(shape[1:]) if znd else None
+        check(load().torch_batch_download(self.handle, int(ident), ptr(state), ptr(lam), ptr(beta)))
+        return state, lam, beta
 
     def tune(self, reps=3):
         bx, by, zc, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_double()

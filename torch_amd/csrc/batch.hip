@@ -32,7 +32,8 @@ struct Inst {
     fvl::Grid grid;
     long ncell = 0;
     int nf = 0;
-    void* U[2] = {nullptr, nullptr};
+    void* U[2] = {nullptr, nullptr};   // inert: the two state buffers
+    void* slot[3][4] = {};             // reactive: [U, lambda, beta] x 4 rotating buffers
     void* scratch = nullptr;  // 1D interface fluxes
     size_t bytes = 0;
     double volume = 0;
@@ -40,7 +41,7 @@ struct Inst {
     bool fresh = true;  // first dt not derived yet
     // layout of the current build
     int tiles[3] = {0, 0, 0};
-    int nblk = 0, nllf = 0, ncfl = 0;  // blocks of the three passes
+    int nblk = 0, nllf = 0, ncfl = 0, nreact = 0;  // blocks of the passes
     long pslot = 0;                    // first partial slot
 };
 
@@ -53,9 +54,10 @@ struct Table {
 size_t state_bytes(const torch_config& c, size_t elem) {
     long ncell = (long)c.n[0] * (c.dim >= 2 ? c.n[1] : 1) * (c.dim >= 3 ? c.n[2] : 1);
     size_t b = 2 * (size_t)(c.dim + 2) * ncell * elem;
+    if (c.reactive) b = 4 * (size_t)(c.dim + 2 + 1 + (c.kinetic_model == 1)) * ncell * elem;
     if (c.dim == 1) b += elem * (size_t)(3 * (c.n[0] + 1) + 64);
     // partial slots (default tiles) and bookkeeping
-    size_t slots = (size_t)(ncell / 256 + 8) * 3;
+    size_t slots = (size_t)(ncell / 256 + 8) * (c.reactive ? 5 : 3);
     return b + slots * fvl::NSLOT * sizeof(double) + sizeof(fvl::BatchItem) + sizeof(fvl::BatchCtl);
 }
 
@@ -71,6 +73,8 @@ struct torch_batch {
     bool dirty = true;
     int seg[4] = {0, 0, 0, 0};  // [3D | 2D | 1D] segment starts, seg[3] = n
     int lim[3][6] = {};         // limiter sub-segment starts within each dimension segment
+    int rseg = 0;               // first reactive instance of the 3D segment (reactive ones last)
+    int rlim[6] = {};           // limiter sub-segments of the reactive 3D instances
     fvl::BatchItem* items = nullptr;
     fvl::BatchItem* pitems = nullptr;  // copies of the items of new instances (first dt)
     fvl::BatchCtl* ctl = nullptr;
@@ -79,7 +83,9 @@ struct torch_batch {
     int* tables = nullptr;
     fvl::BatchCtl* host_ctl = nullptr;  // pinned
     size_t cap_n = 0, cap_partials = 0, cap_tables = 0, cap_pitems = 0;
-    Table hyd[3][5];          // second-order tables per dimension and limiter
+    Table hyd[3][5];          // second-order tables per dimension and limiter (inert)
+    Table hydr[5];            // second-order tables of reactive 3D instances per limiter
+    Table rx;                 // chemistry tables of reactive 3D instances
     Table l3, l2, l1, c1;     // first-order tables per dimension, 1D CFL
     Table p3, p2, p1;         // first-dt tables over pitems
     int pseg[4] = {0, 0, 0, 0};
@@ -98,6 +104,8 @@ int seg_of(int dim) { return dim == 3 ? 0 : (dim == 2 ? 1 : 2); }
 void free_inst(Inst* p) {
     (void)hipFree(p->U[0]);
     (void)hipFree(p->U[1]);
+    for (auto& kind : p->slot)
+        for (void* q : kind) (void)hipFree(q);
     (void)hipFree(p->scratch);
     delete p;
 }
@@ -124,6 +132,7 @@ void layout(torch_batch* b, Inst* p) {
         p->nblk = p->tiles[0] * p->tiles[1] * p->tiles[2];
         p->nllf = o->llf3d_blocks(g);
         p->ncfl = o->cfl_blocks(g);
+        p->nreact = p->cfg.reactive ? o->react_blocks(g) : 0;
     } else if (g.dim == 2) {
         const int rows = o->strip2d_rows(g);
         p->tiles[0] = (g.nx + 61) / 62;
@@ -168,12 +177,19 @@ int grow(void** p, size_t* cap, size_t bytes) {
     return rc;
 }
 
-// The second-order pass of every 3D instance (one launch per limiter).
+// The second-order pass of every 3D instance (one launch per limiter, the
+// reactive ones with species transport).
 hipError_t enqueue_3d(torch_batch* b, hipStream_t st) {
     for (int L = 0; L < 5; ++L) {
         const Table& t = b->hyd[0][L];
         hipError_t e = b->ops->batch3d(b->items + b->lim[0][L], b->tables + t.offset, t.count, t.blocks, b->bx, b->by,
                                        L, 0, st);
+        if (e != hipSuccess) return e;
+    }
+    for (int L = 0; L < 5; ++L) {
+        const Table& t = b->hydr[L];
+        hipError_t e = b->ops->batch3d(b->items + b->rlim[L], b->tables + t.offset, t.count, t.blocks, b->bx, b->by,
+                                       L, 1, st);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
@@ -185,9 +201,15 @@ hipError_t enqueue_step(torch_batch* b, hipStream_t st) {
     fvl::BatchItem* I3 = b->items + b->seg[0];
     fvl::BatchItem* I2 = b->items + b->seg[1];
     fvl::BatchItem* I1 = b->items + b->seg[2];
+    fvl::BatchItem* IR = b->items + b->rseg;
+    const int nr = b->seg[1] - b->rseg;
     hipError_t e;
+    // Reactive 3D: R(dt/2), then the CFL test of the heated state.
+    if ((e = o->batch_react(IR, T + b->rx.offset, b->rx.count, b->rx.blocks, 1, st))) return e;
+    if ((e = o->batch_check(IR, b->ctl + b->rseg, nr, st))) return e;
     if ((e = enqueue_3d(b, st))) return e;
     if ((e = o->batch_llf3d(I3, T + b->l3.offset, b->l3.count, b->l3.blocks, st))) return e;
+    if ((e = o->batch_react(IR, T + b->rx.offset, b->rx.count, b->rx.blocks, 2, st))) return e;
     for (int L = 0; L < 5; ++L) {
         const Table& t = b->hyd[1][L];
         if ((e = o->batch2d(b->items + b->lim[1][L], T + t.offset, t.count, t.blocks, L, st))) return e;
@@ -241,6 +263,7 @@ int rebuild(torch_batch* b) {
     std::stable_sort(v.begin(), v.end(), [](const Inst* x, const Inst* y) {
         int sx = seg_of(x->grid.dim), sy = seg_of(y->grid.dim);
         if (sx != sy) return sx < sy;
+        if (x->cfg.reactive != y->cfg.reactive) return x->cfg.reactive < y->cfg.reactive;
         return x->cfg.limiter < y->cfg.limiter;
     });
     const int n = (int)v.size();
@@ -250,18 +273,26 @@ int rebuild(torch_batch* b) {
         b->seg[s] = i;
     }
     b->seg[3] = n;
+    b->rseg = b->seg[0];
+    while (b->rseg < b->seg[1] && !v[b->rseg]->cfg.reactive) ++b->rseg;
     for (int s = 0; s < 3; ++s) {
+        const int end = s == 0 ? b->rseg : b->seg[s + 1];
         for (int L = 0; L <= 5; ++L) {
             int i = b->seg[s];
-            while (i < b->seg[s + 1] && v[i]->cfg.limiter < L) ++i;
+            while (i < end && v[i]->cfg.limiter < L) ++i;
             b->lim[s][L] = i;
         }
+    }
+    for (int L = 0; L <= 5; ++L) {
+        int i = b->rseg;
+        while (i < b->seg[1] && v[i]->cfg.limiter < L) ++i;
+        b->rlim[L] = i;
     }
     long slots = 0;
     for (Inst* p : v) {
         layout(b, p);
         p->pslot = slots;
-        slots += (p->grid.dim == 1 ? 0 : p->nblk + p->nllf) + p->ncfl;
+        slots += (p->grid.dim == 1 ? 0 : p->nblk + p->nllf) + p->ncfl + 2 * p->nreact;
     }
     std::vector<Inst*> fresh;
     for (Inst* p : v)
@@ -275,6 +306,8 @@ int rebuild(torch_batch* b) {
     std::vector<int> tb;
     for (int s = 0; s < 3; ++s)
         for (int L = 0; L < 5; ++L) b->hyd[s][L] = table(tb, v, b->lim[s][L], b->lim[s][L + 1], &Inst::nblk);
+    for (int L = 0; L < 5; ++L) b->hydr[L] = table(tb, v, b->rlim[L], b->rlim[L + 1], &Inst::nblk);
+    b->rx = table(tb, v, b->rseg, b->seg[1], &Inst::nreact);
     b->l3 = table(tb, v, b->seg[0], b->seg[1], &Inst::nllf);
     b->l2 = table(tb, v, b->seg[1], b->seg[2], &Inst::nllf);
     b->l1 = table(tb, v, b->seg[2], b->seg[3], &Inst::nllf);
@@ -336,9 +369,31 @@ int rebuild(torch_batch* b) {
         it.llf_first = p->grid.dim == 1 ? 0 : p->nblk;
         it.cfl_first = p->grid.dim == 1 ? 0 : p->nblk + p->nllf;
         it.scratch = p->scratch;
+        if (p->cfg.reactive) {
+            const torch_config& c = p->cfg;
+            s.want_cfl = 0;  // the maxima come from the second chemistry half-step
+            s.nq = c.kinetic_model == 1 ? 2 : 1;
+            it.reactive = 1;
+            it.znd = c.kinetic_model == 1;
+            for (int k = 0; k < 3; ++k)
+                for (int j = 0; j < 4; ++j) it.slot[k][j] = p->slot[k][j];
+            for (fvl::React* r : {&it.r1, &it.r2}) {
+                r->model = c.kinetic_model;
+                r->q_heat = c.q_heat;
+                r->activation_energy = c.activation_energy;
+                r->pre_exponential = c.pre_exponential;
+                r->flags = s.flags;
+                r->partials = s.partials;
+                r->want_cfl = 1;
+            }
+            it.r1_first = p->nblk + p->nllf + p->ncfl;
+            it.r2_first = it.r1_first + p->nreact;
+            fvl::set_reactive_slots(it, p->ctl.cur);
+        }
         p->ctl.nblk = p->nblk;
         p->ctl.nllf = p->nllf;
         p->ctl.ncfl = p->ncfl;
+        p->ctl.nreact = p->nreact;
         b->host_ctl[i] = p->ctl;
     }
     hipStream_t st = b->ctx->stream;
@@ -415,13 +470,15 @@ int torch_batch_destroy(torch_batch* b) {
     return TORCH_OK;
 }
 
-int torch_batch_add(torch_batch* b, const torch_config* config, const double* U, double time, double t_end,
-                    int max_steps, int* id) {
+int torch_batch_add(torch_batch* b, const torch_config* config, const double* U, const double* lam,
+                    const double* beta, double time, double t_end, int max_steps, int* id) {
     if (!b || !config || !U || !id) return fail(TORCH_INVALID, "null argument");
     const torch_config& c = *config;
     int rc = check_config(c);
     if (rc) return rc;
-    if (c.reactive || c.gamma_mode) return fail(TORCH_INVALID, "batches hold inert scalar-gamma instances");
+    if (c.reactive ? c.dim != 3 : c.gamma_mode != 0)
+        return fail(TORCH_INVALID, "batches hold inert scalar-gamma instances and reactive 3D instances");
+    if (c.reactive && !lam) return fail(TORCH_INVALID, "reactive instance needs a lambda array");
     if (!isfinite(time) || !isfinite(t_end) || !(time < t_end)) return fail(TORCH_INVALID, "require time < t_end");
     if (max_steps <= 0) return fail(TORCH_INVALID, "max_steps must be positive");
     const size_t bytes = state_bytes(c, b->ops->elem);
@@ -436,10 +493,27 @@ int torch_batch_add(torch_batch* b, const torch_config* config, const double* U,
     p->nf = c.dim + 2;
     p->bytes = bytes;
     const size_t e = b->ops->elem;
-    rc = alloc(&p->U[0], p->nf * p->ncell * e);
-    if (!rc) rc = alloc(&p->U[1], p->nf * p->ncell * e);
-    if (!rc && c.dim == 1) rc = alloc(&p->scratch, b->ops->step1d_scratch(p->grid));
-    if (!rc) rc = upload_as(b->ops, b->ctx->stream, p->U[0], U, p->nf * p->ncell);
+    hipStream_t st = b->ctx->stream;
+    if (c.reactive) {
+        const bool znd = c.kinetic_model == 1;
+        for (int j = 0; j < 4 && !rc; ++j) {
+            rc = alloc(&p->slot[0][j], p->nf * p->ncell * e);
+            if (!rc) rc = alloc(&p->slot[1][j], p->ncell * e);
+            if (!rc && znd) rc = alloc(&p->slot[2][j], p->ncell * e);
+        }
+        if (!rc) rc = upload_as(b->ops, st, p->slot[0][0], U, p->nf * p->ncell);
+        if (!rc) rc = upload_as(b->ops, st, p->slot[1][0], lam, p->ncell);
+        if (!rc && znd) {
+            std::vector<double> ones;
+            if (!beta) ones.assign(p->ncell, 1.0);
+            rc = upload_as(b->ops, st, p->slot[2][0], beta ? beta : ones.data(), p->ncell);
+        }
+    } else {
+        rc = alloc(&p->U[0], p->nf * p->ncell * e);
+        if (!rc) rc = alloc(&p->U[1], p->nf * p->ncell * e);
+        if (!rc && c.dim == 1) rc = alloc(&p->scratch, b->ops->step1d_scratch(p->grid));
+        if (!rc) rc = upload_as(b->ops, st, p->U[0], U, p->nf * p->ncell);
+    }
     if (rc) {
         free_inst(p);
         return rc;
@@ -454,6 +528,7 @@ int torch_batch_add(torch_batch* b, const torch_config* config, const double* U,
     k.dim = c.dim;
     k.max_steps = max_steps;
     k.active = 2;
+    k.reactive = c.reactive != 0;
     p->id = b->next_id++;
     b->inst.push_back(p);
     b->used += bytes;
@@ -515,18 +590,28 @@ int torch_batch_info(torch_batch* b, int id, torch_batch_progress* info) {
     info->active = k.active != 0;
     info->completed = k.time == k.t_end;
     info->cells = p->ncell;
-    for (int f = 0; f < 5; ++f) info->outward[f] = k.outward[f];
+    info->retries = k.retries;
+    for (int f = 0; f < 5; ++f) {
+        info->outward[f] = k.outward[f];
+        info->source[f] = k.source[f];
+    }
     for (int m = 0; m < 3; ++m) info->wave_max[m] = k.maxima[m];
     return TORCH_OK;
 }
 
-int torch_batch_download(torch_batch* b, int id, double* U) {
+int torch_batch_download(torch_batch* b, int id, double* U, double* lam, double* beta) {
     if (!b || !U) return fail(TORCH_INVALID, "null argument");
     std::lock_guard<std::mutex> lock(b->ctx->mu);
     HIP_TRY(hipSetDevice(b->ctx->device));
     Inst* p = find(b, id);
     if (!p) return fail(TORCH_INVALID, "no such batch instance");
-    return download_as(b->ops, b->ctx->stream, U, p->U[p->ctl.cur], p->nf * p->ncell);
+    hipStream_t st = b->ctx->stream;
+    if (!p->cfg.reactive) return download_as(b->ops, st, U, p->U[p->ctl.cur], p->nf * p->ncell);
+    const int cur = p->ctl.cur;
+    int rc = download_as(b->ops, st, U, p->slot[0][cur], p->nf * p->ncell);
+    if (!rc && lam) rc = download_as(b->ops, st, lam, p->slot[1][cur], p->ncell);
+    if (!rc && beta && p->slot[2][cur]) rc = download_as(b->ops, st, beta, p->slot[2][cur], p->ncell);
+    return rc;
 }
 
 int torch_batch_tune(torch_batch* b, int reps, int* block_x, int* block_y, int* zchunk, double* best_ms) {

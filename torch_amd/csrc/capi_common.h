@@ -66,6 +66,8 @@ struct Ops {
     hipError_t (*batch_llf1d)(const fvl::BatchItem*, const int*, int, int, hipStream_t);
     hipError_t (*batch_cfl1d)(const fvl::BatchItem*, const int*, int, int, int, hipStream_t);
     hipError_t (*batch_post)(fvl::BatchItem*, fvl::BatchCtl*, int, int, hipStream_t);
+    hipError_t (*batch_react)(const fvl::BatchItem*, const int*, int, int, int, hipStream_t);
+    hipError_t (*batch_check)(fvl::BatchItem*, fvl::BatchCtl*, int, hipStream_t);
     int (*strip2d_rows)(const fvl::Grid&);
     int (*step1d_tiles)(const fvl::Grid&);
 };

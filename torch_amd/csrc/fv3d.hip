@@ -311,7 +311,7 @@ __device__ __forceinline__ void step3d_tile(const Step& s, const G3& g, int tbx,
             for (int j = 0; j < nq; ++j) {
                 real v = rq[j] - c2 * (sfz[j] - sfzp[j]);
                 real f = v / o[0];
-                if (f < (real)-1e-10 || f > (real)(1 + 1e-10) || f != f) bad |= fvl::F_SPEC_BAD;
+                if (f < -kSpecTol || f > (real)1 + kSpecTol || f != f) bad |= fvl::F_SPEC_BAD;
                 ((real*)s.qout[j])[i] = rmin(rmax(f, (real)0), (real)1);
             }
         }
@@ -499,7 +499,7 @@ __device__ __forceinline__ void llf3d_cells(const Step& s, const G3& g, long sta
         if (s.want_cfl) wave3(o, (double)s.gamma, mx);
         for (int j = 0; j < s.nq; ++j) {
             real f = rqv[j] / o[0];
-            if (f < (real)-1e-10 || f > (real)(1 + 1e-10) || f != f) bad |= fvl::F_SPEC_BAD_LLF;
+            if (f < -kSpecTol || f > (real)1 + kSpecTol || f != f) bad |= fvl::F_SPEC_BAD_LLF;
             ((real*)s.qout[j])[i] = rmin(rmax(f, (real)0), (real)1);
         }
     }
@@ -553,10 +553,11 @@ __device__ __forceinline__ void advance(const Kin& k, double lam0, double beta0,
     e = n2 + k.q * n0 * (lam0 - lam);
 }
 
-__global__ void __launch_bounds__(256) k_react(fvl::React r, Grid g) {
+// Chemistry of cell i (one per thread) with half-step duration `dt`;
+// partial slot pblk of r.partials.
+__device__ __forceinline__ void react_cells(const fvl::React& r, const Grid& g, double dt, long i, long pblk) {
     const int nf = g.dim + 2;
     const long N = (long)g.nx * g.ny * g.nz;
-    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
     double acc[5] = {0, 0, 0, 0, 0}, mx[3] = {0, 0, 0}, src = 0;
     int bad = 0;
     if (i < N) {
@@ -576,8 +577,8 @@ __global__ void __launch_bounds__(256) k_react(fvl::React r, Grid g) {
         if (!ok) bad |= fvl::F_REACT_BAD;
         Kin k{r.model, r.q_heat, r.activation_energy, r.pre_exponential};
         double lm, bm, em, lam, beta, e;
-        advance(k, lam0, beta0, u[0], n2, rate_at(k, lam0, u[0], u[1], n2), 0.5 * r.dt, lm, bm, em);
-        advance(k, lam0, beta0, u[0], n2, rate_at(k, lm, u[0], u[1], em), r.dt, lam, beta, e);
+        advance(k, lam0, beta0, u[0], n2, rate_at(k, lam0, u[0], u[1], n2), 0.5 * dt, lm, bm, em);
+        advance(k, lam0, beta0, u[0], n2, rate_at(k, lm, u[0], u[1], em), dt, lam, beta, e);
         const double Enew = E + (e - n2);
         real* Uo = (real*)r.Uout;
         for (int c = 0; c < nf - 1; ++c) Uo[c * N + i] = U[c * N + i];
@@ -606,7 +607,25 @@ __global__ void __launch_bounds__(256) k_react(fvl::React r, Grid g) {
             }
         }
     }
-    block_partials<256>(acc, mx, src, r.partials, r.flags, bad);
+    block_partials<256>(acc, mx, src, r.partials, r.flags, bad, pblk);
+}
+
+__global__ void __launch_bounds__(256) k_react(fvl::React r, Grid g) {
+    react_cells(r, g, r.dt, blockIdx.x * (long)blockDim.x + threadIdx.x, blockIdx.x);
+}
+
+// Work-list form over reactive instances: phase 1 is the first half-step
+// (gated by the chemistry gate), phase 2 the second (gated by the hydro gate,
+// i.e. only after the first half-step's dt was accepted).
+__global__ void __launch_bounds__(256) k_batch_react(const fvl::BatchItem* items, const int* first, int n,
+                                                     int phase) {
+    const int i = fvl::owner(first, n, blockIdx.x);
+    const fvl::BatchItem& it = fvl::item_at(items, i);
+    const double* k = it.step.coef;
+    if (k[phase == 1 ? fvl::K_CHEM : fvl::K_HYDRO] == 0.0) return;
+    const int local = blockIdx.x - first[i];
+    react_cells(phase == 1 ? it.r1 : it.r2, it.grid, k[fvl::K_HALF], local * 256L + threadIdx.x,
+                (phase == 1 ? it.r1_first : it.r2_first) + local);
 }
 
 // CFL maxima of a stored 3D state (torchSession.py:356-360).
@@ -751,12 +770,15 @@ __global__ void __launch_bounds__(256) k_batch_cfl3d(const fvl::BatchItem* items
     const fvl::BatchItem& it = fvl::item_at(items, lo);
     const int local = b - first[lo], nblk = first[lo + 1] - first[lo];
     const long N = (long)it.grid.nx * it.grid.ny * it.grid.nz;
-    const real* U = (const real*)it.step.U;
+    // A reactive instance's committed state is the first half-step's input.
+    const real* U = (const real*)(it.reactive ? it.r1.U : it.step.U);
+    const real* lam = it.reactive ? (const real*)it.r1.lam : nullptr;
     double acc[5] = {0, 0, 0, 0, 0}, mx[3] = {0, 0, 0};
     for (long i = local * 256L + threadIdx.x; i < N; i += nblk * 256L) {
         real o[5];
         for (int c = 0; c < 5; ++c) o[c] = U[c * N + i];
-        wave3(o, it.step.gamma, mx);
+        const double gg = lam ? (4.5 - (double)lam[i]) / (3.5 - (double)lam[i]) : it.step.gamma;
+        wave3(o, gg, mx);
     }
     block_partials<256>(acc, mx, 0.0, it.step.partials, nullptr, 0, it.cfl_first + local);
 }
@@ -771,6 +793,13 @@ int react_blocks(const Grid& g) { return (int)(((long)g.nx * g.ny * g.nz + 255) 
 
 hipError_t react(const Grid& g, const fvl::React& r, hipStream_t st) {
     k_react<<<react_blocks(g), 256, 0, st>>>(r, g);
+    return hipGetLastError();
+}
+
+hipError_t batch_react(const fvl::BatchItem* items, const int* first, int n, int blocks, int phase,
+                       hipStream_t st) {
+    if (blocks <= 0) return hipSuccess;
+    k_batch_react<<<blocks, 256, 0, st>>>(items, first, n, phase);
     return hipGetLastError();
 }
 

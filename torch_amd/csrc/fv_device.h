@@ -292,6 +292,11 @@ __device__ __forceinline__ void hllc_cart4(const real* l, const real* r, real gl
     f[0] = o[0]; f[dir + 1] = o[1]; f[t] = o[2]; f[3] = o[3];
 }
 
+// Range guard of transported fractions (torchSolver3D.py:254-255): the
+// reference's 1e-10 absorbs double rounding of rho*q/rho; FP32 storage
+// rounds that quotient to ~6e-8, so the FP32 form widens the guard to 1e-6.
+constexpr real kSpecTol = sizeof(real) == sizeof(double) ? (real)1e-10 : (real)1e-6;
+
 __device__ __forceinline__ real gamma_of(real lam) { return ((real)4.5 - lam) / ((real)3.5 - lam); }
 // Per-cell gamma from a lambda array (gamma_calc, torchSolver.py:71-78) or
 // taken as stored when the array already holds gamma.

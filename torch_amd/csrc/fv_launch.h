@@ -25,7 +25,7 @@ enum Flag {
     F_PRED_BAD = 1,      // a predicted face state failed physical_state
     F_UPD_BAD = 2,       // second-order update unphysical
     F_LLF_BAD = 4,       // first-order update unphysical -> RuntimeError
-    F_SPEC_BAD = 8,      // species left [-1e-10, 1+1e-10] (second-order fluxes)
+    F_SPEC_BAD = 8,      // species left [-tol, 1+tol] (second-order fluxes; tol kSpecTol)
     F_SPEC_BAD_LLF = 16, // same with first-order fluxes
     F_REACT_BAD = 32,    // chemistry input outside its domain -> ValueError
     F_STATE_BAD = 64     // state after chemistry unphysical
@@ -62,7 +62,8 @@ struct Step {
     int controller, variant, wavespeed;
     double cfl;
     // When set, the step coefficients come from this device array instead
-    // of c/hc/w: [dt/h x3, 0.5*dt/h x3, dt*vol/h x3, active flag].
+    // of c/hc/w: [dt/h x3, 0.5*dt/h x3, dt*vol/h x3, hydro gate, ...]
+    // (enum CoefSlot).
     const double* coef;
 };
 
@@ -92,8 +93,25 @@ __device__ __forceinline__ Coef coef_of(const Step& s) {
     return k;
 }
 
-// One instance of a batch (device memory, one array per buffer parity).
-// Partial slots of an instance: [second-order | first-order | CFL pass].
+struct React {
+    const void* U;     // input state
+    void* Uout;
+    const void* lam;
+    void* lamout;
+    const void* beta;  // null for Arrhenius without induction array
+    void* betaout;
+    const void* Ustart;  // for the source integral (final half-step), or null
+    const void* Uold;
+    int model;           // 0 arrhenius, 1 znd
+    double dt;
+    double q_heat, activation_energy, pre_exponential;
+    int* flags;
+    double* partials;
+    int want_cfl;
+};
+
+// One instance of a batch (device memory). Partial slots of an instance:
+// [second-order | first-order | CFL pass | first chemistry half | second].
 struct BatchItem {
     Step step;
     Grid grid;
@@ -101,24 +119,72 @@ struct BatchItem {
     int llf_first;     // first slot of the first-order pass
     int cfl_first;     // first slot of the CFL pass
     void* scratch;     // 1D work arrays
+    // Reactive instances (3D): chemistry half-steps around the hydro step
+    // and the four rotating buffers of state, lambda and beta.
+    int reactive, znd;
+    React r1, r2;
+    int r1_first, r2_first;
+    void* slot[3][4];  // [U, lambda, beta][buffer]
 };
 
+// Step coefficients of a batch instance (BatchCtl.coef): dt/h x3,
+// 0.5*dt/h x3, dt*vol/h x3, then the hydro-pass gate, the chemistry
+// half-step duration dt/2 and the chemistry gate.
+enum CoefSlot { K_HYDRO = 9, K_HALF = 10, K_CHEM = 11, NCOEF = 12 };
+
+// Pointers of a reactive instance for the step whose committed buffer is
+// `cur` (A): R(dt/2) A->B, hydro B->C with species, R(dt/2) C->D with the
+// source integral against A and B (torchSession.py:398-515).
+__host__ __device__ inline void set_reactive_slots(BatchItem& it, int cur) {
+    const int A = cur, B = (cur + 1) % 4, C = (cur + 2) % 4, D = (cur + 3) % 4;
+    void* const* U = it.slot[0];
+    void* const* L = it.slot[1];
+    void* const* Bt = it.slot[2];
+    it.r1.U = U[A];
+    it.r1.Uout = U[B];
+    it.r1.lam = L[A];
+    it.r1.lamout = L[B];
+    it.r1.beta = it.znd ? Bt[A] : nullptr;
+    it.r1.betaout = it.znd ? Bt[B] : nullptr;
+    it.r1.Ustart = nullptr;
+    it.r1.Uold = nullptr;
+    it.step.U = U[B];
+    it.step.Uout = U[C];
+    it.step.lam = L[B];
+    it.step.q[0] = L[B];
+    it.step.qout[0] = L[C];
+    it.step.q[1] = it.znd ? Bt[B] : nullptr;
+    it.step.qout[1] = it.znd ? Bt[C] : nullptr;
+    it.r2.U = U[C];
+    it.r2.Uout = U[D];
+    it.r2.lam = L[C];
+    it.r2.lamout = L[D];
+    it.r2.beta = it.znd ? Bt[C] : nullptr;
+    it.r2.betaout = it.znd ? Bt[D] : nullptr;
+    it.r2.Ustart = U[A];
+    it.r2.Uold = U[B];
+}
+
 // Per-instance control record of a batch (device memory). The host fills
-// the constants at admission; the batch post kernel advances the rest after
-// every step, as Simulation.step does on the host (torchSession.py:374-522).
+// the constants at admission; the batch kernels advance the rest on the
+// device, as Simulation.step does on the host (torchSession.py:374-522).
 struct BatchCtl {
     double t_end, volume, cfl;
     double h[3];
-    double time, dt;    // current time; dt of the step in flight
+    double time, dt;     // current time; dt of the step (or attempt) in flight
     double maxima[3];    // wave maxima of the committed state
     double outward[5];   // accumulated boundary integral
-    double coef[10];     // Step.coef of the step in flight
+    double source[5];    // accumulated source integral (reactive)
+    double coef[NCOEF];  // Step.coef of the step in flight (enum Coef)
     int dim, max_steps;
-    int nblk, nllf, ncfl;  // partial slots of the second-order, first-order and CFL passes
+    int nblk, nllf, ncfl, nreact;  // partial slots of the passes
     int steps, fallbacks;
+    int retry;           // failed chemistry attempts of the step in flight
+    int retries;         // all failed attempts so far
     int status;          // 0, or the TORCH_* code that stopped the instance
     int active;          // 1 stepping, 2 waiting for its first dt, 0 stopped
-    int cur;             // which of the two state buffers holds the committed state
+    int cur;             // buffer holding the committed state
+    int reactive;
 };
 
 // Instance of block b in a work list: first[i] <= b < first[i+1].
@@ -141,22 +207,6 @@ __device__ __forceinline__ const BatchItem& item_at(const BatchItem* items, int 
     return *(const BatchItem*)((const CItem*)items + i);
 }
 
-struct React {
-    const void* U;     // input state
-    void* Uout;
-    const void* lam;
-    void* lamout;
-    const void* beta;  // null for Arrhenius without induction array
-    void* betaout;
-    const void* Ustart;  // for the source integral (final half-step), or null
-    const void* Uold;
-    int model;           // 0 arrhenius, 1 znd
-    double dt;
-    double q_heat, activation_energy, pre_exponential;
-    int* flags;
-    double* partials;
-    int want_cfl;
-};
 
 struct Launch {
     int block_x, block_y;  // tile shape chosen by the autotuner
@@ -214,6 +264,9 @@ struct Launch {
                            int blocks, int input, hipStream_t st);                         \
     hipError_t batch_post(fvl::BatchItem* items, fvl::BatchCtl* ctl, int n, int prime,    \
                           hipStream_t st);                                                 \
+    hipError_t batch_react(const fvl::BatchItem* items, const int* first, int n,          \
+                           int blocks, int phase, hipStream_t st);                         \
+    hipError_t batch_check(fvl::BatchItem* items, fvl::BatchCtl* ctl, int n, hipStream_t st); \
     int strip2d_rows(const fvl::Grid& g);                                                 \
     int step1d_tiles(const fvl::Grid& g);                                                 \
     }

@@ -1,11 +1,19 @@
-// Batch post kernel: the per-instance bookkeeping of Simulation.step
+// Batch bookkeeping kernels: the per-instance logic of Simulation.step
 // (python/cfdSolver/torchSession.py:374-522) done on the device, so a step of
 // every instance of a batch is a fixed sequence of launches that a HIP graph
-// can replay. One block per instance folds that instance's per-block
-// partials (maxima and boundary sums, in the same order as the
-// single-session finalize), checks the step's flags, commits (time,
-// step count, boundary integral, buffer swap) and derives the next dt and
-// step coefficients from the new wave maxima.
+// can replay.
+//
+// k_batch_check (reactive instances, after the first chemistry half-step):
+// the CFL test of the heated state and the dt halving of the retry loop
+// (:398-411). A rejected attempt closes the hydro and second-chemistry gates
+// for this replay; the next replay repeats the half-step with the new dt
+// from the same committed buffers, so the arithmetic is the loop's.
+//
+// k_batch_post (every instance, last): folds the instance's per-block
+// partials in the order of the single-session finalize, checks the step's
+// flags, commits (time, step count, boundary and source integrals, buffer
+// rotation) and derives the next dt and step coefficients from the new wave
+// maxima.
 #include "fv_device.h"
 #include "fv_launch.h"
 
@@ -36,25 +44,13 @@ __device__ __forceinline__ double dt_of(const fvl::BatchCtl& c, const double* mx
 
 __device__ __forceinline__ double py_min(double a, double b) { return b < a ? b : a; }
 
-__global__ void __launch_bounds__(256) k_batch_post(fvl::BatchItem* items, fvl::BatchCtl* ctl, int prime) {
-#pragma clang fp contract(off)
-    __shared__ double red[8][256];
-    const int i = blockIdx.x, t = threadIdx.x;
-    fvl::BatchCtl& c = ctl[i];
-    if (prime ? c.active != 2 : c.active != 1) return;
-    fvl::BatchItem& it = items[i];
-    const int flags = prime ? 0 : *it.step.flags;
-    const bool fb = (flags & (fvl::F_PRED_BAD | fvl::F_UPD_BAD)) != 0;
-    int first = 0, nb = c.nblk;
-    if (prime || c.dim == 1) {
-        first = it.cfl_first;
-        nb = c.ncfl;
-    } else if (fb) {
-        first = it.llf_first;
-        nb = c.nllf;
-    }
-    const double* part = it.step.partials + (long)first * fvl::NSLOT;
-    for (int slot = 0; slot < 8; ++slot) {
+// Folds slots [0, nslot) of nb per-block partials like the single-session
+// finalize: 256 strided partial results, then a tree (maxima for slots
+// 0-2, sums for the rest). Every thread of the block must call it; thread 0
+// reads the results from red[slot][0].
+__device__ void fold(const double* part, int nb, int nslot, double (*red)[256]) {
+    const int t = threadIdx.x;
+    for (int slot = 0; slot < nslot; ++slot) {
         const bool is_max = slot < 3;
         double v = 0.0;
         for (int b = t; b < nb; b += 256) {
@@ -66,14 +62,97 @@ __global__ void __launch_bounds__(256) k_batch_post(fvl::BatchItem* items, fvl::
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
         if (t < s)
-            for (int slot = 0; slot < 8; ++slot) {
+            for (int slot = 0; slot < nslot; ++slot) {
                 double a = red[slot][t], b = red[slot][t + s];
                 red[slot][t] = slot < 3 ? fmax(a, b) : a + b;
             }
         __syncthreads();
     }
+}
+
+// Step coefficients for dt (the host driver's base_step expressions).
+__device__ __forceinline__ void set_coef(fvl::BatchCtl& c, double dt) {
+    for (int d = 0; d < 3; ++d) {
+        c.coef[d] = dt / c.h[d];
+        c.coef[3 + d] = 0.5 * dt / c.h[d];
+        c.coef[6 + d] = dt * c.volume / c.h[d];
+    }
+}
+
+__device__ __forceinline__ void stop(fvl::BatchCtl& c, int status) {
+    c.status = status;
+    c.active = 0;
+    c.coef[fvl::K_HYDRO] = 0.0;
+    c.coef[fvl::K_CHEM] = 0.0;
+}
+
+__global__ void __launch_bounds__(256) k_batch_check(fvl::BatchItem* items, fvl::BatchCtl* ctl) {
+#pragma clang fp contract(off)
+    __shared__ double red[3][256];
+    const int i = blockIdx.x;
+    fvl::BatchCtl& c = ctl[i];
+    if (c.active != 1 || c.coef[fvl::K_CHEM] == 0.0) return;
+    fvl::BatchItem& it = items[i];
+    fold(it.step.partials + (long)it.r1_first * fvl::NSLOT, c.nreact, 3, red);
+    if (threadIdx.x != 0) return;
+    int* flags = it.step.flags;
+    if (*flags & fvl::F_REACT_BAD) {
+        stop(c, TORCH_INVALID);
+        return;
+    }
+    const double mx[3] = {red[0][0], red[1][0], red[2][0]};
+    const double dt = c.dt, allowed = dt_of(c, mx);
+    *flags = 0;  // the hydro pass (or the next attempt) starts from clear flags
+    if (dt <= allowed * (1 + 1e-12)) {
+        set_coef(c, dt);
+        c.coef[fvl::K_HYDRO] = 1.0;
+        c.retries += c.retry;
+        return;
+    }
+    c.retry += 1;
+    if (c.retry == 60) {
+        stop(c, TORCH_NO_STABLE_STEP);
+        return;
+    }
+    const double next = py_min(dt / 2, allowed);
+    c.dt = next;
+    c.coef[fvl::K_HALF] = next / 2;
+    c.coef[fvl::K_HYDRO] = 0.0;
+}
+
+__global__ void __launch_bounds__(256) k_batch_post(fvl::BatchItem* items, fvl::BatchCtl* ctl, int prime) {
+#pragma clang fp contract(off)
+    __shared__ double red[9][256];
+    const int i = blockIdx.x, t = threadIdx.x;
+    fvl::BatchCtl& c = ctl[i];
+    if (prime ? c.active != 2 : c.active != 1) return;
+    // A reactive attempt rejected by k_batch_check: nothing to commit yet.
+    if (!prime && c.reactive && c.coef[fvl::K_HYDRO] == 0.0) return;
+    fvl::BatchItem& it = items[i];
+    const int flags = prime ? 0 : *it.step.flags;
+    const bool fb = (flags & (fvl::F_PRED_BAD | fvl::F_UPD_BAD)) != 0;
+    const double* part = it.step.partials;
+    double mx[3], out[5] = {0, 0, 0, 0, 0}, src = 0;
+    if (prime) {
+        fold(part + (long)it.cfl_first * fvl::NSLOT, c.ncfl, 3, red);
+        for (int m = 0; m < 3; ++m) mx[m] = red[m][0];
+    } else if (c.dim == 1) {
+        fold(part + (long)it.cfl_first * fvl::NSLOT, c.ncfl, 3, red);
+        for (int m = 0; m < 3; ++m) mx[m] = red[m][0];
+    } else {
+        // Boundary sums (and, inert, maxima) of the pass whose state was kept.
+        fold(part + (long)(fb ? it.llf_first : 0) * fvl::NSLOT, fb ? c.nllf : c.nblk, 8, red);
+        for (int m = 0; m < 3; ++m) mx[m] = red[m][0];
+        for (int k = 0; k < 5; ++k) out[k] = red[fvl::S_OUT + k][0];
+        if (c.reactive) {
+            __syncthreads();
+            // Maxima and source integral of the second chemistry half-step.
+            fold(part + (long)it.r2_first * fvl::NSLOT, c.nreact, 9, red);
+            for (int m = 0; m < 3; ++m) mx[m] = red[m][0];
+            src = red[fvl::S_SRC][0];
+        }
+    }
     if (t != 0) return;
-    double mx[3] = {red[0][0], red[1][0], red[2][0]};
     if (c.dim == 2 && it.grid.bc[1] == BC_INFLOW) {
         // The inlet ghost rows belong to the padded state the 2D CFL covers.
         real u[4];
@@ -83,57 +162,62 @@ __global__ void __launch_bounds__(256) k_batch_post(fvl::BatchItem* items, fvl::
         mx[1] = fmax(mx[1], (double)(fabs(p.vy) + p.a));
     }
     if (!prime) {
-        if (flags & fvl::F_LLF_BAD) {
-            c.status = TORCH_NONPHYSICAL;  // the committed state stays as it was
-            c.active = 0;
-            c.coef[9] = 0.0;
-            return;
-        }
+        // Error order of the host driver (torch_state_step); the committed
+        // state stays as it was.
+        if (flags & fvl::F_LLF_BAD) return stop(c, TORCH_NONPHYSICAL);
+        if (c.dim == 3 && (flags & (fb ? fvl::F_SPEC_BAD_LLF : fvl::F_SPEC_BAD))) return stop(c, TORCH_SPECIES_RANGE);
+        if (flags & fvl::F_REACT_BAD) return stop(c, TORCH_INVALID);
+        if (flags & fvl::F_STATE_BAD) return stop(c, TORCH_REACTION_STATE);
         const int nf = c.dim + 2;
-        double out[5] = {0, 0, 0, 0, 0};
         if (c.dim == 1) {
             const real* Fs = (const real*)it.scratch;
             const int n = it.grid.nx;
-            for (int k = 0; k < 3; ++k) out[k] = c.coef[6] * (double)(Fs[k * (n + 1) + n] - Fs[k * (n + 1)]);
-        } else {
-            for (int k = 0; k < nf; ++k) out[k] = red[fvl::S_OUT + k][0];
+            for (int k = 0; k < 3; ++k)
+                out[k] = c.coef[6] * (double)(Fs[k * (n + 1) + n] - Fs[k * (n + 1)]);
         }
         for (int k = 0; k < nf; ++k) c.outward[k] = c.outward[k] + out[k];
+        if (c.reactive) c.source[nf - 1] = c.source[nf - 1] + c.volume * src;
         c.time = py_min(c.time + c.dt, c.t_end);
         c.steps += 1;
         c.fallbacks += fb ? 1 : 0;
-        const void* u = it.step.U;
-        it.step.U = it.step.Uout;
-        it.step.Uout = const_cast<void*>(u);
-        c.cur ^= 1;
+        if (c.reactive) {
+            c.cur = (c.cur + 3) % 4;  // D of this step is A of the next
+            fvl::set_reactive_slots(it, c.cur);
+        } else {
+            const void* u = it.step.U;
+            it.step.U = it.step.Uout;
+            it.step.Uout = const_cast<void*>(u);
+            c.cur ^= 1;
+        }
     }
     for (int m = 0; m < 3; ++m) c.maxima[m] = mx[m];
-    if (!prime && (c.time == c.t_end || c.steps >= c.max_steps)) {
-        c.active = 0;
-        c.coef[9] = 0.0;
-        return;
-    }
+    if (!prime && (c.time == c.t_end || c.steps >= c.max_steps)) return stop(c, TORCH_OK);
     const double dt = py_min(dt_of(c, mx), c.t_end - c.time);
-    if (!isfinite(dt) || dt <= 0 || c.time + dt == c.time) {
-        c.status = TORCH_STALLED;
-        c.active = 0;
-        c.coef[9] = 0.0;
-        return;
-    }
+    if (!isfinite(dt) || dt <= 0 || c.time + dt == c.time) return stop(c, TORCH_STALLED);
     c.dt = dt;
-    for (int d = 0; d < 3; ++d) {
-        c.coef[d] = dt / c.h[d];
-        c.coef[3 + d] = 0.5 * dt / c.h[d];
-        c.coef[6 + d] = dt * c.volume / c.h[d];
-    }
-    c.coef[9] = 1.0;
     c.active = 1;
+    c.retry = 0;
     *it.step.flags = 0;
+    if (c.reactive) {
+        // The first half-step decides whether dt stands (k_batch_check).
+        c.coef[fvl::K_HALF] = dt / 2;
+        c.coef[fvl::K_CHEM] = 1.0;
+        c.coef[fvl::K_HYDRO] = 0.0;
+    } else {
+        set_coef(c, dt);
+        c.coef[fvl::K_HYDRO] = 1.0;
+    }
 }
 
 hipError_t batch_post(fvl::BatchItem* items, fvl::BatchCtl* ctl, int n, int prime, hipStream_t st) {
     if (n <= 0) return hipSuccess;
     k_batch_post<<<n, 256, 0, st>>>(items, ctl, prime);
+    return hipGetLastError();
+}
+
+hipError_t batch_check(fvl::BatchItem* items, fvl::BatchCtl* ctl, int n, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    k_batch_check<<<n, 256, 0, st>>>(items, ctl);
     return hipGetLastError();
 }
 

@@ -2,11 +2,12 @@
 
 The reference steps every ``Simulation`` on its own
 (python/cfdSolver/torchSession.py:374-522); a set of problems is a Python
-loop over sessions. ``Batch`` keeps many inert sessions resident on the
-device and advances all of them with one graph replay per step. Each
-instance keeps the single-session arithmetic and bookkeeping (its own dt,
-time, step count and boundary integral), so a session stepped in a batch
-ends where it ends stepped alone.
+loop over sessions. ``Batch`` keeps many sessions (inert of any dimension,
+reactive in 3D) resident on the device and advances all of them with one
+graph replay per step. Each instance keeps the single-session arithmetic
+and bookkeeping (its own dt, time, step count, boundary and source
+integrals, reactive dt halvings), so a session stepped in a batch ends where
+it ends stepped alone.
 
 Typical use with existing sessions::
 
@@ -25,7 +26,7 @@ from .torchSolver3D import config as _config
 
 
 class Batch:
-    """Inert sessions of any dimension and size stepped together.
+    """Sessions of any dimension and size stepped together (reactive: 3D).
 
     ``hbm_budget`` caps the device bytes the batch may hold (0: no cap);
     ``add`` returns None when an instance does not fit, which is the
@@ -47,13 +48,13 @@ class Batch:
         session has left).
         """
         c = sim.config
-        if c["reactive"]:
-            raise ValueError("batches hold inert sessions")
+        if c["reactive"] and c["dimension"] != 3:
+            raise ValueError("batches hold inert sessions and reactive 3D sessions")
         cfg = sim._make_config()
         cfg.precision = _abi.precision_code(self.precision)
         t_end = c["t_end"] if t_end is None else float(t_end)
         max_steps = c["max_steps"] - sim.steps if max_steps is None else int(max_steps)
-        ident = self._batch.add(cfg, sim.state, sim.time, t_end, max_steps)
+        ident = self._batch.add(cfg, sim.state, sim.time, t_end, max_steps, lam=sim.lam, beta=sim.beta)
         if ident is not None:
             self._meta[ident] = (c["dimension"] + 2, sim.steps)
         return ident
@@ -119,11 +120,18 @@ class Batch:
         p = self._batch.info(ident)
         nf = self._meta[ident][0]
         return dict(time=p.time, dt=p.dt, steps=p.steps, fallbacks=p.fallbacks, status=p.status,
-                    active=bool(p.active), completed=bool(p.completed), cells=p.cells,
-                    boundary_integral=np.array(p.outward[:nf]), wave_max=np.array(p.wave_max[:]))
+                    active=bool(p.active), completed=bool(p.completed), cells=p.cells, retries=p.retries,
+                    boundary_integral=np.array(p.outward[:nf]), source_integral=np.array(p.source[:nf]),
+                    wave_max=np.array(p.wave_max[:]))
 
     def state(self, ident):
-        return self._batch.download(ident)
+        """Committed conservative state of an instance."""
+        return self._batch.download(ident)[0]
+
+    def fractions(self, ident):
+        """(lambda, beta) of a reactive instance (beta None unless ZND)."""
+        _, lam, beta = self._batch.download(ident)
+        return lam, beta
 
     def error(self, ident):
         """Raise the exception Simulation.step would have raised, if any."""
@@ -139,11 +147,13 @@ class Batch:
         """Write an instance's progress back into its Simulation."""
         info = self.info(ident)
         start_steps = self._meta[ident][1]
-        sim.state = self.state(ident)
+        state, lam, beta = self._batch.download(ident)
+        sim._upload(state, lam, beta)
         sim.time = info["time"]
         sim.steps = start_steps + info["steps"]
         sim.fallbacks += info["fallbacks"]
         sim.boundary_integral = sim.boundary_integral + info["boundary_integral"]
+        sim.source_integral = sim.source_integral + info["source_integral"]
         return sim
 
     def close(self):

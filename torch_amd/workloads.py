@@ -82,3 +82,79 @@ def uneven_problems(seed=508, count=256):
         prim, opts = uneven_problem(rng, dim)
         yield i, prim, opts
 
+
+
+# ---------------------------------------------------------------------------
+# Served reactive ignition requests
+# ---------------------------------------------------------------------------
+R_UNIVERSAL = 8314.46  # J/(kmol K), the kinetics' gas constant (torchSolver.py:90-114)
+
+
+def molar_mass(lam):
+    """Mixture molar mass of the kinetics at reactant fraction lam."""
+    return lam * (2.0 / 3.0) * 2.016 + lam * (1.0 / 3.0) * 32.0 + (1.0 - lam) * 18.016
+
+
+def gamma_of(lam):
+    return (4.5 - lam) / (3.5 - lam)
+
+
+def induction_integral(theta, q, t_hat, rho0=1.0, p0=1.0, points=4001):
+    """Time for a homogeneous reactor at the hot-spot peak to burn from
+    lambda = 1 to 0.5 with unit pre-exponential factor.
+
+    At rest and with constant density the internal energy is
+    E(lam) = E0 + Q rho (1 - lam) and p = (gamma(lam) - 1) E, so the
+    temperature and the rate are functions of lam alone and the induction
+    time is the integral of dlam / (lam exp(-Ea / (8.314 T(lam)))) over
+    [0.5, 1]; a pre-exponential factor A divides it by A.
+    """
+    t0 = p0 * molar_mass(1.0) / (rho0 * R_UNIVERSAL)
+    ea = theta * 8.314 * t0
+    q_heat = q * (R_UNIVERSAL / molar_mass(1.0)) * t0
+    lam = np.linspace(0.5, 1.0, points)
+    e0 = t_hat * p0 / (gamma_of(1.0) - 1.0)
+    e = e0 + q_heat * rho0 * (1.0 - lam)
+    p = (gamma_of(lam) - 1.0) * e
+    temp = p * molar_mass(lam) / (rho0 * R_UNIVERSAL)
+    f = 1.0 / (lam * np.exp(-ea / (8.314 * temp)))
+    return float(np.sum(0.5 * (f[1:] + f[:-1]) * np.diff(lam)))
+
+
+def served_requests(seed=355, count=64, t_run=0.1, znd_fraction=0.1, sizes=(32, 160)):
+    """Seeded 3D ignition requests (parameters only; ``ignition_problem``
+    builds the arrays). Nondimensional background (rho0, p0) = (1, 1) at
+    lambda = 1; Ea/(R T0) in [20, 30]; Q/(R T0) in [10, 50]; a Gaussian
+    hot spot with peak T/T0 in [1.5, 3] and width 0.05-0.15 of the box at a
+    random centre in [0.25, 0.75]^3; ZND for a tenth of the requests; the
+    pre-exponential factor puts the peak's induction time at 0.1 t_run."""
+    rng = np.random.default_rng(seed)
+    t0 = molar_mass(1.0) / R_UNIVERSAL
+    out = []
+    for i in range(count):
+        n = int(rng.integers(sizes[0], sizes[1] + 1))
+        theta = rng.uniform(20.0, 30.0)
+        q = rng.uniform(10.0, 50.0)
+        t_hat = rng.uniform(1.5, 3.0)
+        sigma = rng.uniform(0.05, 0.15)
+        centre = rng.uniform(0.25, 0.75, size=3)
+        model = "znd" if rng.random() < znd_fraction else "arrhenius"
+        tau = induction_integral(theta, q, t_hat)
+        out.append(dict(index=i, n=n, theta=theta, q=q, t_hat=t_hat, sigma=sigma, centre=centre, model=model,
+                        t_run=t_run, constants=dict(activation_energy=theta * 8.314 * t0,
+                                                    q_heat=q * (R_UNIVERSAL / molar_mass(1.0)) * t0,
+                                                    pre_exponential=tau / (0.1 * t_run))))
+    return out
+
+
+def ignition_problem(req):
+    """Primitive state and lambda of one request on the unit cube."""
+    n = req["n"]
+    x, y, z = _centres([n, n, n])
+    cx, cy, cz = req["centre"]
+    r2 = (x - cx) ** 2 + (y - cy) ** 2 + (z - cz) ** 2
+    prim = np.zeros((5, n, n, n))
+    prim[0] = 1.0
+    prim[4] = 1.0 + (req["t_hat"] - 1.0) * np.exp(-r2 / (2.0 * req["sigma"] ** 2))
+    lam = np.ones((n, n, n))
+    return prim, lam
