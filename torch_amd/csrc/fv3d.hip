@@ -614,15 +614,16 @@ __global__ void __launch_bounds__(256) k_react(fvl::React r, Grid g) {
     react_cells(r, g, r.dt, blockIdx.x * (long)blockDim.x + threadIdx.x, blockIdx.x);
 }
 
-// Work-list form over reactive instances: phase 1 is the first half-step
-// (gated by the chemistry gate), phase 2 the second (gated by the hydro gate,
-// i.e. only after the first half-step's dt was accepted).
+// Work-list form over reactive instances: phase 1 is an attempt of the first
+// half-step (while the chemistry gate is open and dt not yet accepted),
+// phase 2 the second half-step (once the hydro gate is open).
 __global__ void __launch_bounds__(256) k_batch_react(const fvl::BatchItem* items, const int* first, int n,
                                                      int phase) {
     const int i = fvl::owner(first, n, blockIdx.x);
     const fvl::BatchItem& it = fvl::item_at(items, i);
     const double* k = it.step.coef;
-    if (k[phase == 1 ? fvl::K_CHEM : fvl::K_HYDRO] == 0.0) return;
+    // Phase 1 runs until this step's dt is accepted; phase 2 once it is.
+    if (phase == 1 ? (k[fvl::K_CHEM] == 0.0 || k[fvl::K_HYDRO] != 0.0) : k[fvl::K_HYDRO] == 0.0) return;
     const int local = blockIdx.x - first[i];
     react_cells(phase == 1 ? it.r1 : it.r2, it.grid, k[fvl::K_HALF], local * 256L + threadIdx.x,
                 (phase == 1 ? it.r1_first : it.r2_first) + local);

@@ -5,9 +5,11 @@
 //
 // k_batch_check (reactive instances, after the first chemistry half-step):
 // the CFL test of the heated state and the dt halving of the retry loop
-// (:398-411). A rejected attempt closes the hydro and second-chemistry gates
-// for this replay; the next replay repeats the half-step with the new dt
-// from the same committed buffers, so the arithmetic is the loop's.
+// (:398-411). A replay holds a few attempt/check pairs; an accepted attempt
+// opens the hydro gate and later attempts of the replay skip the instance.
+// An instance still rejected after the last pair commits nothing in this
+// replay and continues the loop in the next one from the same committed
+// buffers, so the arithmetic is the loop's.
 //
 // k_batch_post (every instance, last): folds the instance's per-block
 // partials in the order of the single-session finalize, checks the step's
@@ -91,7 +93,8 @@ __global__ void __launch_bounds__(256) k_batch_check(fvl::BatchItem* items, fvl:
     __shared__ double red[3][256];
     const int i = blockIdx.x;
     fvl::BatchCtl& c = ctl[i];
-    if (c.active != 1 || c.coef[fvl::K_CHEM] == 0.0) return;
+    // Stopped, or this step's dt already accepted by an earlier attempt.
+    if (c.active != 1 || c.coef[fvl::K_CHEM] == 0.0 || c.coef[fvl::K_HYDRO] != 0.0) return;
     fvl::BatchItem& it = items[i];
     fold(it.step.partials + (long)it.r1_first * fvl::NSLOT, c.nreact, 3, red);
     if (threadIdx.x != 0) return;

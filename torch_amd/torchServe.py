@@ -75,7 +75,8 @@ class Server:
         self.rounds = 0
         self.device_ms = 0.0
 
-    def _admit(self, req, prim, opts):
+    def payload(self, prim, opts):
+        """Host-side admission data of a request: (config, state, lam, beta, t_end)."""
         from .torchSolver3D import config as _config
 
         dim = prim.ndim - 1
@@ -95,16 +96,26 @@ class Server:
         if reactive and model == "znd":
             beta = opts.get("beta_init")
             beta = np.ones(prim.shape[1:]) if beta is None else beta
-        ident = self.batch._batch.add(cfg, state, 0.0, opts["t_end"], self.max_steps, lam=lam, beta=beta)
+        return cfg, state, lam, beta, opts["t_end"]
+
+    def _admit(self, req, payload):
+        cfg, state, lam, beta, t_end = payload
+        ident = self.batch._batch.add(cfg, state, 0.0, t_end, self.max_steps, lam=lam, beta=beta)
         if ident is not None:
-            self.batch._meta[ident] = (dim + 2, 0)
-            req.cells = int(np.prod(prim.shape[1:]))
+            self.batch._meta[ident] = (cfg.dim + 2, 0)
+            req.cells = int(np.prod(state.shape[1:]))
         return ident
 
-    def serve(self, requests, clock=time.perf_counter):
+    def serve(self, requests, prebuild=False, clock=time.perf_counter):
         """Run the requests (arrival times in seconds from the start); returns them
-        with admitted/completed times, steps and status filled in."""
+        with admitted/completed times, steps and status filled in.
+
+        With ``prebuild`` every request's initial arrays are built before the
+        clock starts (they arrive with the request); otherwise the server
+        builds them when the request arrives, in a preparer thread.
+        """
         pending = sorted(requests, key=lambda r: r.arrival)
+        built = {id(r): self.payload(*r.build()) for r in pending} if prebuild else {}
         ready = queue.Queue()
         start = clock()
 
@@ -113,8 +124,7 @@ class Server:
                 wait = req.arrival - (clock() - start)
                 if wait > 0:
                     time.sleep(wait)
-                prim, opts = req.build()
-                ready.put((req, prim, opts))
+                ready.put((req, built.pop(id(req)) if prebuild else self.payload(*req.build())))
             ready.put(None)
 
         worker = threading.Thread(target=prepare, daemon=True)
@@ -136,13 +146,13 @@ class Server:
                 block = False
             # Admit in arrival order while the budget allows.
             still = []
-            for req, prim, opts in waiting:
+            for req, payload in waiting:
                 if still:
-                    still.append((req, prim, opts))
+                    still.append((req, payload))
                     continue
-                ident = self._admit(req, prim, opts)
+                ident = self._admit(req, payload)
                 if ident is None:
-                    still.append((req, prim, opts))
+                    still.append((req, payload))
                 else:
                     req.admitted = clock() - start
                     active[ident] = req
