@@ -195,10 +195,19 @@ def run_served(seed, count, t_run, budget_gib, round_steps, precision="fp32"):
     return out
 
 
-def roofline(n, bytes_per_cell, kernel_ms):
+# HBM bytes per launch of the 3D step kernel on the 256^3 Sod tube with the
+# 32x16 / z16 tile, from rocprofv3 FETCH_SIZE and WRITE_SIZE counter runs
+# scaled by a 1 GiB device copy (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000); the
+# commands and per-counter values are kept with the profile summaries.
+TRAFFIC = {("f32", 256, (32, 16, 16)): 1340908557, ("f64", 256, (32, 16, 16)): 2262793198}
+
+
+def roofline(n, bytes_per_cell, kernel_ms, dtype=None, tile=None):
     achieved = bytes_per_cell * n ** 3 / (kernel_ms * 1e-3) / 1e9
+    traffic = TRAFFIC.get((dtype, n, tuple(tile) if tile else None))
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None}
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "algorithmic_bytes": bytes_per_cell * n ** 3}
 
 
 def fp32_error(n, warmup, steps, dts64):
@@ -365,10 +374,10 @@ def main(argv=None):
                                "gamma 1.4, full Simulation.step; FP32 form (f32 bulk, f64 reductions) as a "
                                "graph-replayed batch of one",
                    "cells": n ** 3, "tile": list(r32["tile"]) if r32 else None, "parallelism": "1 GPU"},
-        "roofline": roofline(n, 40, head["kernel_ms"]) if head else None,
+        "roofline": roofline(n, 40, head["kernel_ms"], "f32", r32["tile"] if r32 else (32, 16, 16)) if head else None,
         "fp64_plain": None if not r64 else {
             "value": round(r64["value"], 1), "ms_per_step": round(r64["ms_per_step"], 4), "dtype": "f64",
-            "kernel_ms": round(r64["kernel_ms"], 4), "roofline": roofline(n, 80, r64["kernel_ms"]),
+            "kernel_ms": round(r64["kernel_ms"], 4), "roofline": roofline(n, 80, r64["kernel_ms"], "f64", (32, 16, 16)),
             "fallbacks": r64["fallbacks"]},
         "fp32_session_loop": None if not r32s else {
             "value": round(r32s["value"], 1), "ms_per_step": round(r32s["ms_per_step"], 4),
