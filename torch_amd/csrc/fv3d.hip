@@ -86,6 +86,20 @@ __device__ void block_partials(double (&acc)[5], double (&mx)[3], double src, do
     }
 }
 
+// |u_m| + a of a 3D state accumulated as maxima in the storage precision
+// (widening to double afterwards is exact, so this equals reducing in
+// double); a NaN speed sticks.
+__device__ __forceinline__ void wave3r(const real* o, real g, real (&mx)[3]) {
+    const real ir = rcp(o[0]);
+    const real p = (g - (real)1) * (o[4] - ((real)0.5 * (o[1] * o[1] + o[2] * o[2] + o[3] * o[3])) * ir);
+    const real a = rsqrt_((g * p) * ir);
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+        const real v = fabs(o[m + 1] * ir) + a;
+        mx[m] = (v != v || v > mx[m]) ? v : mx[m];
+    }
+}
+
 // |u_m| + a of a committed 3D state (torchSession.py:356-360), reduced in
 // double. The FP32 form evaluates the speeds in float (exact when widened).
 __device__ __forceinline__ void wave3(const real* o, double g, double (&mx)[3]) {
@@ -183,7 +197,8 @@ __device__ __forceinline__ void step3d_tile(const Step& s, const G3& g, int tbx,
     real qp[2] = {0, 0}, qc[2] = {0, 0}, qn[2] = {0, 0};
     real zr[5] = {0, 0, 0, 0, 0}, Fzp[5] = {0, 0, 0, 0, 0}, sfzp[2] = {0, 0};
     real R[5] = {0, 0, 0, 0, 0}, rq[2] = {0, 0};
-    double acc[5] = {0, 0, 0, 0, 0}, mx[3] = {0, 0, 0};
+    double acc[5] = {0, 0, 0, 0, 0}, mx[3];
+    real mxr[3] = {0, 0, 0};
     int bad = 0;
 
     load5(U, N, g, x, y, z0 - 2, Ap);
@@ -250,9 +265,9 @@ __device__ __forceinline__ void step3d_tile(const Step& s, const G3& g, int tbx,
         }
         // Positivity of every active predicted face state (:152-159).
         if (in_active && k >= -1 && k <= nz) {
-            if (!physical5(xl) || !physical5(xr) || !physical5(yl) || !physical5(yr) || !physical5(zl) ||
-                !physical5(zn))
-                bad |= fvl::F_PRED_BAD;
+            const bool ok = physical5(xl) & physical5(xr) & physical5(yl) & physical5(yr) & physical5(zl) &
+                            physical5(zn);
+            if (!ok) bad |= fvl::F_PRED_BAD;
         }
 #pragma unroll
         for (int c = 0; c < 5; ++c) sY[c][row][lane] = yr[c];
@@ -307,7 +322,7 @@ __device__ __forceinline__ void step3d_tile(const Step& s, const G3& g, int tbx,
 #pragma unroll
             for (int c = 0; c < 5; ++c) Uo[c * N + i] = o[c];
             if (!physical5(o)) bad |= fvl::F_UPD_BAD;
-            if (s.want_cfl) wave3(o, (double)gconst, mx);
+            if (s.want_cfl) wave3r(o, gconst, mxr);
             for (int j = 0; j < nq; ++j) {
                 real v = rq[j] - c2 * (sfz[j] - sfzp[j]);
                 real f = v / o[0];
@@ -382,6 +397,8 @@ __device__ __forceinline__ void step3d_tile(const Step& s, const G3& g, int tbx,
             sfzp[j] = sfz[j];
         }
     }
+#pragma unroll
+    for (int m = 0; m < 3; ++m) mx[m] = (double)mxr[m];
     block_partials<BX * BY>(acc, mx, 0.0, s.partials, s.flags, bad, pblk);
 }
 

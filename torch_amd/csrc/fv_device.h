@@ -23,8 +23,12 @@ namespace fv {
 enum Limiter { VANLEER = 0, ALBADA = 1, MINBEE = 2, SUPERBEE = 3, UNOS = 4 };
 enum Boundary { BC_OPEN = 0, BC_PERIODIC = 1, BC_REFLECTIVE = 2, BC_INFLOW = 3 };
 
-__device__ __forceinline__ real rmax(real a, real b) { return a > b ? a : (b > a ? b : (a == a ? a : b)); }
-__device__ __forceinline__ real rmin(real a, real b) { return a < b ? a : (b < a ? b : (a == a ? a : b)); }
+// Floors and clamps: the non-NaN operand wins (a floor maps NaN to the
+// floor, like the comparisons the reference's floors reduce to here). Every
+// call site has a non-NaN second operand or operands that cannot tie at +-0,
+// so the single-instruction IEEE maxNum/minNum is exact.
+__device__ __forceinline__ real rmax(real a, real b) { return fmax(a, b); }
+__device__ __forceinline__ real rmin(real a, real b) { return fmin(a, b); }
 // NumPy's maximum/minimum propagate NaN; so do these (a NaN operand wins).
 __device__ __forceinline__ real nmax(real a, real b) { return (a != a || b != b) ? (a + b) : (a > b ? a : b); }
 __device__ __forceinline__ real nmin(real a, real b) { return (a != a || b != b) ? (a + b) : (a < b ? a : b); }
@@ -115,6 +119,11 @@ __device__ __forceinline__ real rsqrt_(real x) { return sqrt(x); }
 // minus's sign alone); that branch is evaluated literally.
 template <int LIM, int DIM>
 __device__ __forceinline__ real limited(real minus, real plus) {
+    // plus == +-0 (uniform neighbours): phi(minus / 1e-14) is finite and >= 0
+    // for every limiter when |minus| is far from overflow, so the product is
+    // a zero of plus's sign, up to the sign of a zero slope (which adding it
+    // to the cell value erases).
+    if (plus == (real)0 && fabs(minus) < (real)1e280) return plus;
     if (fabs(plus) < (real)1e-14) {
         real r = minus / (real)1e-14;
         return (DIM == 3 ? phi3<LIM>(r) : phi2<LIM>(r)) * plus;
@@ -173,18 +182,17 @@ __device__ __forceinline__ bool energy_ok(real e_total, real half_s2, real rho) 
 }
 
 // physical_state on one 5-component state (torchSolver.py:1038-1046).
+// Evaluated without short-circuit branches (bitwise &), so a wave tests
+// all its states with straight-line code.
 __device__ __forceinline__ bool physical5(const real* s) {
-    bool finite = true;
-#pragma unroll
-    for (int c = 0; c < 5; ++c) finite = finite && isfinite(s[c]);
-    if (!(finite && s[0] > (real)1e-12)) return false;
-    return energy_ok(s[4], (real)0.5 * (s[1] * s[1] + s[2] * s[2] + s[3] * s[3]), s[0]);
+    const bool finite = isfinite(s[0]) & isfinite(s[1]) & isfinite(s[2]) & isfinite(s[3]) & isfinite(s[4]);
+    return finite & (s[0] > (real)1e-12) &
+           energy_ok(s[4], (real)0.5 * (s[1] * s[1] + s[2] * s[2] + s[3] * s[3]), s[0]);
 }
 
 __device__ __forceinline__ bool physical4(const real* s) {
-    bool finite = isfinite(s[0]) && isfinite(s[1]) && isfinite(s[2]) && isfinite(s[3]);
-    if (!(finite && s[0] > (real)1e-12)) return false;
-    return energy_ok(s[3], (real)0.5 * (s[1] * s[1] + s[2] * s[2]), s[0]);
+    const bool finite = isfinite(s[0]) & isfinite(s[1]) & isfinite(s[2]) & isfinite(s[3]);
+    return finite & (s[0] > (real)1e-12) & energy_ok(s[3], (real)0.5 * (s[1] * s[1] + s[2] * s[2]), s[0]);
 }
 
 // Normal-basis HLLC flux (torchFlux.py:125-248). NV = 5 (3D) or 4 (2D);
