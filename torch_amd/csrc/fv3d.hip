@@ -243,16 +243,22 @@ __device__ __forceinline__ void step3d_tile(const Step& s, const G3& g, int tbx,
 #pragma unroll
         for (int m = 0; m < 3; ++m) {
             const real* sl = m == 0 ? sx : (m == 1 ? sy : sz);
-            real up[5], um[5], fp[5], fm[5];
+            // Zero slopes give equal half states, equal fluxes and an exact
+            // zero change: skip the two flux evaluations.
+            const bool flat = (sl[0] == (real)0) & (sl[1] == (real)0) & (sl[2] == (real)0) & (sl[3] == (real)0) &
+                              (sl[4] == (real)0);
+            if (!flat) {
+                real up[5], um[5], fp[5], fm[5];
 #pragma unroll
-            for (int c = 0; c < 5; ++c) {
-                up[c] = Ac[c] + (real)0.5 * sl[c];
-                um[c] = Ac[c] - (real)0.5 * sl[c];
+                for (int c = 0; c < 5; ++c) {
+                    up[c] = Ac[c] + (real)0.5 * sl[c];
+                    um[c] = Ac[c] - (real)0.5 * sl[c];
+                }
+                flux3(up, gc, m + 1, fp);
+                flux3(um, gc, m + 1, fm);
+#pragma unroll
+                for (int c = 0; c < 5; ++c) P[c] = P[c] - hc[m] * (fp[c] - fm[c]);
             }
-            flux3(up, gc, m + 1, fp);
-            flux3(um, gc, m + 1, fm);
-#pragma unroll
-            for (int c = 0; c < 5; ++c) P[c] = P[c] - hc[m] * (fp[c] - fm[c]);
         }
         // Face states: P -+ s/2 (torchSolver3D.py:178-180).
         real xl[5], xr[5], yl[5], yr[5], zl[5], zn[5];

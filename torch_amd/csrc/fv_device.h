@@ -279,15 +279,43 @@ __device__ __forceinline__ void hllc(const real* ul, const real* ur, real gl, re
     for (int c = 0; c < NV; ++c) out[c] = f[c] + s * (us[c] - u[c]);
 }
 
+// HLLC with equal left and right states (and gammas): the physical flux
+// with the solver's floors, i.e. its F_L. The full solver returns F_L
+// itself for a supersonic face and F_L + S_L (U*_L - U_L) with U*_L equal to
+// U_L up to rounding otherwise, so this differs from it by rounding only.
+template <int NV>
+__device__ __forceinline__ void hllc_equal(const real* u, real g, real* out) {
+    const int E = NV - 1;
+    const real rho = rmax(u[0], (real)1e-12), ir = rcp(rho);
+    const real un = u[1] * ir;
+    real ut[NV - 3], s2 = 0;
+#pragma unroll
+    for (int t = 0; t < NV - 3; ++t) {
+        ut[t] = u[2 + t] * ir;
+        s2 = t == 0 ? ut[t] * ut[t] : s2 + ut[t] * ut[t];
+    }
+    const real ke = ((real)0.5 * rho) * (un * un + s2);
+    const real p = rmax((g - (real)1) * (u[E] - ke), (real)1e-12);
+    out[0] = rho * un;
+    out[1] = rho * (un * un) + p;
+#pragma unroll
+    for (int t = 0; t < NV - 3; ++t) out[2 + t] = (rho * un) * ut[t];
+    out[E] = un * (u[E] + p);
+}
+
 // HLLC between Cartesian-ordered 5-component states across a face normal to
 // axis `dir` (0 x, 1 y, 2 z): rotate to [rho, m_n, m_t(asc), E] and back
-// (torchFlux.py:36-39, 75-76).
+// (torchFlux.py:36-39, 75-76). Faces between equal states (uniform regions)
+// take the solver's own flux of that state without the wave-speed work.
 __device__ __forceinline__ void hllc_cart5(const real* l, const real* r, real gl, real gr, int dir, real* f) {
     real a[5], b[5], o[5];
     int t0 = dir == 0 ? 2 : 1, t1 = dir == 2 ? 2 : 3;
     a[0] = l[0]; a[1] = l[dir + 1]; a[2] = l[t0]; a[3] = l[t1]; a[4] = l[4];
     b[0] = r[0]; b[1] = r[dir + 1]; b[2] = r[t0]; b[3] = r[t1]; b[4] = r[4];
-    hllc<5>(a, b, gl, gr, o);
+    const bool equal = (a[0] == b[0]) & (a[1] == b[1]) & (a[2] == b[2]) & (a[3] == b[3]) & (a[4] == b[4]) &
+                       (gl == gr);
+    if (equal) hllc_equal<5>(a, gl, o);
+    else hllc<5>(a, b, gl, gr, o);
     f[0] = o[0]; f[dir + 1] = o[1]; f[t0] = o[2]; f[t1] = o[3]; f[4] = o[4];
 }
 
