@@ -162,7 +162,7 @@ def run_uneven(seed, count, steps, precisions=("fp32", "fp64"), tile=None):
     return out, err, sum(cells.values())
 
 
-def run_served(seed, count, t_run, budget_gib, round_steps, precision="fp32"):
+def run_served(seed, count, t_run, budget_gib, round_steps, precision="fp32", open_loop=True):
     """Served reactive ignition solves: a closed-loop run (all requests at
     t=0) measures capacity, then Poisson arrivals at 80% of it give the
     p50/p99 time-to-solution. Each request's initial arrays arrive with it
@@ -179,6 +179,10 @@ def run_served(seed, count, t_run, budget_gib, round_steps, precision="fp32"):
     capacity = closed["completed"] / closed["makespan_s"]
     print(f"# served closed loop: {closed['makespan_s']:.2f} s for {count} requests, "
           f"{closed['cell_updates_per_s']:.4g} cell-updates/s", file=sys.stderr, flush=True)
+    out["closed_loop"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in closed.items()}
+    out["capacity_per_s"] = round(capacity, 3)
+    if not open_loop:
+        return out
     rng = np.random.default_rng(seed + 1)
     rate = 0.8 * capacity
     arrivals = np.cumsum(rng.exponential(1.0 / rate, size=count))
@@ -188,10 +192,8 @@ def run_served(seed, count, t_run, budget_gib, round_steps, precision="fp32"):
     server.close()
     print(f"# served open loop at {rate:.2f} req/s: p50 {opened['p50_s']:.3f} s, p99 {opened['p99_s']:.3f} s",
           file=sys.stderr, flush=True)
-    out["closed_loop"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in closed.items()}
     out["open_loop"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in opened.items()}
     out["arrival_rate_per_s"] = round(rate, 3)
-    out["capacity_per_s"] = round(capacity, 3)
     return out
 
 
@@ -306,6 +308,7 @@ def main(argv=None):
     ap.add_argument("--serve-t-run", type=float, default=0.1, help="run length of each served solve")
     ap.add_argument("--serve-budget-gib", type=float, default=4.0, help="HBM budget of the served batch")
     ap.add_argument("--serve-round", type=int, default=4, help="steps between admission checks")
+    ap.add_argument("--serve-closed-only", action="store_true", help="skip the open-loop (Poisson) run")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device copies of MIB MiB (counter calibration) and exit")
     args = ap.parse_args(argv)
@@ -355,7 +358,7 @@ def main(argv=None):
     served = None
     if "served" in legs:
         served = run_served(args.serve_seed, args.serve_count, args.serve_t_run, args.serve_budget_gib,
-                            args.serve_round)
+                            args.serve_round, open_loop=not args.serve_closed_only)
         served["workload"] = (
             f"{args.serve_count} seeded (rng {args.serve_seed}) 3D ignition solves, n^3 with n in 32..160, "
             f"Arrhenius (ZND for ~10%), Ea/RT0 in [20,30], Q/RT0 in [10,50], Gaussian hot spot T/T0 in "
