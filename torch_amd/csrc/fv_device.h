@@ -124,9 +124,13 @@ __device__ __forceinline__ real limited(real minus, real plus) {
     // a zero of plus's sign, up to the sign of a zero slope (which adding it
     // to the cell value erases).
     if (plus == (real)0 && fabs(minus) < (real)1e280) return plus;
-    if (fabs(plus) < (real)1e-14) {
-        real r = minus / (real)1e-14;
-        return (DIM == 3 ? phi3<LIM>(r) : phi2<LIM>(r)) * plus;
+    // Only when some lane of the wave needs it (the test is wave-uniform, so
+    // the rare literal path is a real branch, not speculated code).
+    if (__any(fabs(plus) < (real)1e-14)) {
+        if (fabs(plus) < (real)1e-14) {
+            real r = minus / (real)1e-14;
+            return (DIM == 3 ? phi3<LIM>(r) : phi2<LIM>(r)) * plus;
+        }
     }
     if (!(minus * plus > (real)0)) {
         // r <= 0 (or NaN): every limiter gives phi = 0 except through NaN.
