@@ -10,7 +10,7 @@ workload's length). The JSON line reports:
 
 - value: whole-job cell-updates/s of the FP32 batched form (FP32 bulk
   arithmetic, FP64 CFL/balance reductions; the instance stepped by graph
-  replays of the batch engine, tile picked by the autotuner);
+  replays of the batch engine with the library's default tile);
 - fp64_plain: the FP64 plain method through the session step loop;
 - roofline: the 3D step kernel's algorithmic HBM bytes (40 B per FP32
   cell-update: one read and one write of the five conserved fields) over
@@ -91,10 +91,11 @@ def run_batched_sod(n, steps, warmup, precision="fp32", tile=None):
     ident = batch.add_problem(sod_primitive(n), [1.0, 1.0, 1.0], t_end=OPTIONS["t_end"], cfl=OPTIONS["cfl"],
                               limiter=OPTIONS["limiter"], boundaries=OPTIONS["boundaries"],
                               max_steps=warmup + steps)
-    if tile:
-        batch.set_tiles(*tile)
-    else:
+    if tile == "tune":
         tile, _ = batch.tune(reps=3)
+    else:
+        tile = tile or DEFAULT_TILE[precision]
+        batch.set_tiles(*tile)
     batch.run(warmup)
     kernel_ms = batch.time_pass(reps=10)  # while the instance is still stepping
     t0 = time.perf_counter()
@@ -198,10 +199,14 @@ def run_served(seed, count, t_run, budget_gib, round_steps, precision="fp32", op
 
 
 # HBM bytes per launch of the 3D step kernel on the 256^3 Sod tube with the
-# 32x16 / z16 tile, from rocprofv3 FETCH_SIZE and WRITE_SIZE counter runs
-# scaled by a 1 GiB device copy (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000); the
-# commands and per-counter values are kept with the profile summaries.
-TRAFFIC = {("f32", 256, (32, 16, 16)): 1340908557, ("f64", 256, (32, 16, 16)): 2262793198}
+# default tiles (FP32 32x8, FP64 32x16, z chunk 16), from rocprofv3
+# FETCH_SIZE and WRITE_SIZE counter runs scaled by a 1 GiB device copy
+# (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000); the commands and per-counter
+# values are kept with the profile summaries.
+TRAFFIC = {("f32", 256, (32, 8, 16)): 1557912111, ("f64", 256, (32, 16, 16)): 2307245692}
+
+# The library's default tiles (BX, BY, z chunk) per precision.
+DEFAULT_TILE = {"fp32": (32, 8, 16), "fp64": (32, 16, 16)}
 
 
 def roofline(n, bytes_per_cell, kernel_ms, dtype=None, tile=None):
@@ -282,6 +287,8 @@ LEGS = ("fp64", "fp32", "batched", "error", "uneven", "served", "cpu")
 def _tile(text):
     if not text:
         return None
+    if text == "tune":
+        return text
     parts = tuple(int(v) for v in text.split(","))
     if len(parts) != 3:
         raise argparse.ArgumentTypeError("tile is BX,BY,ZCHUNK")
@@ -297,7 +304,8 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--legs", default=",".join(LEGS),
                     help="comma list of legs to run (profiling runs pick one): " + ",".join(LEGS))
-    ap.add_argument("--tile", type=_tile, default=None, help="BX,BY,ZCHUNK of the batched Sod (skips the tuner)")
+    ap.add_argument("--tile", type=_tile, default=None,
+                    help="BX,BY,ZCHUNK of the batched Sod (default: the library default; 'tune' runs the autotuner)")
     ap.add_argument("--error-steps", type=int, default=50, help="steps of the FP32-vs-FP64 error pass")
     ap.add_argument("--batch-count", type=int, default=256, help="instances of the uneven mix")
     ap.add_argument("--batch-steps", type=int, default=100)
@@ -377,10 +385,10 @@ def main(argv=None):
                                "gamma 1.4, full Simulation.step; FP32 form (f32 bulk, f64 reductions) as a "
                                "graph-replayed batch of one",
                    "cells": n ** 3, "tile": list(r32["tile"]) if r32 else None, "parallelism": "1 GPU"},
-        "roofline": roofline(n, 40, head["kernel_ms"], "f32", r32["tile"] if r32 else (32, 16, 16)) if head else None,
+        "roofline": roofline(n, 40, head["kernel_ms"], "f32", r32["tile"] if r32 else DEFAULT_TILE["fp32"]) if head else None,
         "fp64_plain": None if not r64 else {
             "value": round(r64["value"], 1), "ms_per_step": round(r64["ms_per_step"], 4), "dtype": "f64",
-            "kernel_ms": round(r64["kernel_ms"], 4), "roofline": roofline(n, 80, r64["kernel_ms"], "f64", (32, 16, 16)),
+            "kernel_ms": round(r64["kernel_ms"], 4), "roofline": roofline(n, 80, r64["kernel_ms"], "f64", DEFAULT_TILE["fp64"]),
             "fallbacks": r64["fallbacks"]},
         "fp32_session_loop": None if not r32s else {
             "value": round(r32s["value"], 1), "ms_per_step": round(r32s["ms_per_step"], 4),

@@ -91,7 +91,7 @@ struct torch_batch {
     Table p3, p2, p1;         // first-dt tables over pitems
     int pseg[4] = {0, 0, 0, 0};
     hipGraphExec_t graph = nullptr;
-    int bx = 32, by = 16, zchunk = 16;  // the tuned single-instance default
+    int bx = 32, by = 16, zchunk = 16;  // tile of the 3D work lists (by: see create)
     bool profile = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double prof_ms = 0;
@@ -451,6 +451,7 @@ int torch_batch_create(torch_ctx* ctx, int precision, size_t hbm_budget, torch_b
     b->precision = precision;
     b->ops = precision == TORCH_FP64 ? &kOps64 : &kOps32;
     b->budget = hbm_budget;
+    b->by = precision == TORCH_FP32 ? 8 : 16;  // the single-session default tile
     *out = b;
     return TORCH_OK;
 }

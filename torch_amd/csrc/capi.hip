@@ -470,12 +470,13 @@ int torch_state_create(torch_ctx* ctx, const torch_config* cfg, torch_state** ou
     s->ctx = ctx;
     s->cfg = c;
     s->ops = c.precision == TORCH_FP64 ? &kOps64 : &kOps32;
-    // Tuned default tile: 32x16 threads (30x14 cells per plane) marching
-    // 16 planes, the fastest shape of the autotuner's sweep for both
-    // precisions on MI355X from 96^3 to 256^3.
+    // Tuned default tile, marching 16 planes: FP64 32x16 threads (30x14
+    // cells per plane, two waves per SIMD at 256 VGPRs); FP32 32x8 (three
+    // 256-thread blocks per CU at <= 168 VGPRs). The fastest shapes of the
+    // autotuner's sweep on MI355X from 96^3 to 256^3.
     if (s->cfg.block_x == 0 || s->cfg.block_y == 0) {
         s->cfg.block_x = 32;
-        s->cfg.block_y = 16;
+        s->cfg.block_y = c.precision == TORCH_FP32 ? 8 : 16;
     }
     if (s->cfg.zchunk <= 0) s->cfg.zchunk = 16;
     fvl::Grid& g = s->grid;

@@ -145,25 +145,33 @@ __device__ __forceinline__ void step2d_strip(const Step& s, const G2& g, int ych
             dxs[c] = limited<LIM, 2>(Ac[c] - Al[c], Ar[c] - Ac[c]);
             dys[c] = limited<LIM, 2>(Ac[c] - Ap[c], An[c] - Ac[c]);
         }
+        // Zero slopes in a direction give equal half states and an exact
+        // zero change: its two flux evaluations are skipped.
         real ub[4], a[4], b[4], fa[4], fb[4];
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            a[c] = Ac[c] + (real)0.5 * dxs[c];
-            b[c] = Ac[c] - (real)0.5 * dxs[c];
+        for (int c = 0; c < 4; ++c) ub[c] = Ac[c];
+        if (!((dxs[0] == (real)0) & (dxs[1] == (real)0) & (dxs[2] == (real)0) & (dxs[3] == (real)0))) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                a[c] = Ac[c] + (real)0.5 * dxs[c];
+                b[c] = Ac[c] - (real)0.5 * dxs[c];
+            }
+            flux2f(a, gc, 0, fa);
+            flux2f(b, gc, 0, fb);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) ub[c] = Ac[c] - h0 * (fa[c] - fb[c]);
         }
-        flux2f(a, gc, 0, fa);
-        flux2f(b, gc, 0, fb);
+        if (!((dys[0] == (real)0) & (dys[1] == (real)0) & (dys[2] == (real)0) & (dys[3] == (real)0))) {
 #pragma unroll
-        for (int c = 0; c < 4; ++c) ub[c] = Ac[c] - h0 * (fa[c] - fb[c]);
+            for (int c = 0; c < 4; ++c) {
+                a[c] = Ac[c] + (real)0.5 * dys[c];
+                b[c] = Ac[c] - (real)0.5 * dys[c];
+            }
+            flux2f(a, gc, 1, fa);
+            flux2f(b, gc, 1, fb);
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            a[c] = Ac[c] + (real)0.5 * dys[c];
-            b[c] = Ac[c] - (real)0.5 * dys[c];
+            for (int c = 0; c < 4; ++c) ub[c] = ub[c] - h1 * (fa[c] - fb[c]);
         }
-        flux2f(a, gc, 1, fa);
-        flux2f(b, gc, 1, fb);
-#pragma unroll
-        for (int c = 0; c < 4; ++c) ub[c] = ub[c] - h1 * (fa[c] - fb[c]);
         // Positivity repair (torchSolver2D.py:367-377).
         ub[0] = rmax(ub[0], (real)1e-12);
         real kin = ((real)0.5 * (ub[1] * ub[1] + ub[2] * ub[2])) / ub[0];

@@ -127,6 +127,11 @@ __device__ __forceinline__ void wave3(const real* o, double g, double (&mx)[3]) 
 #endif
 }
 
+// Waves per SIMD the register allocation of a step kernel aims for. A
+// 256-thread FP32 tile fits three blocks per CU at <= 168 VGPRs, which hides
+// the z-march's load and barrier latency better than two.
+constexpr int occupancy_of(int threads) { return (sizeof(real) == 4 && threads == 256) ? 3 : 1; }
+
 // LDS footprint of one tile: the A plane with a two-cell halo (reused for
 // the y fluxes) and the y face states.
 template <int BX, int BY, bool SPEC>
@@ -240,14 +245,19 @@ __device__ __forceinline__ void step3d_tile(const Step& s, const G3& g, int tbx,
         real P[5];
 #pragma unroll
         for (int c = 0; c < 5; ++c) P[c] = Ac[c];
+        bool flat[3];
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+            const real* sl = m == 0 ? sx : (m == 1 ? sy : sz);
+            flat[m] = (sl[0] == (real)0) & (sl[1] == (real)0) & (sl[2] == (real)0) & (sl[3] == (real)0) &
+                      (sl[4] == (real)0);
+        }
 #pragma unroll
         for (int m = 0; m < 3; ++m) {
             const real* sl = m == 0 ? sx : (m == 1 ? sy : sz);
             // Zero slopes give equal half states, equal fluxes and an exact
             // zero change: skip the two flux evaluations.
-            const bool flat = (sl[0] == (real)0) & (sl[1] == (real)0) & (sl[2] == (real)0) & (sl[3] == (real)0) &
-                              (sl[4] == (real)0);
-            if (!flat) {
+            if (!flat[m]) {
                 real up[5], um[5], fp[5], fm[5];
 #pragma unroll
                 for (int c = 0; c < 5; ++c) {
@@ -271,8 +281,13 @@ __device__ __forceinline__ void step3d_tile(const Step& s, const G3& g, int tbx,
         }
         // Positivity of every active predicted face state (:152-159).
         if (in_active && k >= -1 && k <= nz) {
-            const bool ok = physical5(xl) & physical5(xr) & physical5(yl) & physical5(yr) & physical5(zl) &
-                            physical5(zn);
+            // Both face states of a flat direction are P itself.
+            bool ok = true;
+            if (flat[0] & flat[1] & flat[2]) {
+                ok = physical5(P);
+            } else {
+                ok = physical5(xl) & physical5(xr) & physical5(yl) & physical5(yr) & physical5(zl) & physical5(zn);
+            }
             if (!ok) bad |= fvl::F_PRED_BAD;
         }
 #pragma unroll
@@ -409,7 +424,7 @@ __device__ __forceinline__ void step3d_tile(const Step& s, const G3& g, int tbx,
 }
 
 template <int LIM, int BX, int BY, bool SPEC>
-__global__ void __launch_bounds__(BX* BY) k_step3d(Step s, G3 g) {
+__global__ void __launch_bounds__(BX* BY, occupancy_of(BX* BY)) k_step3d(Step s, G3 g) {
     using T = Tile3<BX, BY, SPEC>;
     __shared__ real sbuf[T::BUF];
     __shared__ real sy[T::YS];
@@ -423,7 +438,7 @@ __global__ void __launch_bounds__(BX* BY) k_step3d(Step s, G3 g) {
 // instance's tile counts. Instances that are idle this step (coef active flag
 // clear) only publish empty partials.
 template <int LIM, int BX, int BY, bool SPEC>
-__global__ void __launch_bounds__(BX* BY) k_batch3d(const fvl::BatchItem* items, const int* first, int n) {
+__global__ void __launch_bounds__(BX* BY, occupancy_of(BX* BY)) k_batch3d(const fvl::BatchItem* items, const int* first, int n) {
     using T = Tile3<BX, BY, SPEC>;
     __shared__ real sbuf[T::BUF];
     __shared__ real sy[T::YS];

@@ -328,7 +328,9 @@ __device__ __forceinline__ void hllc_cart4(const real* l, const real* r, real gl
     int t = dir == 0 ? 2 : 1;
     a[0] = l[0]; a[1] = l[dir + 1]; a[2] = l[t]; a[3] = l[3];
     b[0] = r[0]; b[1] = r[dir + 1]; b[2] = r[t]; b[3] = r[3];
-    hllc<4>(a, b, gl, gr, o);
+    const bool equal = (a[0] == b[0]) & (a[1] == b[1]) & (a[2] == b[2]) & (a[3] == b[3]) & (gl == gr);
+    if (equal) hllc_equal<4>(a, gl, o);
+    else hllc<4>(a, b, gl, gr, o);
     f[0] = o[0]; f[dir + 1] = o[1]; f[t] = o[2]; f[3] = o[3];
 }
 
