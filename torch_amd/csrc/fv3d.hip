@@ -129,8 +129,12 @@ __device__ __forceinline__ void wave3(const real* o, double g, double (&mx)[3]) 
 
 // Waves per SIMD the register allocation of a step kernel aims for. A
 // 256-thread FP32 tile fits three blocks per CU at <= 168 VGPRs, which hides
-// the z-march's load and barrier latency better than two.
-constexpr int occupancy_of(int threads) { return (sizeof(real) == 4 && threads == 256) ? 3 : 1; }
+// the z-march's load and barrier latency better than two; a 256-thread FP64
+// tile is held to two (<= 256 VGPRs, no accumulation registers) so two
+// blocks share a CU instead of one.
+constexpr int occupancy_of(int threads) {
+    return threads == 256 ? (sizeof(real) == 4 ? 3 : 2) : 1;
+}
 
 // LDS footprint of one tile: the A plane with a two-cell halo (reused for
 // the y fluxes) and the y face states.
