@@ -129,11 +129,11 @@ __device__ __forceinline__ void wave3(const real* o, double g, double (&mx)[3]) 
 
 // Waves per SIMD the register allocation of a step kernel aims for. A
 // 256-thread FP32 tile fits three blocks per CU at <= 168 VGPRs, which hides
-// the z-march's load and barrier latency better than two; a 256-thread FP64
-// tile is held to two (<= 256 VGPRs, no accumulation registers) so two
-// blocks share a CU instead of one.
-constexpr int occupancy_of(int threads) {
-    return threads == 256 ? (sizeof(real) == 4 ? 3 : 2) : 1;
+// the z-march's load and barrier latency better than two; 256-thread FP64
+// and species-carrying tiles are held to two (<= 256 VGPRs, no accumulation
+// registers) so two blocks share a CU instead of one.
+constexpr int occupancy_of(int threads, bool spec = false) {
+    return threads == 256 ? (sizeof(real) == 4 && !spec ? 3 : 2) : 1;
 }
 
 // LDS footprint of one tile: the A plane with a two-cell halo (reused for
@@ -217,7 +217,7 @@ __device__ __forceinline__ void step3d_tile(const Step& s, const G3& g, int tbx,
         gp = gamma_from(load1(lamp, g, x, y, z0 - 2), direct);
         gc = gamma_from(load1(lamp, g, x, y, z0 - 1), direct);
     }
-    for (int j = 0; j < nq; ++j) {
+    for (int j = 0; j < NQ; ++j) if (j < nq) {
         qp[j] = load1((const real*)s.q[j], g, x, y, z0 - 2);
         qc[j] = load1((const real*)s.q[j], g, x, y, z0 - 1);
     }
@@ -225,7 +225,7 @@ __device__ __forceinline__ void step3d_tile(const Step& s, const G3& g, int tbx,
     for (int k = z0 - 1; k <= z1; ++k) {
         load5(U, N, g, x, y, k + 1, An);
         if (lamp) gn = gamma_from(load1(lamp, g, x, y, k + 1), direct);
-        for (int j = 0; j < nq; ++j) qn[j] = load1((const real*)s.q[j], g, x, y, k + 1);
+        for (int j = 0; j < NQ; ++j) if (j < nq) qn[j] = load1((const real*)s.q[j], g, x, y, k + 1);
 
         __syncthreads();  // previous iteration's readers of sF are done
 #pragma unroll
@@ -310,7 +310,7 @@ __device__ __forceinline__ void step3d_tile(const Step& s, const G3& g, int tbx,
         for (int j = 0; j < NQ; ++j) lq[j] = shfl_up1(qc[j]);
         if (plane_out && xface) {
             hllc_cart5(lxr, xl, lg, gc, 0, Fx);
-            for (int j = 0; j < nq; ++j) sfx[j] = Fx[0] * (Fx[0] >= (real)0 ? lq[j] : qc[j]);
+            for (int j = 0; j < NQ; ++j) if (j < nq) sfx[j] = Fx[0] * (Fx[0] >= (real)0 ? lq[j] : qc[j]);
         }
         // y face y-1/2 from the row below.
         real Fy[5] = {0, 0, 0, 0, 0}, sfy[2] = {0, 0};
@@ -319,14 +319,14 @@ __device__ __forceinline__ void step3d_tile(const Step& s, const G3& g, int tbx,
 #pragma unroll
             for (int c = 0; c < 5; ++c) below[c] = sY[c][row - 1][lane];
             hllc_cart5(below, yl, sY[5][row - 1][lane], gc, 1, Fy);
-            for (int j = 0; j < nq; ++j) sfy[j] = Fy[0] * (Fy[0] >= (real)0 ? sY[6 + j][row - 1][lane] : qc[j]);
+            for (int j = 0; j < NQ; ++j) if (j < nq) sfy[j] = Fy[0] * (Fy[0] >= (real)0 ? sY[6 + j][row - 1][lane] : qc[j]);
         }
         // z face k-1/2 from the carried state of plane k-1.
         real Fz[5] = {0, 0, 0, 0, 0}, sfz[2] = {0, 0};
         const bool zface = owns && k >= z0;
         if (zface) {
             hllc_cart5(zr, zl, gp, gc, 2, Fz);
-            for (int j = 0; j < nq; ++j) sfz[j] = Fz[0] * (Fz[0] >= (real)0 ? qp[j] : qc[j]);
+            for (int j = 0; j < NQ; ++j) if (j < nq) sfz[j] = Fz[0] * (Fz[0] >= (real)0 ? qp[j] : qc[j]);
         }
 #pragma unroll
         for (int c = 0; c < 5; ++c) sF[c][row][lane] = Fy[c];
@@ -348,7 +348,7 @@ __device__ __forceinline__ void step3d_tile(const Step& s, const G3& g, int tbx,
             for (int c = 0; c < 5; ++c) Uo[c * N + i] = o[c];
             if (!physical5(o)) bad |= fvl::F_UPD_BAD;
             if (s.want_cfl) wave3r(o, gconst, mxr);
-            for (int j = 0; j < nq; ++j) {
+            for (int j = 0; j < NQ; ++j) if (j < nq) {
                 real v = rq[j] - c2 * (sfz[j] - sfzp[j]);
                 real f = v / o[0];
                 if (f < -kSpecTol || f > (real)1 + kSpecTol || f != f) bad |= fvl::F_SPEC_BAD;
@@ -365,7 +365,7 @@ __device__ __forceinline__ void step3d_tile(const Step& s, const G3& g, int tbx,
                 R[c] = Ac[c] - c0 * (FxR[c] - Fx[c]);
                 R[c] = R[c] - c1 * (FyT[c] - Fy[c]);
             }
-            for (int j = 0; j < nq; ++j) {
+            for (int j = 0; j < NQ; ++j) if (j < nq) {
                 rq[j] = Ac[0] * qc[j] - c0 * (sfxR[j] - sfx[j]);
                 rq[j] = rq[j] - c1 * (sF[5 + j][row + 1][lane] - sfy[j]);
             }
@@ -428,7 +428,7 @@ __device__ __forceinline__ void step3d_tile(const Step& s, const G3& g, int tbx,
 }
 
 template <int LIM, int BX, int BY, bool SPEC>
-__global__ void __launch_bounds__(BX* BY, occupancy_of(BX* BY)) k_step3d(Step s, G3 g) {
+__global__ void __launch_bounds__(BX* BY, occupancy_of(BX* BY, SPEC)) k_step3d(Step s, G3 g) {
     using T = Tile3<BX, BY, SPEC>;
     __shared__ real sbuf[T::BUF];
     __shared__ real sy[T::YS];
@@ -442,7 +442,7 @@ __global__ void __launch_bounds__(BX* BY, occupancy_of(BX* BY)) k_step3d(Step s,
 // instance's tile counts. Instances that are idle this step (coef active flag
 // clear) only publish empty partials.
 template <int LIM, int BX, int BY, bool SPEC>
-__global__ void __launch_bounds__(BX* BY, occupancy_of(BX* BY)) k_batch3d(const fvl::BatchItem* items, const int* first, int n) {
+__global__ void __launch_bounds__(BX* BY, occupancy_of(BX* BY, SPEC)) k_batch3d(const fvl::BatchItem* items, const int* first, int n) {
     using T = Tile3<BX, BY, SPEC>;
     __shared__ real sbuf[T::BUF];
     __shared__ real sy[T::YS];
@@ -495,12 +495,13 @@ __device__ __forceinline__ void llf3d_cells(const Step& s, const G3& g, long sta
         const bool direct = s.lam_is_gamma != 0;
         const real gcen = lamp ? gamma_from(load1(lamp, g, x, y, z), direct) : (real)s.gamma;
         real qcen[2] = {0, 0};
-        for (int j = 0; j < s.nq; ++j) {
+        for (int j = 0; j < 2; ++j) if (j < s.nq) {
             qcen[j] = load1((const real*)s.q[j], g, x, y, z);
             rqv[j] = Ac[0] * qcen[j];
         }
 #pragma unroll
         for (int c = 0; c < 5; ++c) o[c] = Ac[c];
+#pragma unroll
         for (int m = 0; m < 3; ++m) {
             const int dx = m == 0, dy = m == 1, dz = m == 2;
             real Lo[5], Hi[5], Fl[5], Fh[5];
@@ -513,7 +514,7 @@ __device__ __forceinline__ void llf3d_cells(const Step& s, const G3& g, long sta
             const real cm = (real)K.c[m];
 #pragma unroll
             for (int c = 0; c < 5; ++c) o[c] = o[c] - cm * (Fh[c] - Fl[c]);
-            for (int j = 0; j < s.nq; ++j) {
+            for (int j = 0; j < 2; ++j) if (j < s.nq) {
                 const real* q = (const real*)s.q[j];
                 real qlo = load1(q, g, x - dx, y - dy, z - dz), qhi = load1(q, g, x + dx, y + dy, z + dz);
                 real sl = Fl[0] * (Fl[0] >= (real)0 ? qlo : qcen[j]);
@@ -539,7 +540,7 @@ __device__ __forceinline__ void llf3d_cells(const Step& s, const G3& g, long sta
         for (int c = 0; c < 5; ++c) ((real*)s.Uout)[c * N + i] = o[c];
         if (!physical5(o)) bad |= fvl::F_LLF_BAD;
         if (s.want_cfl) wave3(o, (double)s.gamma, mx);
-        for (int j = 0; j < s.nq; ++j) {
+        for (int j = 0; j < 2; ++j) if (j < s.nq) {
             real f = rqv[j] / o[0];
             if (f < -kSpecTol || f > (real)1 + kSpecTol || f != f) bad |= fvl::F_SPEC_BAD_LLF;
             ((real*)s.qout[j])[i] = rmin(rmax(f, (real)0), (real)1);
