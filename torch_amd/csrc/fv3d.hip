@@ -136,8 +136,9 @@ constexpr int occupancy_of(int threads, bool spec = false) {
     return threads == 256 ? (sizeof(real) == 4 && !spec ? 3 : 2) : 1;
 }
 
-// LDS footprint of one tile: the A plane with a two-cell halo (reused for
-// the y fluxes) and the y face states.
+// LDS footprint of one tile: the A plane with a two-cell halo, the y face
+// states and the y fluxes (separate buffers, so a plane needs three barriers:
+// A plane written, y face states written, y fluxes written).
 template <int BX, int BY, bool SPEC>
 struct Tile3 {
     static constexpr int NQ = SPEC ? 2 : 0;
@@ -146,7 +147,7 @@ struct Tile3 {
     static constexpr int NFY = 5 + NQ;
     static constexpr int PLANE = 5 * PW * PH;
     static constexpr int FLUX = NFY * BX * BY;
-    static constexpr int BUF = PLANE > FLUX ? PLANE : FLUX;
+    static constexpr int BUF = PLANE;
     static constexpr int YS = NYS * BY * BX;
 };
 
@@ -157,13 +158,13 @@ using fvl::coef_of;
 // block's slot in s.partials.
 template <int LIM, int BX, int BY, bool SPEC>
 __device__ __forceinline__ void step3d_tile(const Step& s, const G3& g, int tbx, int tby, int tbz, long pblk,
-                                            real* sbuf, real* sybuf) {
+                                            real* sbuf, real* sybuf, real* sfbuf) {
     using T = Tile3<BX, BY, SPEC>;
     constexpr int TX = BX - 2, TY = BY - 2;
     constexpr int NQ = T::NQ;
     constexpr int PW = T::PW, PH = T::PH;
     real(*sA)[PH][PW] = reinterpret_cast<real(*)[PH][PW]>(sbuf);
-    real(*sF)[BY][BX] = reinterpret_cast<real(*)[BY][BX]>(sbuf);
+    real(*sF)[BY][BX] = reinterpret_cast<real(*)[BY][BX]>(sfbuf);
     real(*sY)[BY][BX] = reinterpret_cast<real(*)[BY][BX]>(sybuf);
 
     const int lane = threadIdx.x, row = threadIdx.y;
@@ -227,7 +228,8 @@ __device__ __forceinline__ void step3d_tile(const Step& s, const G3& g, int tbx,
         if (lamp) gn = gamma_from(load1(lamp, g, x, y, k + 1), direct);
         for (int j = 0; j < NQ; ++j) if (j < nq) qn[j] = load1((const real*)s.q[j], g, x, y, k + 1);
 
-        __syncthreads();  // previous iteration's readers of sF are done
+        // No barrier needed before rewriting sA: its last readers finished
+        // before the previous plane's second barrier.
 #pragma unroll
         for (int c = 0; c < 5; ++c) sA[c][ly + 2][lx + 2] = Ac[c];
         if (tid < nring) {
@@ -298,7 +300,7 @@ __device__ __forceinline__ void step3d_tile(const Step& s, const G3& g, int tbx,
         for (int c = 0; c < 5; ++c) sY[c][row][lane] = yr[c];
         sY[5][row][lane] = gc;
         for (int j = 0; j < NQ; ++j) sY[6 + j][row][lane] = qc[j];
-        __syncthreads();  // sA readers done; sY visible
+        __syncthreads();  // sY visible
 
         const bool plane_out = k >= z0 && k < z1;
         // x face x-1/2 from the left neighbour's right state.
@@ -432,8 +434,9 @@ __global__ void __launch_bounds__(BX* BY, occupancy_of(BX* BY, SPEC)) k_step3d(S
     using T = Tile3<BX, BY, SPEC>;
     __shared__ real sbuf[T::BUF];
     __shared__ real sy[T::YS];
+    __shared__ real sf[T::FLUX];
     const long b = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
-    step3d_tile<LIM, BX, BY, SPEC>(s, g, blockIdx.x, blockIdx.y, blockIdx.z, b, sbuf, sy);
+    step3d_tile<LIM, BX, BY, SPEC>(s, g, blockIdx.x, blockIdx.y, blockIdx.z, b, sbuf, sy, sf);
 }
 
 // Work-list form: blocks of many instances of different sizes and
@@ -446,6 +449,7 @@ __global__ void __launch_bounds__(BX* BY, occupancy_of(BX* BY, SPEC)) k_batch3d(
     using T = Tile3<BX, BY, SPEC>;
     __shared__ real sbuf[T::BUF];
     __shared__ real sy[T::YS];
+    __shared__ real sf[T::FLUX];
     const int b = blockIdx.x;
     const int lo = fvl::owner(first, n, b);
     const fvl::BatchItem& it = fvl::item_at(items, lo);
@@ -459,7 +463,7 @@ __global__ void __launch_bounds__(BX* BY, occupancy_of(BX* BY, SPEC)) k_batch3d(
         block_partials<BX * BY>(acc, mx, 0.0, s.partials, s.flags, 0, local);
         return;
     }
-    step3d_tile<LIM, BX, BY, SPEC>(s, g, tbx, tby, tbz, local, sbuf, sy);
+    step3d_tile<LIM, BX, BY, SPEC>(s, g, tbx, tby, tbz, local, sbuf, sy, sf);
 }
 
 // First-order LLF pass over the whole grid, run only when the second-order
