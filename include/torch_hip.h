@@ -175,6 +175,12 @@ int torch_batch_destroy(torch_batch* batch);
  * beta NULL means ones). TORCH_BUDGET when it does not fit now. */
 int torch_batch_add(torch_batch* batch, const torch_config* config, const double* U, const double* lam,
                     const double* beta, double time, double t_end, int max_steps, int* id);
+/* Starts an admitted instance's boundary and source integrals (dim+2 values
+ * each; NULL leaves zeros) at a session's running totals, before its first
+ * step, so the device adds each step's terms in the session's order
+ * (torchSession.py:518-521) and a session resumed in a batch keeps its
+ * balances bit for bit. */
+int torch_batch_seed(torch_batch* batch, int id, const double* outward, const double* source);
 int torch_batch_remove(torch_batch* batch, int id);
 /* Advances every active instance by up to `steps` steps, then waits. An
  * instance stops at t_end, at max_steps, or on the error its status names

@@ -5,13 +5,15 @@ python/cfdSolver): ``Simulation`` (torchSession), ``Execution`` and its
 device drop-in ``DeviceExecution`` (torchExecution), ``hllc_flux_grid`` /
 ``hllc_flux`` (torchFlux), ``muscl_hancock_step_3d`` / ``reaction_step_3d``
 (torchSolver3D) and ``muscl_hancock_step_2d`` / ``reaction_step_2d``
-(torchSolver2D). The arithmetic runs in libtorchhip.so (HIP kernels for
+(torchSolver2D), and ``read_checkpoint`` / ``write_checkpoint`` (torchIO,
+schema-2 layout). The arithmetic runs in libtorchhip.so (HIP kernels for
 gfx950, built by ``python -m torch_amd.build``) behind the C ABI in
 include/torch_hip.h; there is no CPU fallback.
 """
 
 from .torchExecution import DeviceExecution, Execution, validate_workers  # noqa: F401
 from .torchFlux import hllc_flux, hllc_flux_grid  # noqa: F401
+from .torchIO import read_checkpoint, write_checkpoint  # noqa: F401
 from .torchSession import Simulation  # noqa: F401
 from .torchSolver2D import muscl_hancock_step_2d, reaction_step_2d  # noqa: F401
 from .torchSolver3D import (  # noqa: F401

@@ -30,7 +30,7 @@ EXPORTS = (
     "torch_state_step", "torch_state_hydro", "torch_state_fluxes", "torch_state_react",
     "torch_state_totals", "torch_state_profile", "torch_hllc_faces",
     "torch_batch_footprint", "torch_batch_create", "torch_batch_destroy", "torch_batch_add",
-    "torch_batch_remove", "torch_batch_run", "torch_batch_info", "torch_batch_download",
+    "torch_batch_remove", "torch_batch_seed", "torch_batch_run", "torch_batch_info", "torch_batch_download",
     "torch_batch_tune", "torch_batch_set_tiles", "torch_batch_profile", "torch_batch_time_pass",
 )
 
@@ -111,6 +111,7 @@ def load():
                 "torch_batch_add": (ctypes.c_int, [vp, ctypes.POINTER(Config), _dp, _dp, _dp, ctypes.c_double,
                                                    ctypes.c_double, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
                 "torch_batch_remove": (ctypes.c_int, [vp, ctypes.c_int]),
+                "torch_batch_seed": (ctypes.c_int, [vp, ctypes.c_int, _dp, _dp]),
                 "torch_batch_run": (ctypes.c_int, [vp, ctypes.c_int]),
                 "torch_batch_info": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(BatchProgress)]),
                 "torch_batch_download": (ctypes.c_int, [vp, ctypes.c_int, _dp, _dp, _dp]),
@@ -322,6 +323,13 @@ class DeviceBatch:
     def remove(self, ident):
         check(load().torch_batch_remove(self.handle, int(ident)))
         self.shapes.pop(ident, None)
+
+    def seed(self, ident, outward, source):
+        """Start an instance's boundary/source integrals at a session's totals."""
+        nf = self.shapes[ident][0][0]
+        out = np.ascontiguousarray(np.asarray(outward, dtype=np.float64).reshape(nf))
+        src = np.ascontiguousarray(np.asarray(source, dtype=np.float64).reshape(nf))
+        check(load().torch_batch_seed(self.handle, int(ident), ptr(out), ptr(src)))
 
     def run(self, steps):
         check(load().torch_batch_run(self.handle, int(steps)))

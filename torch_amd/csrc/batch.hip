@@ -543,6 +543,26 @@ int torch_batch_add(torch_batch* b, const torch_config* config, const double* U,
     return TORCH_OK;
 }
 
+int torch_batch_seed(torch_batch* b, int id, const double* outward, const double* source) {
+    if (!b) return fail(TORCH_INVALID, "null argument");
+    std::lock_guard<std::mutex> lock(b->ctx->mu);
+    for (Inst* p : b->inst)
+        if (p->id == id) {
+            // Until its first step the host copy of the control block is the
+            // one the next rebuild uploads.
+            if (!p->fresh) return fail(TORCH_INVALID, "integrals can be seeded only before an instance's first step");
+            for (int f = 0; f < p->nf; ++f)
+                if ((outward && !isfinite(outward[f])) || (source && !isfinite(source[f])))
+                    return fail(TORCH_INVALID, "integrals must be finite");
+            for (int f = 0; f < p->nf; ++f) {
+                if (outward) p->ctl.outward[f] = outward[f];
+                if (source) p->ctl.source[f] = source[f];
+            }
+            return TORCH_OK;
+        }
+    return fail(TORCH_INVALID, "no such batch instance");
+}
+
 int torch_batch_remove(torch_batch* b, int id) {
     if (!b) return fail(TORCH_INVALID, "null argument");
     std::lock_guard<std::mutex> lock(b->ctx->mu);

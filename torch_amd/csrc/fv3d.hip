@@ -86,45 +86,39 @@ __device__ void block_partials(double (&acc)[5], double (&mx)[3], double src, do
     }
 }
 
-// |u_m| + a of a 3D state accumulated as maxima in the storage precision
-// (widening to double afterwards is exact, so this equals reducing in
-// double); a NaN speed sticks.
-__device__ __forceinline__ void wave3r(const real* o, real g, real (&mx)[3]) {
+// |u_m| + a of a committed 3D state (torchSession.py:356-360) in the storage
+// precision. Every 3D CFL maximum goes through this one formula, whether the
+// update, the chemistry half-step or a standalone pass finds it, so a
+// state's dt does not depend on which pass saw it last (a resumed session
+// steps on exactly). Contraction is off so it rounds the same wherever it
+// is inlined.
+__device__ __forceinline__ void speeds3(const real* o, real g, real (&v)[3]) {
+#pragma clang fp contract(off)
     const real ir = rcp(o[0]);
     const real p = (g - (real)1) * (o[4] - ((real)0.5 * (o[1] * o[1] + o[2] * o[2] + o[3] * o[3])) * ir);
     const real a = rsqrt_((g * p) * ir);
 #pragma unroll
-    for (int m = 0; m < 3; ++m) {
-        const real v = fabs(o[m + 1] * ir) + a;
-        mx[m] = (v != v || v > mx[m]) ? v : mx[m];
-    }
+    for (int m = 0; m < 3; ++m) v[m] = fabs(o[m + 1] * ir) + a;
 }
 
-// |u_m| + a of a committed 3D state (torchSession.py:356-360), reduced in
-// double. The FP32 form evaluates the speeds in float (exact when widened).
+// Maxima of speeds3 kept in the storage precision (widening to double
+// afterwards is exact, so this equals reducing in double); a NaN sticks.
+__device__ __forceinline__ void wave3r(const real* o, real g, real (&mx)[3]) {
+    real v[3];
+    speeds3(o, g, v);
+#pragma unroll
+    for (int m = 0; m < 3; ++m) mx[m] = (v[m] != v[m] || v[m] > mx[m]) ? v[m] : mx[m];
+}
+
+// Maxima of speeds3 reduced in double.
 __device__ __forceinline__ void wave3(const real* o, double g, double (&mx)[3]) {
-#ifdef FV_FAST
-    const real ir = rcp(o[0]), gg = (real)g;
-    const real p = (gg - (real)1) * (o[4] - ((real)0.5 * (o[1] * o[1] + o[2] * o[2] + o[3] * o[3])) * ir);
-    const real a = rsqrt_((gg * p) * ir);
+    real v[3];
+    speeds3(o, (real)g, v);
 #pragma unroll
     for (int m = 0; m < 3; ++m) {
-        double v = (double)(fabs(o[m + 1] * ir) + a);
-        mx[m] = (v != v || v > mx[m]) ? v : mx[m];
+        const double w = (double)v[m];
+        mx[m] = (w != w || w > mx[m]) ? w : mx[m];
     }
-#else
-    double u[5];
-#pragma unroll
-    for (int c = 0; c < 5; ++c) u[c] = (double)o[c];
-    const double ir = 1.0 / u[0];
-    double p = (g - 1.0) * (u[4] - (0.5 * (u[1] * u[1] + u[2] * u[2] + u[3] * u[3])) * ir);
-    double a = sqrt((g * p) * ir);
-#pragma unroll
-    for (int m = 0; m < 3; ++m) {
-        double v = fabs(u[m + 1] * ir) + a;
-        mx[m] = (v != v || v > mx[m]) ? v : mx[m];
-    }
-#endif
 }
 
 // Waves per SIMD the register allocation of a step kernel aims for. A
@@ -645,13 +639,11 @@ __device__ __forceinline__ void react_cells(const fvl::React& r, const Grid& g, 
         if (!(o[0] > 1e-12 && o[nf - 1] - (0.5 * s2) / o[0] > 1e-12 && isfinite(o[nf - 1])))
             bad |= fvl::F_STATE_BAD;
         if (r.want_cfl && g.dim == 3) {
-            double gg = (4.5 - (double)(real)lam) / (3.5 - (double)(real)lam);
-            double p = (gg - 1.0) * (o[4] - (0.5 * (o[1] * o[1] + o[2] * o[2] + o[3] * o[3])) / o[0]);
-            double a = sqrt((gg * p) / o[0]);
-            for (int m = 0; m < 3; ++m) {
-                double v = fabs(o[m + 1] / o[0]) + a;
-                mx[m] = (v != v || v > mx[m]) ? v : mx[m];
-            }
+            // The stored state and fraction, as a standalone CFL pass reads them.
+            real so[5];
+            for (int c = 0; c < 5; ++c) so[c] = Uo[c * N + i];
+            const double ls = (double)(real)lam;
+            wave3(so, (4.5 - ls) / (3.5 - ls), mx);
         }
     }
     block_partials<256>(acc, mx, src, r.partials, r.flags, bad, pblk);

@@ -349,7 +349,10 @@ __device__ __forceinline__ real gamma_from(real v, bool direct) { return direct 
 struct Prim2 {
     real rho, ux, vy, p, a;
 };
+// Contraction is off: the 2D CFL speeds come from here, and they must round
+// the same in the fused update and in a standalone pass over a stored state.
 __device__ __forceinline__ Prim2 prim2(const real* u, real g) {
+#pragma clang fp contract(off)
     Prim2 q;
     q.rho = rmax(u[0], (real)1e-12);
     q.ux = u[1] / q.rho;
@@ -360,9 +363,24 @@ __device__ __forceinline__ Prim2 prim2(const real* u, real g) {
     return q;
 }
 
-// 64-bit-safe lane shuffles for one wave (width 64).
-__device__ __forceinline__ real shfl_up1(real v) { return __shfl_up(v, 1, 64); }
-__device__ __forceinline__ real shfl_down1(real v) { return __shfl_down(v, 1, 64); }
+// Whole-wave shifts by one lane on the DPP path (wave_shr:1 = 0x138,
+// wave_shl:1 = 0x130): a VALU move instead of an LDS permute round trip.
+// Lane i receives lane i-1 (up) or lane i+1 (down); the edge lane (0 or 63)
+// receives 0. Callers use neither edge lane's result.
+__device__ __forceinline__ int dpp_up1(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, false); }
+__device__ __forceinline__ int dpp_down1(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xf, 0xf, false); }
+__device__ __forceinline__ float shfl_up1(float v) { return __int_as_float(dpp_up1(__float_as_int(v))); }
+__device__ __forceinline__ float shfl_down1(float v) { return __int_as_float(dpp_down1(__float_as_int(v))); }
+__device__ __forceinline__ double shfl_up1(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_up1((int)b), hi = dpp_up1((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double shfl_down1(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_down1((int)b), hi = dpp_down1((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
 
 // Wave/block max of a nonnegative double (order independent, so exact).
 __device__ __forceinline__ double wave_max(double v) {

@@ -56,7 +56,10 @@ class Batch:
         max_steps = c["max_steps"] - sim.steps if max_steps is None else int(max_steps)
         ident = self._batch.add(cfg, sim.state, sim.time, t_end, max_steps, lam=sim.lam, beta=sim.beta)
         if ident is not None:
-            self._meta[ident] = (c["dimension"] + 2, sim.steps)
+            # The device adds each step's terms to the session's running totals,
+            # in the session's order.
+            self._batch.seed(ident, sim.boundary_integral, sim.source_integral)
+            self._meta[ident] = (c["dimension"] + 2, sim.steps, sim.fallbacks)
         return ident
 
     def add_problem(self, primitive, lengths, *, t_end, cfl=0.4, gamma=1.4, boundaries=None, limiter="VanLeer",
@@ -83,7 +86,7 @@ class Batch:
         state[-1] = primitive[-1] / (gamma - 1) + 0.5 * primitive[0] * np.sum(primitive[1:-1] ** 2, axis=0)
         ident = self._batch.add(cfg, state, time, t_end, max_steps)
         if ident is not None:
-            self._meta[ident] = (dim + 2, 0)
+            self._meta[ident] = (dim + 2, 0, 0)
         return ident
 
     def footprint(self, sim):
@@ -117,6 +120,9 @@ class Batch:
 
     # -- results -------------------------------------------------------------
     def info(self, ident):
+        """Progress of an instance. Its boundary and source integrals are the
+        running totals: an instance admitted from a session starts from the
+        session's totals, one admitted from primitives from zero."""
         p = self._batch.info(ident)
         nf = self._meta[ident][0]
         return dict(time=p.time, dt=p.dt, steps=p.steps, fallbacks=p.fallbacks, status=p.status,
@@ -146,14 +152,14 @@ class Batch:
     def commit(self, ident, sim):
         """Write an instance's progress back into its Simulation."""
         info = self.info(ident)
-        start_steps = self._meta[ident][1]
+        _, start_steps, start_fallbacks = self._meta[ident]
         state, lam, beta = self._batch.download(ident)
         sim._upload(state, lam, beta)
         sim.time = info["time"]
         sim.steps = start_steps + info["steps"]
-        sim.fallbacks += info["fallbacks"]
-        sim.boundary_integral = sim.boundary_integral + info["boundary_integral"]
-        sim.source_integral = sim.source_integral + info["source_integral"]
+        sim.fallbacks = start_fallbacks + info["fallbacks"]
+        sim.boundary_integral = info["boundary_integral"]
+        sim.source_integral = info["source_integral"]
         return sim
 
     def close(self):

@@ -102,7 +102,7 @@ class Server:
         cfg, state, lam, beta, t_end = payload
         ident = self.batch._batch.add(cfg, state, 0.0, t_end, self.max_steps, lam=lam, beta=beta)
         if ident is not None:
-            self.batch._meta[ident] = (cfg.dim + 2, 0)
+            self.batch._meta[ident] = (cfg.dim + 2, 0, 0)
             req.cells = int(np.prod(state.shape[1:]))
         return ident
 

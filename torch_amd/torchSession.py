@@ -4,8 +4,9 @@ Mirrors the Cartesian path of python/cfdSolver/torchSession.py:19-694:
 the ``Simulation`` constructor and its validation (:24-158, 190-271), the
 ``state`` layout (dim+2, ...reversed axes) float64 with ``lam``/``beta``,
 ``time``/``steps`` and the balance integrals (:285-331), ``step`` (:374-522),
-``run`` (:578-633), ``preset`` (:653-694) and the worker backend
-(:160-183). The state lives on the device between steps; ``step`` runs the
+``run`` with checkpoints (:578-633), ``save``/``load`` (:635-651, with the
+schema-2 checkpoint layout of torch_amd.torchIO), ``preset`` (:653-694) and
+the worker backend (:160-183). The state lives on the device between steps; ``step`` runs the
 whole Strang-split step in libtorchhip (one host synchronisation per step,
 two when reactive) and commits only on success, so a failed step leaves the
 session unchanged.
@@ -202,7 +203,8 @@ class Simulation:
         if self._device is not None:
             self._device.close()
             self._device = None
-        self._execution.close()
+        if getattr(self, "_execution", None) is not None:
+            self._execution.close()
 
     @property
     def gamma(self):
@@ -268,20 +270,59 @@ class Simulation:
 
     def run(self, *, step_limit=None, checkpoint=None, checkpoint_every=0, output_prefix=None, output_every=1,
             formats=("vtk", "xdmf")):
-        """Run to the end time, or pause after ``step_limit`` more steps.
+        """Run to the end time, or pause after ``step_limit`` more steps (torchSession.py:578-633).
 
-        Checkpoint and VTK/XDMF output belong to the reference's HDF5 IO
-        layer, which this backend does not include.
+        A checkpoint is written every ``checkpoint_every`` steps and at the
+        end; checkpoints never shorten a step, so a restart takes the same
+        step sequence. VTK/XDMF output is not part of the device backend.
         """
         for value in (step_limit, checkpoint_every, output_every):
             if value is not None and (not isinstance(value, int) or value < 0):
                 raise ValueError("Step intervals must be nonnegative integers")
-        if checkpoint or output_prefix:
-            raise NotImplementedError("checkpoint/output writing is not part of the device backend")
+        if checkpoint_every and not checkpoint:
+            raise ValueError("checkpoint_every requires a checkpoint destination")
+        if output_prefix:
+            raise NotImplementedError("VTK/XDMF output is not part of the device backend")
+        from .torchIO import write_checkpoint
+
         count = 0
         while not self.completed and (step_limit is None or count < step_limit):
             self.step()
             count += 1
+            if checkpoint and checkpoint_every and self.steps % checkpoint_every == 0:
+                write_checkpoint(checkpoint, self)
+        if checkpoint:
+            write_checkpoint(checkpoint, self)
+        return self
+
+    def save(self, path):
+        """Write a checkpoint of the committed state (torchSession.py:635-638)."""
+        from .torchIO import write_checkpoint
+
+        write_checkpoint(path, self)
+
+    @classmethod
+    def load(cls, path, *, t_end=None, precision=None, device=0, workers=1):
+        """Resume a checkpoint, optionally with a later end time (torchSession.py:640-651)."""
+        from .torchIO import read_checkpoint, record, load_content
+
+        if t_end is None:
+            return read_checkpoint(path, precision=precision, device=device, workers=workers)
+        sim = record(load_content(path))
+        if not np.isfinite(t_end) or t_end < sim.time:
+            raise ValueError("New end time must be finite and not precede checkpoint")
+        sim.config["t_end"] = t_end
+        if precision is not None:
+            sim.precision = str(precision).lower()
+        sim._validate_config()
+        return sim._attach(workers, device)
+
+    def _attach(self, workers, device):
+        """Give a host-side session shell (a validated checkpoint) its device state."""
+        state, lam, beta = self._host
+        self._execution = DeviceExecution(validate_workers(workers), device=device, precision=self.precision)
+        self._device = None
+        self._upload(state, lam, beta)
         return self
 
     @classmethod
