@@ -1,16 +1,19 @@
 """Step-boundary session driver (oracle restatement).
 
-Restates the Cartesian path of python/cfdSolver/torchSession.py:19-522
-(``Simulation.__init__``, ``_dt``, ``_reaction`` and ``step``): Strang split
-R(dt/2) -> H(dt) -> R(dt/2) with up to 60 dt halvings while the heated state
-violates the CFL bound, outward boundary integrals, source integrals and a
-commit-at-end step. ``checked=False`` skips the reference's 0 < CFL <= 0.5
+Restates python/cfdSolver/torchSession.py:19-576 (``Simulation.__init__``,
+``_dt``, ``_reaction``, ``step`` and ``_metric_step``). Cartesian sessions:
+Strang split R(dt/2) -> H(dt) -> R(dt/2) with up to 60 dt halvings while the
+heated state violates the CFL bound, outward boundary integrals, source
+integrals and a commit-at-end step. Sessions with a curvilinear geometry or
+explicit face boundaries take the metric step instead (oracle.metric: SSPRK2
+MUSCL-Rusanov, up to 20 dt halvings). ``checked=False`` skips the reference's 0 < CFL <= 0.5
 validation (torchSolver.py:1074-1088) for workloads run outside it;
 ``constants`` passes kinetics constants the reference's 3D wrapper fixes.
 """
 
 import numpy as np
 
+from . import metric
 from . import solver1d as one
 from . import solver2d as two
 from . import solver3d as three
@@ -21,7 +24,8 @@ class Session:
                  limiter="VanLeer", reactive=False, kinetic_model="arrhenius",
                  lambda_init=None, beta_init=None, inlet_primitive=None,
                  controller="Sound Speed Controlled", variant="Variante 1 - Original",
-                 wavespeed="Pressure Based", max_steps=100000, checked=True, constants=None):
+                 wavespeed="Pressure Based", max_steps=100000, checked=True, constants=None,
+                 geometry="cartesian", origins=None, face_boundaries=None):
         primitive = np.array(primitive, dtype=float, copy=True)
         dim = primitive.ndim - 1
         if dim not in (1, 2, 3) or primitive.shape[0] != dim + 2:
@@ -37,7 +41,14 @@ class Session:
             kinetic_model=kinetic_model,
             inlet_primitive=list(two.INLET_DEFAULT if inlet_primitive is None else inlet_primitive),
             controller=controller, variant=variant, wavespeed=wavespeed, max_steps=max_steps,
+            geometry=geometry, origins_xyz=list(np.atleast_1d(origins)) if origins is not None else [0.0] * dim,
         )
+        self.faces = None
+        if geometry != "cartesian" or face_boundaries is not None:
+            metric.validate_geometry(self.config)
+            self.faces = metric.default_faces(self.config, face_boundaries)
+            self.config["face_boundaries"] = self.faces
+        self.metrics = metric.Metrics(self.config) if self.faces is not None else None
         self.constants = dict(constants or {})
         for length, n in zip(lengths, counts):
             if checked:
@@ -78,7 +89,11 @@ class Session:
 
     @property
     def integral(self):
-        return self.state.reshape(self.config["dimension"] + 2, -1).sum(axis=1) * np.prod(self.spacing)
+        """torchSession.py:285-295 (volume-weighted on curvilinear grids)."""
+        nf = self.config["dimension"] + 2
+        if self.config["geometry"] != "cartesian":
+            return (self.state * self.metrics.volumes[None]).reshape(nf, -1).sum(axis=1)
+        return self.state.reshape(nf, -1).sum(axis=1) * np.prod(self.spacing)
 
     @property
     def balance_residual(self):
@@ -132,10 +147,53 @@ class Session:
             return two.advect(q, old, state, fluxes[0][0], fluxes[1][0], dt / h[0], dt / h[1], bc[0], bc[1])
         return three.advect(q, old, state, fluxes, dt, tuple(h), tuple(bc[::-1]))
 
+    def metric_step(self):
+        """torchSession.py:523-576."""
+        c = self.config
+        if self.steps >= c["max_steps"]:
+            raise RuntimeError("Maximum steps reached before requested final time")
+        start = self.state
+        dt = min(metric.cfl_dt(start, self.lam, c, self.metrics, self.faces), c["t_end"] - self.time)
+        for _ in range(20):
+            if not np.isfinite(dt) or dt <= 0 or self.time + dt == self.time:
+                raise RuntimeError("Integration stalled before requested final time")
+            old, lam, beta = start, self.lam, self.beta
+            if c["reactive"]:
+                old, lam, beta = self.reaction(start, self.lam, self.beta, dt / 2)
+                allowed = metric.cfl_dt(old, lam, c, self.metrics, self.faces)
+                if dt > allowed * (1 + 1e-12):
+                    dt = min(dt / 2, allowed)
+                    continue
+            try:
+                hydro, lam, beta, boundary, source, first = metric.advance(old, lam, beta, dt, c, self.metrics,
+                                                                           self.faces)
+            except RuntimeError:
+                dt /= 2
+                continue
+            state = hydro
+            if c["reactive"]:
+                state, lam, beta = self.reaction(hydro, lam, beta, dt / 2)
+                nf = c["dimension"] + 2
+                source = source + ((old - start + state - hydro) * self.metrics.volumes[None]).reshape(nf, -1).sum(
+                    axis=1)
+            if not one.physical_state(state):
+                raise RuntimeError("Reaction produced a nonphysical state")
+            self.state, self.lam, self.beta = state, lam, beta
+            self.boundary_integral += boundary
+            self.source_integral += source
+            self.time = min(self.time + dt, c["t_end"])
+            self.steps += 1
+            self.fallbacks += int(first)
+            self.last_dt = dt
+            return
+        raise RuntimeError("Could not find a positive metric/face step")
+
     def step(self):
         """torchSession.py:374-522."""
         if self.completed:
             return
+        if self.faces is not None:
+            return self.metric_step()
         c = self.config
         dim = c["dimension"]
         h = self.spacing
