@@ -133,6 +133,57 @@ int torch_hllc_faces(torch_ctx* ctx, int precision, int nv, int direction, long 
                      const double* right, const double* gl, const double* gr, double* out);
 
 /* ---------------------------------------------------------------------------
+ * Metric/face scheme: the reference's second finite-volume family, used by
+ * sessions with a curvilinear geometry or explicit face boundaries
+ * (Simulation._metric_step, torchSession.py:523-576; torchFiniteVolume.py:16-276):
+ * primitive MUSCL with Rusanov faces on logically uniform Cartesian,
+ * cylindrical, annular or spherical grids, per-face boundaries, SSPRK2 with a
+ * first-order retry, area-weighted divergence and geometric sources.
+ * ------------------------------------------------------------------------- */
+enum torch_geometry { TORCH_CARTESIAN = 0, TORCH_CYLINDRICAL = 1, TORCH_ANNULAR = 2, TORCH_SPHERICAL = 3 };
+enum torch_face_type {
+    TORCH_FACE_OUTFLOW = 0, TORCH_FACE_WALL = 1, TORCH_FACE_PERIODIC = 2, TORCH_FACE_INFLOW = 3,
+    TORCH_FACE_PRESSURE_OUTLET = 4, TORCH_FACE_STAGNATION_INLET = 5
+};
+/* One face boundary, normalized like torchBoundaries.normalize_faces
+ * (torchBoundaries.py:17-174): type, and the parameters of its type. */
+typedef struct torch_face {
+    int type;                 /* enum torch_face_type */
+    int has_backflow;         /* pressure outlet: a backflow state is given */
+    double primitive[5];      /* inflow primitive, or the pressure outlet's backflow primitive (dim+2 used) */
+    double pressure;          /* pressure outlet */
+    double total_pressure, total_temperature, gas_constant;  /* stagnation inlet */
+    double lambda, beta;      /* boundary fractions (1 when not given) */
+} torch_face;
+
+/* Attaches a metric to a state (Metrics, torchGeometry.py:93-182). factors
+ * holds the per-axis factors whose outer products (in axis order x, y, z)
+ * are the cell volumes and face areas: the volume factor of each axis (n_a
+ * values), then for each direction d the area factors of each axis a (n_a
+ * values, n_d + 1 when a == d); faces holds x_min, x_max, y_min, ... */
+int torch_state_set_metric(torch_state* state, int geometry, const double* factors, long nfactors,
+                           const torch_face* faces);
+/* cfl_dt of the committed state (torchFiniteVolume.py:66-98). */
+int torch_state_metric_dt(torch_state* state, double* dt);
+/* rhs of the committed state (torchFiniteVolume.py:125-229): change
+ * (dim+2, cells), species change (one row per fraction), outward flux and
+ * integrated geometric source (dim+2 each); NULL skips an output. */
+int torch_state_metric_rhs(torch_state* state, int first_order, double* change, double* species, double* outward,
+                           double* source);
+/* advance (torchFiniteVolume.py:232-276) of the committed state by dt,
+ * committing the result: outward = boundary, source = geometric source,
+ * fallback = the first-order retry was taken. */
+int torch_state_metric_advance(torch_state* state, double dt, torch_step_info* info);
+/* Simulation._metric_step (torchSession.py:523-576): cfl_dt clipped to
+ * t_end - time, reactive R(dt/2) with the CFL test, advance, R(dt/2), up to
+ * 20 halvings; commits only on success. */
+int torch_state_metric_step(torch_state* state, double time, double t_end, torch_step_info* info);
+/* rusanov_flux_grid on host face arrays (torchFlux.py:80-122): left/right
+ * are (nv, m) primitive face states, gl/gr m gammas, out (nv, m). */
+int torch_rusanov_faces(torch_ctx* ctx, int precision, int nv, int direction, long m, const double* left,
+                        const double* right, const double* gl, const double* gr, double* out);
+
+/* ---------------------------------------------------------------------------
  * Batches. No counterpart in the reference, where every Simulation steps on
  * its own (torchSession.py:374-522) and a batch of problems is a Python loop
  * over sessions. Here many instances (inert of any dimension, reactive 3D;

@@ -32,7 +32,12 @@ EXPORTS = (
     "torch_batch_footprint", "torch_batch_create", "torch_batch_destroy", "torch_batch_add",
     "torch_batch_remove", "torch_batch_seed", "torch_batch_run", "torch_batch_info", "torch_batch_download",
     "torch_batch_tune", "torch_batch_set_tiles", "torch_batch_profile", "torch_batch_time_pass",
+    "torch_state_set_metric", "torch_state_metric_dt", "torch_state_metric_rhs", "torch_state_metric_advance",
+    "torch_state_metric_step", "torch_rusanov_faces",
 )
+
+GEOMETRY_CODES = {"cartesian": 0, "cylindrical": 1, "annular": 2, "spherical": 3}
+FACE_CODES = {"outflow": 0, "wall": 1, "periodic": 2, "inflow": 3, "pressure_outlet": 4, "stagnation_inlet": 5}
 
 
 class Config(ctypes.Structure):
@@ -55,6 +60,35 @@ class StepInfo(ctypes.Structure):
         ("outward", ctypes.c_double * 5), ("source", ctypes.c_double * 5),
         ("wave_max", ctypes.c_double * 3),
     ]
+
+
+class Face(ctypes.Structure):
+    _fields_ = [
+        ("type", ctypes.c_int), ("has_backflow", ctypes.c_int), ("primitive", ctypes.c_double * 5),
+        ("pressure", ctypes.c_double), ("total_pressure", ctypes.c_double),
+        ("total_temperature", ctypes.c_double), ("gas_constant", ctypes.c_double),
+        ("lam", ctypes.c_double), ("beta", ctypes.c_double),
+    ]
+
+
+def faces_of(faces):
+    """Normalized face dicts (torchBoundaries.normalize_faces) -> Face array."""
+    out = (Face * 6)()
+    for i, f in enumerate(faces):
+        o = out[i]
+        o.type = FACE_CODES[f["type"]]
+        state = f.get("primitive", f.get("backflow"))
+        o.has_backflow = int("backflow" in f)
+        if state is not None:
+            for k, v in enumerate(state):
+                o.primitive[k] = float(v)
+        o.pressure = float(f.get("pressure", 0.0))
+        o.total_pressure = float(f.get("total_pressure", 0.0))
+        o.total_temperature = float(f.get("total_temperature", 0.0))
+        o.gas_constant = float(f.get("gas_constant", 0.0))
+        o.lam = float(f.get("lambda", 1.0))
+        o.beta = float(f.get("beta", 1.0))
+    return out
 
 
 class BatchProgress(ctypes.Structure):
@@ -121,6 +155,15 @@ def load():
                 "torch_batch_set_tiles": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
                 "torch_batch_profile": (ctypes.c_int, [vp, ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_long)]),
                 "torch_batch_time_pass": (ctypes.c_int, [vp, ctypes.c_int, _dp]),
+                "torch_state_set_metric": (ctypes.c_int, [vp, ctypes.c_int, _dp, ctypes.c_long,
+                                                          ctypes.POINTER(Face)]),
+                "torch_state_metric_dt": (ctypes.c_int, [vp, _dp]),
+                "torch_state_metric_rhs": (ctypes.c_int, [vp, ctypes.c_int, _dp, _dp, _dp, _dp]),
+                "torch_state_metric_advance": (ctypes.c_int, [vp, ctypes.c_double, ctypes.POINTER(StepInfo)]),
+                "torch_state_metric_step": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_double,
+                                                           ctypes.POINTER(StepInfo)]),
+                "torch_rusanov_faces": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                       ctypes.c_long, _dp, _dp, _dp, _dp, _dp]),
             }
             for name, (res, args) in sig.items():
                 fn = getattr(lib, name)
@@ -258,6 +301,35 @@ class DeviceState:
         check(load().torch_state_profile(self.handle, int(enable), ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
 
+    # -- metric/face scheme ------------------------------------------------
+    def set_metric(self, geometry, factors, faces):
+        factors = np.ascontiguousarray(factors, dtype=np.float64)
+        check(load().torch_state_set_metric(self.handle, GEOMETRY_CODES[geometry], ptr(factors), factors.size,
+                                            faces_of(faces)))
+
+    def metric_dt(self):
+        out = ctypes.c_double()
+        check(load().torch_state_metric_dt(self.handle, ctypes.byref(out)))
+        return out.value
+
+    def metric_rhs(self, first_order=False, nq=0):
+        change = np.empty((self.nf,) + self.shape)
+        species = np.empty((max(nq, 1),) + self.shape)
+        outward, source = np.empty(self.nf), np.empty(self.nf)
+        check(load().torch_state_metric_rhs(self.handle, int(bool(first_order)), ptr(change),
+                                            ptr(species) if nq else None, ptr(outward), ptr(source)))
+        return change, [species[j] for j in range(nq)], outward, source
+
+    def metric_advance(self, dt):
+        info = StepInfo()
+        check(load().torch_state_metric_advance(self.handle, float(dt), ctypes.byref(info)))
+        return info
+
+    def metric_step(self, time, t_end):
+        info = StepInfo()
+        check(load().torch_state_metric_step(self.handle, float(time), float(t_end), ctypes.byref(info)))
+        return info
+
     def close(self):
         if getattr(self, "handle", None):
             load().torch_state_destroy(self.handle)
@@ -279,6 +351,18 @@ def hllc_faces(context, nv, direction, left, right, gl, gr, precision=0):
     out = np.empty_like(left)
     check(load().torch_hllc_faces(context.handle, int(precision), int(nv), int(direction), m, ptr(left),
                                   ptr(right), ptr(gl), ptr(gr), ptr(out)))
+    return out
+
+
+def rusanov_faces(context, nv, direction, left, right, gl, gr, precision=0):
+    left = np.ascontiguousarray(left, dtype=np.float64)
+    right = np.ascontiguousarray(right, dtype=np.float64)
+    m = left.shape[1]
+    gl = np.ascontiguousarray(np.broadcast_to(gl, (m,)), dtype=np.float64)
+    gr = np.ascontiguousarray(np.broadcast_to(gr, (m,)), dtype=np.float64)
+    out = np.empty_like(left)
+    check(load().torch_rusanov_faces(context.handle, int(precision), int(nv), int(direction), m, ptr(left),
+                                     ptr(right), ptr(gl), ptr(gr), ptr(out)))
     return out
 
 

@@ -22,7 +22,7 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libtorchhip.so")
 ARCH = "gfx950"
 
-KERNELS = ("fv3d.hip", "fv2d.hip", "fv1d.hip", "fvbatch.hip")
+KERNELS = ("fv3d.hip", "fv2d.hip", "fv1d.hip", "fvbatch.hip", "fvmetric.hip")
 HOST = ("capi.hip", "batch.hip")
 COMMON = [
     "--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",

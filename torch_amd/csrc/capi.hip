@@ -111,7 +111,8 @@ int download_as(const Ops* ops, hipStream_t st, double* dst, const void* src, lo
         NS::convert_in,   NS::convert_out,                                                               \
         NS::batch3d,      NS::batch_llf3d,   NS::batch_cfl3d, NS::batch2d,      NS::batch_llf2d,          \
         NS::batch_cfl2d,  NS::batch1d,       NS::batch_llf1d, NS::batch_cfl1d,  NS::batch_post,           \
-        NS::batch_react,  NS::batch_check,   NS::strip2d_rows, NS::step1d_tiles}
+        NS::batch_react,  NS::batch_check,   NS::strip2d_rows, NS::step1d_tiles,                       \
+        NS::metric_stage, NS::metric_cfl,    NS::metric_source, NS::rusanov_faces, NS::metric_blocks}
 
 // Not const: a const table would be promoted to device memory, where the
 // host launchers it points to do not exist.
@@ -197,6 +198,10 @@ struct torch_state {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double prof_ms = 0;
     long prof_n = 0;
+    // metric/face scheme (torch_state_set_metric)
+    bool has_metric = false;
+    fvl::Metric metric;
+    double* mfactors = nullptr;
 };
 
 namespace {
@@ -396,6 +401,104 @@ int run_hydro(torch_state* s, int in, int out, double dt, int lam_in, int lam_ou
     return TORCH_OK;
 }
 
+// -- metric/face scheme -------------------------------------------------------
+
+const int kMetricSlots = 4;
+
+const void* frac(torch_state* s, int slot, int j) {
+    if (!s->cfg.reactive) return nullptr;
+    return j == 0 ? s->lam[slot] : (s->cfg.kinetic_model == 1 ? s->beta[slot] : nullptr);
+}
+
+int metric_nq(const torch_state* s) { return s->cfg.reactive ? (s->cfg.kinetic_model == 1 ? 2 : 1) : 0; }
+
+// Errors the reference raises (ValueError) while building boundary planes.
+int boundary_error(int flags) {
+    if (flags & fvl::F_BC_RESERVOIR) return fail(TORCH_INVALID, "No subsonic reservoir-inlet solution");
+    if (flags & fvl::F_BC_SUPPLY)
+        return fail(TORCH_INVALID, "Reservoir cannot supply inflow against the interior pressure");
+    if (flags & fvl::F_BC_BACKFLOW)
+        return fail(TORCH_INVALID, "Pressure outlet backflow requires an explicit backflow primitive state");
+    return TORCH_OK;
+}
+
+// cfl_dt of slot u (torchFiniteVolume.py:66-98).
+int metric_dt(torch_state* s, int u, double* dt) {
+    hipStream_t st = s->ctx->stream;
+    HIP_TRY(clear_flags(s->flags, st));
+    HIP_TRY(s->ops->metric_cfl(s->grid, s->metric, s->U[u], frac(s, u, 0), s->cfg.gamma, s->cfg.reactive,
+                               region(s, 3), s->flags, st));
+    HIP_TRY(fvl::finalize(region(s, 3), s->ops->metric_blocks(s->grid), s->scalars + 3 * fvl::NSLOT, st));
+    int rc = sync_read(s);
+    if (rc) return rc;
+    if ((rc = boundary_error(host_flags(s)))) return rc;
+    *dt = s->cfg.cfl / s->host[3 * fvl::NSLOT];
+    return TORCH_OK;
+}
+
+// One SSPRK2 stage (mode 0: raw change) into slot `out`; returns the flags
+// and leaves the outward / source sums in scalars[0].
+int metric_stage(torch_state* s, int mode, int S, int U0, int out, double dt, int first_order, int* flags) {
+    fvl::MStage m;
+    memset(&m, 0, sizeof(m));
+    m.S = s->U[S];
+    m.U0 = s->U[U0];
+    m.out = s->U[out];
+    for (int j = 0; j < 2; ++j) {
+        m.qS[j] = frac(s, S, j);
+        m.q0[j] = frac(s, U0, j);
+        m.qout[j] = const_cast<void*>(frac(s, out, j));
+    }
+    m.dt = dt;
+    m.gamma = s->cfg.gamma;
+    m.mode = mode;
+    m.first_order = first_order;
+    m.limiter = s->cfg.limiter;
+    m.nq = metric_nq(s);
+    m.reactive = s->cfg.reactive;
+    m.flags = s->flags;
+    m.partials = region(s, 0);
+    hipStream_t st = s->ctx->stream;
+    HIP_TRY(clear_flags(s->flags, st));
+    HIP_TRY(s->ops->metric_stage(s->grid, s->metric, m, st));
+    HIP_TRY(fvl::finalize(region(s, 0), s->ops->metric_blocks(s->grid), s->scalars, st));
+    int rc = sync_read(s);
+    if (rc) return rc;
+    *flags = host_flags(s);
+    return boundary_error(*flags);
+}
+
+// advance (torchFiniteVolume.py:232-276): slot `in` -> `out` through the
+// stage slot. Returns TORCH_NONPHYSICAL when both orders lose positivity.
+int metric_advance(torch_state* s, int in, int stage, int out, double dt, double* boundary, double* source,
+                   int* first_used) {
+    const int nf = s->nf;
+    for (int first = 0; first < 2; ++first) {
+        int flags = 0, rc;
+        if ((rc = metric_stage(s, 1, in, in, stage, dt, first, &flags))) return rc;
+        if (flags & (fvl::F_M_UNPHYS | fvl::F_M_SPEC)) continue;
+        double b1[5], s1[5];
+        for (int f = 0; f < nf; ++f) {
+            b1[f] = s->host[fvl::S_OUT + f];
+            s1[f] = s->host[fvl::S_MSRC + f];
+        }
+        if ((rc = metric_stage(s, 2, stage, in, out, dt, first, &flags))) return rc;
+        if (flags & (fvl::F_M_UNPHYS | fvl::F_M_SPEC)) continue;
+        for (int f = 0; f < nf; ++f) {
+            boundary[f] = (0.5 * dt) * (b1[f] + s->host[fvl::S_OUT + f]);
+            source[f] = (0.5 * dt) * (s1[f] + s->host[fvl::S_MSRC + f]);
+        }
+        *first_used = first;
+        return TORCH_OK;
+    }
+    return fail(TORCH_NONPHYSICAL, "Metric/face update lost positivity; reduce CFL");
+}
+
+int need_metric(const torch_state* s) {
+    if (!s->has_metric) return fail(TORCH_INVALID, "state has no metric (torch_state_set_metric)");
+    return TORCH_OK;
+}
+
 const char* hydro_message(int dim) {
     return dim == 3 ? "3D hydrodynamic update produced a nonphysical state"
                     : "Hydrodynamic update produced a nonphysical state";
@@ -447,6 +550,7 @@ int torch_state_destroy(torch_state* s) {
         (void)hipFree(s->beta[i]);
     }
     for (int d = 0; d < 3; ++d) (void)hipFree(s->F[d]);
+    (void)hipFree(s->mfactors);
     (void)hipFree(s->scratch1d);
     (void)hipFree(s->partials);
     if (s->ev0) (void)hipEventDestroy(s->ev0);
@@ -498,6 +602,7 @@ int torch_state_create(torch_ctx* ctx, const torch_config* cfg, torch_state** ou
     int pb = std::max(s->ops->cfl_blocks(g), s->ops->react_blocks(g));
     if (c.dim == 3) pb = std::max(pb, std::max(s->ops->step3d_blocks(g, l, s->cfg.zchunk), s->ops->llf3d_blocks(g)));
     if (c.dim == 2) pb = std::max(pb, std::max(s->ops->step2d_blocks(g, l), s->ops->llf2d_blocks(g)));
+    pb = std::max(pb, s->ops->metric_blocks(g));
     if (c.dim == 1) {
         pb = std::max(pb, s->ops->step1d_blocks(g));
         if (!rc) rc = alloc(&s->scratch1d, s->ops->step1d_scratch(g));
@@ -718,11 +823,193 @@ int torch_state_profile(torch_state* s, int enable, double* kernel_ms, long* lau
     return TORCH_OK;
 }
 
-int torch_hllc_faces(torch_ctx* ctx, int precision, int nv, int direction, long m, const double* left,
-                     const double* right, const double* gl, const double* gr, double* out) {
-    if (!ctx || !left || !right || !gl || !gr || !out) return fail(TORCH_INVALID, "null argument");
-    if ((nv != 4 && nv != 5) || direction < 0 || direction >= nv - 2 || m < 0)
-        return fail(TORCH_INVALID, "HLLC requires matching four/five-component grids");
+int torch_state_set_metric(torch_state* s, int geometry, const double* factors, long nfactors,
+                           const torch_face* faces) {
+    if (!s || !factors || !faces) return fail(TORCH_INVALID, "null argument");
+    if (geometry < TORCH_CARTESIAN || geometry > TORCH_SPHERICAL) return fail(TORCH_INVALID, "Unknown geometry");
+    const int dim = s->cfg.dim;
+    const int n[3] = {s->grid.nx, s->grid.ny, s->grid.nz};
+    long need = 0;
+    for (int a = 0; a < dim; ++a) need += n[a];
+    for (int d = 0; d < dim; ++d)
+        for (int a = 0; a < dim; ++a) need += n[a] + (a == d);
+    if (nfactors != need) return fail(TORCH_INVALID, "metric factor table has the wrong length");
+    for (long i = 0; i < need; ++i)
+        if (!isfinite(factors[i])) return fail(TORCH_INVALID, "metric factors must be finite");
+    for (int f = 0; f < 2 * dim; ++f) {
+        const int t = faces[f].type;
+        if (t < TORCH_FACE_OUTFLOW || t > TORCH_FACE_STAGNATION_INLET)
+            return fail(TORCH_INVALID, "Unknown face boundary type");
+        if ((t == TORCH_FACE_PERIODIC) != (faces[f ^ 1].type == TORCH_FACE_PERIODIC))
+            return fail(TORCH_INVALID, "Periodic boundaries must be paired on an axis");
+    }
+    std::lock_guard<std::mutex> lock(s->ctx->mu);
+    HIP_TRY(hipSetDevice(s->ctx->device));
+    // Four state slots: start, chemistry, stage, hydro.
+    const size_t e = s->ops->elem;
+    for (int i = 0; i < kMetricSlots; ++i)
+        if (!s->U[i]) {
+            int rc = alloc(&s->U[i], s->nf * s->ncell * e);
+            if (rc) return rc;
+        }
+    if (!s->mfactors) {
+        int rc = alloc((void**)&s->mfactors, need * sizeof(double));
+        if (rc) return rc;
+    }
+    HIP_TRY(hipMemcpyAsync(s->mfactors, factors, need * sizeof(double), hipMemcpyHostToDevice, s->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(s->ctx->stream));
+    fvl::Metric& m = s->metric;
+    memset(&m, 0, sizeof(m));
+    long at = 0;
+    for (int a = 0; a < dim; ++a) {
+        m.vf[a] = s->mfactors + at;
+        at += n[a];
+    }
+    for (int d = 0; d < dim; ++d)
+        for (int a = 0; a < dim; ++a) {
+            m.af[d][a] = s->mfactors + at;
+            at += n[a] + (a == d);
+        }
+    m.geometry = geometry;
+    for (int f = 0; f < 2 * dim; ++f) {
+        fvl::MFace& o = m.face[f];
+        const torch_face& i = faces[f];
+        o.type = i.type;
+        o.has_backflow = i.has_backflow;
+        for (int k = 0; k < 5; ++k) o.prim[k] = i.primitive[k];
+        o.pressure = i.pressure;
+        o.total_pressure = i.total_pressure;
+        o.total_temperature = i.total_temperature;
+        o.gas_constant = i.gas_constant;
+        o.lam = i.lambda;
+        o.beta = i.beta;
+    }
+    s->has_metric = true;
+    return TORCH_OK;
+}
+
+int torch_state_metric_dt(torch_state* s, double* dt) {
+    if (!s || !dt) return fail(TORCH_INVALID, "null argument");
+    std::lock_guard<std::mutex> lock(s->ctx->mu);
+    HIP_TRY(hipSetDevice(s->ctx->device));
+    int rc = need_metric(s);
+    if (rc) return rc;
+    return metric_dt(s, s->cur, dt);
+}
+
+int torch_state_metric_rhs(torch_state* s, int first_order, double* change, double* species, double* outward,
+                           double* source) {
+    if (!s) return fail(TORCH_INVALID, "null argument");
+    std::lock_guard<std::mutex> lock(s->ctx->mu);
+    HIP_TRY(hipSetDevice(s->ctx->device));
+    int rc = need_metric(s);
+    if (rc) return rc;
+    const int A = s->cur, B = (A + 1) % kMetricSlots;
+    int flags = 0;
+    if ((rc = metric_stage(s, 0, A, A, B, 0.0, first_order != 0, &flags))) return rc;
+    for (int f = 0; f < s->nf; ++f) {
+        if (outward) outward[f] = s->host[fvl::S_OUT + f];
+        if (source) source[f] = s->host[fvl::S_MSRC + f];
+    }
+    if (change && (rc = download_as(s->ops, s->ctx->stream, change, s->U[B], s->nf * s->ncell))) return rc;
+    for (int j = 0; j < metric_nq(s) && species; ++j)
+        if ((rc = download_as(s->ops, s->ctx->stream, species + j * s->ncell, frac(s, B, j), s->ncell))) return rc;
+    return TORCH_OK;
+}
+
+int torch_state_metric_advance(torch_state* s, double dt, torch_step_info* info) {
+    if (!s || !info) return fail(TORCH_INVALID, "null argument");
+    std::lock_guard<std::mutex> lock(s->ctx->mu);
+    HIP_TRY(hipSetDevice(s->ctx->device));
+    memset(info, 0, sizeof(*info));
+    int rc = need_metric(s);
+    if (rc) return rc;
+    const int A = s->cur, B = (A + 1) % kMetricSlots, C = (A + 2) % kMetricSlots;
+    if ((rc = metric_advance(s, A, B, C, dt, info->outward, info->source, &info->fallback))) return rc;
+    info->dt = dt;
+    s->cur = C;
+    s->maxima_valid = false;
+    return TORCH_OK;
+}
+
+int torch_state_metric_step(torch_state* s, double time, double t_end, torch_step_info* info) {
+    if (!s || !info) return fail(TORCH_INVALID, "null argument");
+    std::lock_guard<std::mutex> lock(s->ctx->mu);
+    HIP_TRY(hipSetDevice(s->ctx->device));
+    memset(info, 0, sizeof(*info));
+    int rc = need_metric(s);
+    if (rc) return rc;
+    const bool reactive = s->cfg.reactive;
+    const int A = s->cur;
+    const int B = (A + 1) % kMetricSlots, C = (A + 2) % kMetricSlots, D = (A + 3) % kMetricSlots;
+    double dt;
+    if ((rc = metric_dt(s, A, &dt))) return rc;
+    dt = py_min(dt, t_end - time);
+    for (int retry = 0; retry < 20; ++retry) {
+        if (!isfinite(dt) || dt <= 0 || time + dt == time)
+            return fail(TORCH_STALLED, "Integration stalled before requested final time");
+        int old = A;
+        if (reactive) {
+            HIP_TRY(clear_flags(s->flags, s->ctx->stream));
+            if ((rc = run_react(s, A, B, dt / 2, -1, -1, false))) return rc;
+            if ((rc = sync_read(s))) return rc;
+            if (host_flags(s) & fvl::F_REACT_BAD)
+                return fail(TORCH_INVALID, "Reaction input outside its domain (state, lambda or beta)");
+            double allowed;
+            if ((rc = metric_dt(s, B, &allowed))) return rc;
+            if (dt > allowed * (1 + 1e-12)) {
+                dt = py_min(dt / 2, allowed);
+                info->retries += 1;
+                continue;
+            }
+            old = B;
+        }
+        // inert: stage B, hydro C; reactive: stage C, hydro D, final C
+        const int stage = reactive ? C : B, hydro = reactive ? D : C;
+        double boundary[5] = {0, 0, 0, 0, 0}, source[5] = {0, 0, 0, 0, 0};
+        int first = 0;
+        rc = metric_advance(s, old, stage, hydro, dt, boundary, source, &first);
+        if (rc == TORCH_NONPHYSICAL) {
+            dt /= 2;
+            info->retries += 1;
+            continue;
+        }
+        if (rc) return rc;
+        int final_slot = hydro;
+        if (reactive) {
+            HIP_TRY(clear_flags(s->flags, s->ctx->stream));
+            if ((rc = run_react(s, hydro, C, dt / 2, -1, -1, false))) return rc;
+            HIP_TRY(s->ops->metric_source(s->grid, s->metric, s->U[A], s->U[B], s->U[C], s->U[hydro], region(s, 1),
+                                          s->ctx->stream));
+            HIP_TRY(fvl::finalize(region(s, 1), s->ops->metric_blocks(s->grid), s->scalars + fvl::NSLOT,
+                                  s->ctx->stream));
+            if ((rc = sync_read(s))) return rc;
+            const int flags = host_flags(s);
+            if (flags & fvl::F_REACT_BAD) return fail(TORCH_INVALID, "Reaction input outside its domain");
+            if (flags & fvl::F_STATE_BAD) return fail(TORCH_REACTION_STATE, "Reaction produced a nonphysical state");
+            const int e = s->nf - 1;
+            source[e] = source[e] + s->host[fvl::NSLOT + fvl::S_MSRC + e];
+            final_slot = C;
+        }
+        info->dt = dt;
+        info->fallback = first;
+        for (int f = 0; f < s->nf; ++f) {
+            info->outward[f] = boundary[f];
+            info->source[f] = source[f];
+        }
+        s->cur = final_slot;
+        s->maxima_valid = false;
+        return TORCH_OK;
+    }
+    return fail(TORCH_NONPHYSICAL, "Could not find a positive metric/face step");
+}
+
+
+// A face-array kernel (HLLC or Rusanov) on host arrays: upload, launch, download.
+static int faces_call(torch_ctx* ctx, int precision, int nv, int direction, long m, const double* left,
+                      const double* right, const double* gl, const double* gr, double* out,
+                      hipError_t (*const Ops::*kernel)(int, int, long, const void*, const void*, const void*,
+                                                       const void*, void*, hipStream_t)) {
     if (m == 0) return TORCH_OK;
     const Ops* ops = precision == TORCH_FP32 ? &kOps32 : &kOps64;
     std::lock_guard<std::mutex> lock(ctx->mu);
@@ -734,43 +1021,37 @@ int torch_hllc_faces(torch_ctx* ctx, int precision, int nv, int direction, long 
     if (!rc) rc = alloc(&dgl, m * e);
     if (!rc) rc = alloc(&dgr, m * e);
     if (!rc) rc = alloc(&dout, nv * m * e);
-    auto put = [&](void* dst, const double* src, long n) -> int {
-        if (e == sizeof(double)) {
-            HIP_TRY(hipMemcpyAsync(dst, src, n * e, hipMemcpyHostToDevice, ctx->stream));
-        } else {
-            std::vector<float> t(src, src + n);
-            HIP_TRY(hipMemcpyAsync(dst, t.data(), n * e, hipMemcpyHostToDevice, ctx->stream));
-            HIP_TRY(hipStreamSynchronize(ctx->stream));
-        }
-        return TORCH_OK;
-    };
-    if (!rc) rc = put(dl, left, nv * m);
-    if (!rc) rc = put(dr, right, nv * m);
-    if (!rc) rc = put(dgl, gl, m);
-    if (!rc) rc = put(dgr, gr, m);
+    if (!rc) rc = upload_as(ops, ctx->stream, dl, left, nv * m);
+    if (!rc) rc = upload_as(ops, ctx->stream, dr, right, nv * m);
+    if (!rc) rc = upload_as(ops, ctx->stream, dgl, gl, m);
+    if (!rc) rc = upload_as(ops, ctx->stream, dgr, gr, m);
     if (!rc) {
-        hipError_t he = ops->hllc_faces(nv, direction, m, dl, dr, dgl, dgr, dout, ctx->stream);
+        hipError_t he = (ops->*kernel)(nv, direction, m, dl, dr, dgl, dgr, dout, ctx->stream);
         if (he != hipSuccess) rc = fail(TORCH_HIP_ERROR, hipGetErrorString(he));
     }
-    if (!rc) {
-        if (e == sizeof(double)) {
-            hipError_t he = hipMemcpyAsync(out, dout, nv * m * e, hipMemcpyDeviceToHost, ctx->stream);
-            if (he == hipSuccess) he = hipStreamSynchronize(ctx->stream);
-            if (he != hipSuccess) rc = fail(TORCH_HIP_ERROR, hipGetErrorString(he));
-        } else {
-            std::vector<float> t(nv * m);
-            hipError_t he = hipMemcpyAsync(t.data(), dout, nv * m * e, hipMemcpyDeviceToHost, ctx->stream);
-            if (he == hipSuccess) he = hipStreamSynchronize(ctx->stream);
-            if (he != hipSuccess) rc = fail(TORCH_HIP_ERROR, hipGetErrorString(he));
-            for (long i = 0; i < nv * m && !rc; ++i) out[i] = t[i];
-        }
-    }
+    if (!rc) rc = download_as(ops, ctx->stream, out, dout, nv * m);
     (void)hipFree(dl);
     (void)hipFree(dr);
     (void)hipFree(dgl);
     (void)hipFree(dgr);
     (void)hipFree(dout);
     return rc;
+}
+
+int torch_hllc_faces(torch_ctx* ctx, int precision, int nv, int direction, long m, const double* left,
+                     const double* right, const double* gl, const double* gr, double* out) {
+    if (!ctx || !left || !right || !gl || !gr || !out) return fail(TORCH_INVALID, "null argument");
+    if ((nv != 4 && nv != 5) || direction < 0 || direction >= nv - 2 || m < 0)
+        return fail(TORCH_INVALID, "HLLC requires matching four/five-component grids");
+    return faces_call(ctx, precision, nv, direction, m, left, right, gl, gr, out, &Ops::hllc_faces);
+}
+
+int torch_rusanov_faces(torch_ctx* ctx, int precision, int nv, int direction, long m, const double* left,
+                        const double* right, const double* gl, const double* gr, double* out) {
+    if (!ctx || !left || !right || !gl || !gr || !out) return fail(TORCH_INVALID, "null argument");
+    if (nv < 3 || nv > 5 || direction < 0 || direction >= nv - 2 || m < 0)
+        return fail(TORCH_INVALID, "Rusanov requires (dim+2)-component face arrays and a valid direction");
+    return faces_call(ctx, precision, nv, direction, m, left, right, gl, gr, out, &Ops::rusanov_faces);
 }
 
 }  // extern "C"

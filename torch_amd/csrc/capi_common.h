@@ -70,6 +70,15 @@ struct Ops {
     hipError_t (*batch_check)(fvl::BatchItem*, fvl::BatchCtl*, int, hipStream_t);
     int (*strip2d_rows)(const fvl::Grid&);
     int (*step1d_tiles)(const fvl::Grid&);
+    // metric/face scheme
+    hipError_t (*metric_stage)(const fvl::Grid&, const fvl::Metric&, const fvl::MStage&, hipStream_t);
+    hipError_t (*metric_cfl)(const fvl::Grid&, const fvl::Metric&, const void*, const void*, double, int, double*,
+                             int*, hipStream_t);
+    hipError_t (*metric_source)(const fvl::Grid&, const fvl::Metric&, const void*, const void*, const void*,
+                                const void*, double*, hipStream_t);
+    hipError_t (*rusanov_faces)(int, int, long, const void*, const void*, const void*, const void*, void*,
+                                hipStream_t);
+    int (*metric_blocks)(const fvl::Grid&);
 };
 
 extern Ops kOps64;

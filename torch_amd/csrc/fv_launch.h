@@ -28,7 +28,13 @@ enum Flag {
     F_SPEC_BAD = 8,      // species left [-tol, 1+tol] (second-order fluxes; tol kSpecTol)
     F_SPEC_BAD_LLF = 16, // same with first-order fluxes
     F_REACT_BAD = 32,    // chemistry input outside its domain -> ValueError
-    F_STATE_BAD = 64     // state after chemistry unphysical
+    F_STATE_BAD = 64,    // state after chemistry unphysical
+    // metric/face step (fvmetric.hip)
+    F_M_UNPHYS = 128,    // an SSPRK2 stage left physical_state
+    F_M_SPEC = 256,      // an SSPRK2 stage's fractions left [-1e-12, 1+1e-12]
+    F_BC_RESERVOIR = 512,   // stagnation inlet: no subsonic reservoir solution
+    F_BC_SUPPLY = 1024,     // stagnation inlet: reservoir cannot supply inflow
+    F_BC_BACKFLOW = 2048    // pressure outlet backflow without a backflow state
 };
 
 struct Grid {
@@ -38,6 +44,43 @@ struct Grid {
     double inlet[4];   // 2D Inflow conservative state (rho, mx, my, E)
     int wavespeed;     // 1D wave-speed estimate: 0 pressure based, 1 enthalpy based
 };
+
+// Face boundary of the metric/face scheme (torchBoundaries.py:8-15, 177-246).
+enum FaceType { FACE_OUTFLOW = 0, FACE_WALL, FACE_PERIODIC, FACE_INFLOW, FACE_PRESSURE_OUTLET, FACE_STAGNATION };
+struct MFace {
+    int type, has_backflow;
+    double prim[5];    // inflow primitive, or the backflow primitive (dim+2 used)
+    double pressure;   // pressure outlet
+    double total_pressure, total_temperature, gas_constant;  // stagnation inlet
+    double lam, beta;  // boundary fractions
+};
+// Metric of a logically uniform grid: cell volumes and face areas are outer
+// products of per-axis factors (torchGeometry.py:113-182), evaluated on the
+// fly as ((1 * f_x) * f_y) * f_z.
+enum Geometry { GEO_CARTESIAN = 0, GEO_CYLINDRICAL, GEO_ANNULAR, GEO_SPHERICAL };
+struct Metric {
+    const double* vf[3];      // volume factor per axis (n_a entries)
+    const double* af[3][3];   // area factors of direction d over axis a (n_a, or n_d + 1 when a == d)
+    MFace face[6];            // x_min, x_max, y_min, y_max, z_min, z_max
+    int geometry;
+};
+// One SSPRK2 stage of the metric scheme (torchFiniteVolume.py:232-276).
+// mode 0 writes the semi-discrete change k (and species change) itself;
+// mode 1 writes S + dt k with fractions (U0_rho q0 + dt kq) / rho_out;
+// mode 2 writes 0.5 ((U0 + S) + dt k) with (0.5 ((U0_rho q0 + S_rho qS) + dt kq)) / rho_out.
+struct MStage {
+    const void* S;            // rhs input state
+    const void* qS[2];        // its fractions (lambda, beta) or null
+    const void* U0;           // step input state
+    const void* q0[2];
+    void* out;
+    void* qout[2];
+    double dt, gamma;
+    int mode, first_order, limiter, nq, reactive;
+    int* flags;
+    double* partials;         // NSLOT per block: S_OUT.. outward, S_MSRC.. integrated source
+};
+enum { S_MSRC = 8 };          // 5 integrated geometric-source sums (metric stages)
 
 struct Step {
     const void* U;     // input state, (dim+2) fields of ncell values
@@ -269,6 +312,18 @@ struct Launch {
     hipError_t batch_check(fvl::BatchItem* items, fvl::BatchCtl* ctl, int n, hipStream_t st); \
     int strip2d_rows(const fvl::Grid& g);                                                 \
     int step1d_tiles(const fvl::Grid& g);                                                 \
+    /* metric/face scheme */                                                              \
+    hipError_t metric_stage(const fvl::Grid& g, const fvl::Metric& m, const fvl::MStage& s, \
+                            hipStream_t st);                                              \
+    hipError_t metric_cfl(const fvl::Grid& g, const fvl::Metric& m, const void* U,         \
+                          const void* lam, double gamma, int reactive, double* partials,   \
+                          int* flags, hipStream_t st);                                     \
+    hipError_t metric_source(const fvl::Grid& g, const fvl::Metric& m, const void* start,  \
+                             const void* old, const void* state, const void* hydro,        \
+                             double* partials, hipStream_t st);                            \
+    hipError_t rusanov_faces(int nv, int dir, long m, const void* l, const void* r,        \
+                             const void* gl, const void* gr, void* out, hipStream_t st);   \
+    int metric_blocks(const fvl::Grid& g);                                                \
     }
 
 FVL_DECLARE(fv64)

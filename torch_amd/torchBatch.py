@@ -48,6 +48,8 @@ class Batch:
         session has left).
         """
         c = sim.config
+        if sim.uses_metric_kernel:
+            raise ValueError("batches hold Cartesian MUSCL-Hancock/HLLC sessions; metric/face sessions step alone")
         if c["reactive"] and c["dimension"] != 3:
             raise ValueError("batches hold inert sessions and reactive 3D sessions")
         cfg = sim._make_config()

@@ -1,10 +1,11 @@
-"""Directional HLLC interface flux on the GPU.
+"""Directional HLLC and Rusanov interface fluxes on the GPU.
 
-Mirrors python/cfdSolver/torchFlux.py:6-77 (``hllc_flux_grid``) and
-:125-248 (``hllc_flux``): same signatures, validation and component order.
-The faces are evaluated by libtorchhip's HLLC kernel; ``batch_size`` is
-validated for compatibility but the whole grid is one launch (the
-reference's result does not depend on batching).
+Mirrors python/cfdSolver/torchFlux.py:6-77 (``hllc_flux_grid``),
+:80-122 (``rusanov_flux``, ``rusanov_flux_grid``) and :125-248
+(``hllc_flux``): same signatures, validation and component order.
+The faces are evaluated by libtorchhip's HLLC and Rusanov kernels;
+``batch_size`` is validated for compatibility but the whole grid is one
+launch (the reference's result does not depend on batching).
 """
 
 import numpy as np
@@ -41,3 +42,28 @@ def hllc_flux(u_l, u_r, gamma=1.4, wave_speed_mode="Pressure Based", gamma_right
     # In the normal basis the normal momentum is row 1, i.e. direction 0.
     return hllc_flux_grid(u_l, u_r, gamma, gamma if gamma_right is None else gamma_right,
                           direction=0, execution=execution)
+
+
+def rusanov_flux_grid(left, right, gl, gr, direction, execution=None):
+    """Rusanov flux of primitive face states (rho, velocities..., p) along
+    ``direction`` (torchFlux.py:103-122); the reference's 8192-face worker
+    batches are one launch here."""
+    left = np.asarray(left, dtype=float)
+    right = np.asarray(right, dtype=float)
+    if left.shape != right.shape or left.ndim < 2 or left.shape[0] not in (3, 4, 5):
+        raise ValueError("Rusanov requires matching (dim+2)-component face grids")
+    if direction not in range(left.shape[0] - 2):
+        raise ValueError("Invalid normal direction")
+    nv = left.shape[0]
+    faces = left.shape[1:]
+    gl = np.broadcast_to(gl, faces).reshape(-1)
+    gr = np.broadcast_to(gr, faces).reshape(-1)
+    ex = device_of(execution)
+    out = _abi.rusanov_faces(ex.context, nv, direction, left.reshape(nv, -1), right.reshape(nv, -1), gl, gr,
+                             _abi.precision_code(ex.precision))
+    return out.reshape(left.shape)
+
+
+def rusanov_flux(left, right, gl, gr, direction, execution=None):
+    """Independent normal Rusanov flux, including tangential momenta (torchFlux.py:80-100)."""
+    return rusanov_flux_grid(left, right, gl, gr, direction, execution)

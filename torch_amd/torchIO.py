@@ -38,8 +38,10 @@ REQUIRED = {"config", "time", "steps", "run_id", "initial_integral", "boundary_i
 
 
 def component_names(config):
-    """Cartesian component names (torchGeometry.py:12-31)."""
-    return ["rho"] + ["momentum_" + a for a in "xyz"[:config["dimension"]]] + ["energy"]
+    """Component dataset names (torchGeometry.py:12-31)."""
+    from .torchGeometry import component_names as names
+
+    return names(config)
 
 
 def _plain(value):
@@ -202,8 +204,6 @@ def record(data):
         for key in ("counts_xyz", "lengths_xyz", "boundaries_xyz", "origins_xyz"):
             if key in config:
                 config[key] = list(np.atleast_1d(config[key]))
-        if config.get("geometry", "cartesian") != "cartesian" or "face_boundaries" in config:
-            raise ValueError("The MI355X backend restores Cartesian sessions only")
         sim.precision = str(device.get("precision", "fp64")).lower()
         sim.checked = bool(device.get("checked", True))
         sim.constants = dict(device.get("constants") or {})

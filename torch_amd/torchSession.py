@@ -24,6 +24,7 @@ import uuid
 import numpy as np
 
 from . import _abi
+from . import torchGeometry as _geometry
 from .presets import preset as _preset
 from .torchExecution import DeviceExecution, validate_workers
 from .torchSolver3D import LIMITERS, config as _config
@@ -66,9 +67,6 @@ class Simulation:
         counts = list(primitive.shape[1:][::-1])
         if len(lengths) != dim:
             raise ValueError("Lengths must be in x,y,z order, one per active axis")
-        if geometry != "cartesian" or face_boundaries is not None:
-            raise ValueError("The MI355X backend runs the Cartesian MUSCL-Hancock/HLLC kernel only; "
-                             "metric geometries and face boundaries are not supported")
         boundaries = list(boundaries) if boundaries is not None else ["Open"] * dim
         self.config = dict(
             dimension=dim, counts_xyz=counts, lengths_xyz=lengths, boundaries_xyz=boundaries, t_end=t_end,
@@ -77,6 +75,8 @@ class Simulation:
             controller=controller, variant=variant, wavespeed=wavespeed, max_steps=max_steps,
             geometry=geometry, origins_xyz=list(np.atleast_1d(origins)) if origins is not None else [0.0] * dim,
         )
+        if geometry != "cartesian" or face_boundaries is not None:
+            self.config["face_boundaries"] = _geometry.session_faces(self.config, face_boundaries)
         self.precision = str(precision).lower()
         self.checked = bool(checked)
         self.constants = dict(constants or {})
@@ -110,6 +110,12 @@ class Simulation:
         c = self.config
         for length, n in zip(c["lengths_xyz"], c["counts_xyz"]):
             _validate_grid(length, n, c["cfl"], c["t_end"], self.checked)
+        _geometry.validate_geometry(c)
+        if self.uses_metric_kernel:
+            c["face_boundaries"] = _geometry.normalize_faces(c, c.get("face_boundaries"))
+            if (c["controller"], c["variant"], c["wavespeed"]) != (_abi.CONTROLLERS[0], _abi.VARIANTS[0],
+                                                                   _abi.WAVESPEEDS[0]):
+                raise ValueError("Metric/face sessions use MUSCL-Rusanov/SSPRK2; legacy HLLC switches are unsupported")
         if not np.isfinite(c["gamma"]) or c["gamma"] <= 1:
             raise ValueError("gamma must be finite and greater than one")
         if c["limiter"] not in LIMITERS or c["kinetic_model"] not in ("arrhenius", "znd"):
@@ -152,6 +158,9 @@ class Simulation:
         if self._device is not None:
             self._device.close()
         self._device = _abi.DeviceState(self._execution.context, self._make_config())
+        if self.uses_metric_kernel:
+            c = self.config
+            self._device.set_metric(c.get("geometry", "cartesian"), _geometry.factor_table(c), c["face_boundaries"])
         self._device.upload(state, lam, beta)
         self._host = (state, lam, beta)
 
@@ -215,7 +224,17 @@ class Simulation:
         return np.array(self.config["lengths_xyz"]) / self.config["counts_xyz"]
 
     def _integral_of(self, state):
-        return state.reshape(self.config["dimension"] + 2, -1).sum(axis=1) * np.prod(self.spacing)
+        nf = self.config["dimension"] + 2
+        if self.config.get("geometry", "cartesian") != "cartesian":
+            return (state * self.metrics.volumes[None]).reshape(nf, -1).sum(axis=1)
+        return state.reshape(nf, -1).sum(axis=1) * np.prod(self.spacing)
+
+    @property
+    def metrics(self):
+        """Cell volumes and face areas (torchGeometry.Metrics, torchSession.py:297-301)."""
+        if getattr(self, "_metrics", None) is None:
+            self._metrics = _geometry.Metrics(self.config)
+        return self._metrics
 
     @property
     def integral(self):
@@ -223,7 +242,8 @@ class Simulation:
 
     @property
     def uses_metric_kernel(self):
-        return False
+        """torchSession.py:303-308."""
+        return self.config.get("geometry", "cartesian") != "cartesian" or "face_boundaries" in self.config
 
     @property
     def primitive(self):
@@ -242,8 +262,9 @@ class Simulation:
         return self.time == self.config["t_end"]
 
     def dt_limit(self):
-        """Simulation._dt on the committed state (torchSession.py:333-360)."""
-        return self._device.dt_limit()
+        """Simulation._dt on the committed state (torchSession.py:333-360), or
+        cfl_dt for metric sessions (torchFiniteVolume.py:66-98)."""
+        return self._device.metric_dt() if self.uses_metric_kernel else self._device.dt_limit()
 
     def device_totals(self):
         """Conserved totals reduced on the device (double)."""
@@ -257,7 +278,10 @@ class Simulation:
         c = self.config
         if self.steps >= c["max_steps"]:
             raise RuntimeError("Maximum steps reached before requested final time")
-        info = self._device.step(self.time, c["t_end"])
+        if self.uses_metric_kernel:
+            info = self._device.metric_step(self.time, c["t_end"])
+        else:
+            info = self._device.step(self.time, c["t_end"])
         nf = c["dimension"] + 2
         self._host = None
         self.boundary_integral += np.array(info.outward[:nf])
