@@ -23,6 +23,9 @@ workload's length). The JSON line reports:
 - served: seeded 3D ignition requests through the continuous-batching
   server (closed loop for capacity, then Poisson arrivals at 80% of it):
   p50/p99 time-to-solution and aggregate cell-updates/s;
+- metric: the metric/face scheme (curvilinear grid, face boundaries,
+  SSPRK2 MUSCL-Rusanov) on a cylindrical-sector blast, FP64 and FP32, with
+  its own CPU baseline;
 - cpu_baseline: the NumPy oracle (a restatement of the reference's CPU path)
   timed on this host on a bounded sample of the same workload.
 
@@ -259,6 +262,61 @@ def cpu_baseline(budget_s=12.0, n=40):
                       f"(restated reference CPU path, 1 thread), {wall:.1f} s"}
 
 
+def metric_problem(n):
+    """A cylindrical-sector blast on the metric/face scheme: (r, theta, z) in
+    [0.5, 1.5] x [0, pi/2] x [0, 1], n^3 cells, rho = 1 at rest, p = 10 in a
+    torus of radius 0.15 around (r, z) = (1, 0.5), p = 1 elsewhere; walls at
+    r = 0.5 and on both theta planes, a pressure outlet (with a backflow
+    state) at r = 1.5, periodic in z; Superbee, CFL 0.4."""
+    r = 0.5 + (np.arange(n) + 0.5) / n
+    z = (np.arange(n) + 0.5) / n
+    prim = np.zeros((5, n, n, n))
+    prim[0] = 1.0
+    hot = ((r[None, None, :] - 1.0) ** 2 + (z[:, None, None] - 0.5) ** 2) < 0.15 ** 2
+    prim[4] = np.where(hot, 10.0, 1.0) + 0.0 * prim[0]
+    faces = {"x_min": "wall", "x_max": {"type": "pressure_outlet", "pressure": 1.0,
+                                        "backflow": [1.0, -0.1, 0.0, 0.0, 1.0]},
+             "y_min": "wall", "y_max": "wall", "z_min": "periodic", "z_max": "periodic"}
+    opts = dict(geometry="annular", origins=[0.5, 0.0, 0.0], t_end=10.0, cfl=0.4, limiter="Superbee",
+                face_boundaries=faces)
+    return prim, [1.0, np.pi / 2, 1.0], opts
+
+
+def run_metric(n, precision, steps, warmup):
+    from torch_amd import Simulation
+
+    prim, lengths, opts = metric_problem(n)
+    sim = Simulation(prim, lengths, precision=precision, **opts)
+    for _ in range(warmup):
+        sim.step()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.step()
+    wall = time.perf_counter() - t0
+    fallbacks = sim.fallbacks
+    sim.close()
+    return dict(value=round(n ** 3 * steps / wall, 1), ms_per_step=round(1e3 * wall / steps, 4),
+                fallbacks=fallbacks, dtype="f64" if precision == "fp64" else "f32")
+
+
+def cpu_metric_baseline(budget_s=8.0, n=24):
+    """The NumPy oracle's metric step on the same workload at n^3, single thread."""
+    os.environ.setdefault("OPENBLAS_NUM_THREADS", "1")
+    from oracle import session
+
+    prim, lengths, opts = metric_problem(n)
+    sim = session.Session(prim, lengths, **opts)
+    sim.step()
+    t0 = time.perf_counter()
+    steps = 0
+    while steps < 3 or time.perf_counter() - t0 < budget_s:
+        sim.step()
+        steps += 1
+    wall = time.perf_counter() - t0
+    return {"value": round(n ** 3 * steps / wall, 1), "unit": "cell-updates/s", "cores": 1, "kind": "port",
+            "sample": f"cylindrical-sector blast {n}^3, {steps} metric steps of the NumPy oracle, {wall:.1f} s"}
+
+
 def copy_calibration(mib=1024, reps=3):
     """Device-to-device copies of a known size, for scaling rocprof's
     FETCH_SIZE/WRITE_SIZE counters (each copy reads and writes `mib` MiB)."""
@@ -281,7 +339,7 @@ def copy_calibration(mib=1024, reps=3):
     print(json.dumps({"copy_mib": mib, "reps": reps, "GBps_rw": round(2 * n * reps / wall / 1e9, 1)}))
 
 
-LEGS = ("fp64", "fp32", "batched", "error", "uneven", "served", "cpu")
+LEGS = ("fp64", "fp32", "batched", "error", "uneven", "served", "metric", "cpu")
 
 
 def _tile(text):
@@ -317,6 +375,8 @@ def main(argv=None):
     ap.add_argument("--serve-budget-gib", type=float, default=4.0, help="HBM budget of the served batch")
     ap.add_argument("--serve-round", type=int, default=4, help="steps between admission checks")
     ap.add_argument("--serve-closed-only", action="store_true", help="skip the open-loop (Poisson) run")
+    ap.add_argument("--metric-n", type=int, default=96, help="cells per axis of the metric/face workload")
+    ap.add_argument("--metric-steps", type=int, default=20)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device copies of MIB MiB (counter calibration) and exit")
     args = ap.parse_args(argv)
@@ -372,6 +432,18 @@ def main(argv=None):
             f"Arrhenius (ZND for ~10%), Ea/RT0 in [20,30], Q/RT0 in [10,50], Gaussian hot spot T/T0 in "
             f"[1.5,3], run length {args.serve_t_run} (induction ~0.1 of it), VanLeer, CFL 0.4, open "
             f"boundaries; FP32 batch, HBM budget {args.serve_budget_gib} GiB, {args.serve_round} steps/round")
+    metric = None
+    if "metric" in legs:
+        mn = args.metric_n
+        metric = {"workload": f"metric/face scheme: cylindrical-sector blast (annular r,theta,z grid) {mn}^3, "
+                              "walls, pressure outlet with backflow, periodic z, Superbee, CFL 0.4, SSPRK2 "
+                              "MUSCL-Rusanov, full Simulation.step (_metric_step)",
+                  "cells": mn ** 3, "steps": args.metric_steps, "unit": "cell-updates/s"}
+        for precision in ("fp64", "fp32"):
+            metric[precision] = run_metric(mn, precision, args.metric_steps, 2)
+            say(f"# metric {precision}: {metric[precision]['ms_per_step']} ms/step")
+        if "cpu" in legs:
+            metric["cpu_baseline"] = cpu_metric_baseline()
     if "cpu" in legs:
         cpu = cpu_baseline()
     head = r32 or r32s
@@ -398,6 +470,7 @@ def main(argv=None):
         "fp32_l1_rel": [float(f"{e:.3e}") for e in err] if err else None,
         "batched": uneven,
         "served": served,
+        "metric": metric,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -4,16 +4,18 @@
 // (python/cfdSolver/torchFiniteVolume.py:16-276, torchGeometry.py:93-213,
 // torchBoundaries.py:177-246, torchFlux.py:80-122).
 //
-// One thread per cell. For each direction a thread evaluates the two faces
-// of its cell: each face's left/right states come from the padded line
+// One thread per cell. For each direction a thread loads the five-entry
+// padded stencil of its cell and evaluates the cell's two faces: each
+// face's left/right states come from the padded line
 // (interior cells plus one boundary plane per end), with limited slopes of
 // the primitives (zeroed where half a slope would take density or pressure
 // to <= 1e-12; wrapped on periodic axes) and mirrored states on walls. A
 // face flux is a pure function of the line, so the two cells that share a
 // face compute it bit for bit alike, and the area-weighted divergence is
-// the reference's diff / volume. The thread of each line's first cell also
-// evaluates the line's last face, so the outward boundary flux is summed as
-// the reference sums it (last minus first per line). Volumes, areas and the
+// the reference's diff / volume. The outward boundary flux sums the
+// weighted fluxes of each line's last face minus its first (the threads of
+// the end cells contribute them). Kernels are instantiated per dimension so
+// every component index is a compile-time constant. Volumes, areas and the
 // geometric sources come from per-axis factor tables (outer products in the
 // reference's order).
 #include "fv_device.h"
@@ -40,73 +42,83 @@ constexpr int MT = 256;  // threads per block
 // NaN-propagating maximum (np.maximum).
 __device__ __forceinline__ real nmax(real a, real b) { return (a > b || a != a) ? a : b; }
 
+template <int NF>
 struct Pad {      // a padded line entry: primitives, gamma, fractions
-    real p[5];
+    real p[NF];
     real g;
     real q[2];
 };
 
-struct Line {     // one grid line along direction d through a cell
-    int d, n;     // direction and cells along it
+struct Line {     // one grid line along a direction through a cell
+    int n;        // cells along it
     long base;    // cell index of the line's first cell
-    long stride;  // index step along d
+    long stride;  // index step along the line
 };
 
 struct Ctx {
     const real* S;
     const real* q[2];
     long N;
-    int dim, nf, nq, reactive;
+    int nq, reactive;
     real gamma;
 };
 
-__device__ __forceinline__ void to_prim(const real* u, int nf, real g, real* p) {
+template <int NF>
+__device__ __forceinline__ void to_prim(const real* u, real g, real* p) {
     const real rho = u[0];
     p[0] = rho;
     real s2 = 0;
-    for (int k = 1; k < nf - 1; ++k) {
+#pragma unroll
+    for (int k = 1; k < NF - 1; ++k) {
         p[k] = u[k] / rho;
         s2 = k == 1 ? u[k] * u[k] : s2 + u[k] * u[k];
     }
-    p[nf - 1] = (g - (real)1) * (u[nf - 1] - ((real)0.5 * s2) / rho);
+    p[NF - 1] = (g - (real)1) * (u[NF - 1] - ((real)0.5 * s2) / rho);
 }
 
-__device__ __forceinline__ void to_cons(const real* p, int nf, real g, real* u) {
+template <int NF>
+__device__ __forceinline__ void to_cons(const real* p, real g, real* u) {
     const real rho = p[0];
     u[0] = rho;
     real s2 = 0;
-    for (int k = 1; k < nf - 1; ++k) {
+#pragma unroll
+    for (int k = 1; k < NF - 1; ++k) {
         u[k] = p[k] * rho;
         s2 = k == 1 ? p[k] * p[k] : s2 + p[k] * p[k];
     }
-    u[nf - 1] = p[nf - 1] / (g - (real)1) + ((real)0.5 * rho) * s2;
+    u[NF - 1] = p[NF - 1] / (g - (real)1) + ((real)0.5 * rho) * s2;
 }
 
-__device__ __forceinline__ Pad load_cell(const Ctx& c, long i) {
-    Pad e;
-    real u[5];
-    for (int k = 0; k < c.nf; ++k) u[k] = c.S[k * c.N + i];
+template <int NF>
+__device__ __forceinline__ Pad<NF> load_cell(const Ctx& c, long i) {
+    Pad<NF> e;
+    real u[NF];
+#pragma unroll
+    for (int k = 0; k < NF; ++k) u[k] = c.S[k * c.N + i];
     e.g = c.reactive ? gamma_of(c.q[0][i]) : c.gamma;
-    to_prim(u, c.nf, e.g, e.p);
-    for (int j = 0; j < 2; ++j) e.q[j] = j < c.nq ? c.q[j][i] : (real)0;
+    to_prim<NF>(u, e.g, e.p);
+    e.q[0] = c.nq > 0 ? c.q[0][i] : (real)0;
+    e.q[1] = c.nq > 1 ? c.q[1][i] : (real)0;
     return e;
 }
 
 // Boundary-plane state from the edge cell (torchBoundaries.py:177-246);
 // side -1 low, +1 high. Errors the reference raises set flag bits.
-__device__ void ghost(Pad& e, const MFace& f, int d, int side, int nf, int nq, int reactive, int& bad) {
-    const int nrm = d + 1;
+template <int NF, int D>
+__device__ __forceinline__ void ghost(Pad<NF>& e, const MFace& f, int side, int nq, int reactive, int& bad) {
+    constexpr int nrm = D + 1, E = NF - 1;
     if (f.type == FACE_WALL) {
         e.p[nrm] = -e.p[nrm];
     } else if (f.type == FACE_INFLOW) {
         if (reactive) e.g = gamma_of((real)f.lam);
         if (nq > 0) e.q[0] = (real)f.lam;
         if (nq > 1) e.q[1] = (real)f.beta;
-        for (int k = 0; k < nf; ++k) e.p[k] = (real)f.prim[k];
+#pragma unroll
+        for (int k = 0; k < NF; ++k) e.p[k] = (real)f.prim[k];
     } else if (f.type == FACE_STAGNATION) {
         const real gin = e.g;
         const real g = reactive ? gamma_of((real)f.lam) : gin;
-        const real a_in = sqrt((gin * e.p[nf - 1]) / e.p[0]);
+        const real a_in = sqrt((gin * e.p[E]) / e.p[0]);
         const real riemann = (real)(-side) * e.p[nrm] - ((real)2 * a_in) / (gin - (real)1);
         const real a0sq = (g * (real)f.gas_constant) * (real)f.total_temperature;
         const real k = (g + (real)1) / (g - (real)1);
@@ -119,33 +131,35 @@ __device__ void ghost(Pad& e, const MFace& f, int d, int side, int nf, int nq, i
             un = a;
         }
         if (un < (real)0 || a <= (real)0) bad |= fvl::F_BC_SUPPLY;
-        for (int m = 1; m < nf - 1; ++m) e.p[m] = 0;
+#pragma unroll
+        for (int m = 1; m < NF - 1; ++m) e.p[m] = 0;
         e.p[nrm] = (real)(-side) * un;
-        e.p[nf - 1] = (real)f.total_pressure * pow((a * a) / a0sq, g / (g - (real)1));
-        e.p[0] = (g * e.p[nf - 1]) / (a * a);
+        e.p[E] = (real)f.total_pressure * pow((a * a) / a0sq, g / (g - (real)1));
+        e.p[0] = (g * e.p[E]) / (a * a);
         e.g = g;
         if (nq > 0) e.q[0] = (real)f.lam;
         if (nq > 1) e.q[1] = (real)f.beta;
     } else if (f.type == FACE_PRESSURE_OUTLET) {
         const real g = e.g;
-        const real a = sqrt((g * e.p[nf - 1]) / e.p[0]);
+        const real a = sqrt((g * e.p[E]) / e.p[0]);
         const real un = (real)side * e.p[nrm];
         const bool sub = un < a;
         const real P = (real)f.pressure;
-        const real rho = e.p[0] * pow(P / e.p[nf - 1], (real)1 / g);
+        const real rho = e.p[0] * pow(P / e.p[E], (real)1 / g);
         const real a_new = sqrt((g * P) / rho);
         const real vel = un + ((real)2 * (a - a_new)) / (g - (real)1);
         const bool back = sub && (vel < (real)0 || un < (real)0);
         if (sub) {
             e.p[0] = rho;
-            e.p[nf - 1] = P;
+            e.p[E] = P;
             e.p[nrm] = (real)side * vel;
         }
         if (back) {
             if (!f.has_backflow) {
                 bad |= fvl::F_BC_BACKFLOW;
             } else {
-                for (int k = 0; k < nf; ++k) e.p[k] = (real)f.prim[k];
+#pragma unroll
+                for (int k = 0; k < NF; ++k) e.p[k] = (real)f.prim[k];
                 if (reactive) e.g = gamma_of((real)f.lam);
                 if (nq > 0) e.q[0] = (real)f.lam;
                 if (nq > 1) e.q[1] = (real)f.beta;
@@ -155,85 +169,75 @@ __device__ void ghost(Pad& e, const MFace& f, int d, int side, int nf, int nq, i
 }
 
 // Padded entry jj in [0, n + 1] of a line (torchFiniteVolume.py:33-63).
-__device__ Pad padded(const Ctx& c, const fvl::Metric& m, const Line& L, int jj, int& bad) {
-    if (jj >= 1 && jj <= L.n) return load_cell(c, L.base + (jj - 1) * L.stride);
+template <int NF, int D>
+__device__ __forceinline__ Pad<NF> padded(const Ctx& c, const fvl::Metric& m, const Line& L, int jj, int& bad) {
+    if (jj >= 1 && jj <= L.n) return load_cell<NF>(c, L.base + (jj - 1) * L.stride);
     const bool low = jj == 0;
-    const MFace& f = m.face[2 * L.d + (low ? 0 : 1)];
-    if (f.type == FACE_PERIODIC) return load_cell(c, L.base + (low ? L.n - 1 : 0) * L.stride);
-    Pad e = load_cell(c, L.base + (low ? 0 : L.n - 1) * L.stride);
-    ghost(e, f, L.d, low ? -1 : 1, c.nf, c.nq, c.reactive, bad);
+    const MFace& f = m.face[2 * D + (low ? 0 : 1)];
+    if (f.type == FACE_PERIODIC) return load_cell<NF>(c, L.base + (low ? L.n - 1 : 0) * L.stride);
+    Pad<NF> e = load_cell<NF>(c, L.base + (low ? 0 : L.n - 1) * L.stride);
+    ghost<NF, D>(e, f, low ? -1 : 1, c.nq, c.reactive, bad);
     return e;
 }
 
-// Limited slope at interior padded index jj (1..n) with the positivity mask.
-template <int LIM>
-__device__ void slope_interior(const Ctx& c, const fvl::Metric& m, const Line& L, int jj, real* s, int& bad) {
-    const Pad a = padded(c, m, L, jj - 1, bad), b = padded(c, m, L, jj, bad), z = padded(c, m, L, jj + 1, bad);
-    for (int k = 0; k < c.nf; ++k) s[k] = limited<LIM, 3>(b.p[k] - a.p[k], z.p[k] - b.p[k]);
-    const int e = c.nf - 1;
-    if (b.p[0] - (real)0.5 * fabs(s[0]) <= (real)1e-12 || b.p[e] - (real)0.5 * fabs(s[e]) <= (real)1e-12)
-        for (int k = 0; k < c.nf; ++k) s[k] = s[k] * (real)0;
+// Limited slope of entry b between a and z, zeroed where half of it would
+// take density or pressure to <= 1e-12 (torchFiniteVolume.py:143-156).
+template <int NF, int LIM>
+__device__ __forceinline__ void slope3(const Pad<NF>& a, const Pad<NF>& b, const Pad<NF>& z, real* s) {
+#pragma unroll
+    for (int k = 0; k < NF; ++k) s[k] = limited<LIM, 3>(b.p[k] - a.p[k], z.p[k] - b.p[k]);
+    if (b.p[0] - (real)0.5 * fabs(s[0]) <= (real)1e-12 || b.p[NF - 1] - (real)0.5 * fabs(s[NF - 1]) <= (real)1e-12) {
+#pragma unroll
+        for (int k = 0; k < NF; ++k) s[k] = s[k] * (real)0;
+    }
 }
 
-// Slope at padded index jj (0..n+1): zero on boundary planes except on
-// periodic axes, which copy the wrapped interior slope.
-template <int LIM>
-__device__ void slope_at(const Ctx& c, const fvl::Metric& m, const Line& L, int jj, int first_order, real* s,
-                         int& bad) {
-    for (int k = 0; k < 5; ++k) s[k] = 0;
-    if (first_order) return;
-    if (jj >= 1 && jj <= L.n) return slope_interior<LIM>(c, m, L, jj, s, bad);
-    if (m.face[2 * L.d].type == FACE_PERIODIC) slope_interior<LIM>(c, m, L, jj == 0 ? L.n : 1, s, bad);
-}
-
-// Rusanov flux of primitive face states along d (torchFlux.py:80-100).
-__device__ void rusanov(const real* l, const real* r, real gl, real gr, int d, int nf, real* F) {
-    real ul[5], ur[5], fl[5], fr[5];
-    to_cons(l, nf, gl, ul);
-    to_cons(r, nf, gr, ur);
-    const int n = d + 1, e = nf - 1;
-    for (int k = 0; k < nf; ++k) {
+// Rusanov flux of primitive face states along D (torchFlux.py:80-100).
+template <int NF, int D>
+__device__ __forceinline__ void rusanov(const real* l, const real* r, real gl, real gr, real* F) {
+    constexpr int n = D + 1, E = NF - 1;
+    real ul[NF], ur[NF], fl[NF], fr[NF];
+    to_cons<NF>(l, gl, ul);
+    to_cons<NF>(r, gr, ur);
+#pragma unroll
+    for (int k = 0; k < NF; ++k) {
         fl[k] = ul[k] * l[n];
         fr[k] = ur[k] * r[n];
     }
-    fl[n] = fl[n] + l[e];
-    fr[n] = fr[n] + r[e];
-    fl[e] = fl[e] + l[n] * l[e];
-    fr[e] = fr[e] + r[n] * r[e];
-    const real s = nmax(fabs(l[n]) + sqrt((gl * l[e]) / l[0]), fabs(r[n]) + sqrt((gr * r[e]) / r[0]));
-    for (int k = 0; k < nf; ++k) F[k] = (real)0.5 * ((fl[k] + fr[k]) - s * (ur[k] - ul[k]));
+    fl[n] = fl[n] + l[E];
+    fr[n] = fr[n] + r[E];
+    fl[E] = fl[E] + l[n] * l[E];
+    fr[E] = fr[E] + r[n] * r[E];
+    const real s = nmax(fabs(l[n]) + sqrt((gl * l[E]) / l[0]), fabs(r[n]) + sqrt((gr * r[E]) / r[0]));
+#pragma unroll
+    for (int k = 0; k < NF; ++k) F[k] = (real)0.5 * ((fl[k] + fr[k]) - s * (ur[k] - ul[k]));
 }
 
-// Flux through face f (0..n) of a line, and the fractions either side.
-template <int LIM>
-__device__ void face_flux(const Ctx& c, const fvl::Metric& m, const Line& L, int f, int first_order, real* F,
-                          real* ql, real* qr, int& bad) {
-    Pad a = padded(c, m, L, f, bad), b = padded(c, m, L, f + 1, bad);
-    real sa[5], sb[5];
-    slope_at<LIM>(c, m, L, f, first_order, sa, bad);
-    slope_at<LIM>(c, m, L, f + 1, first_order, sb, bad);
-    real lft[5], rgt[5];
-    for (int k = 0; k < c.nf; ++k) {
+// Face flux from the entries either side and their slopes, with the wall
+// mirror at a boundary face (torchFiniteVolume.py:163-188).
+template <int NF, int D>
+__device__ __forceinline__ void face(const Pad<NF>& a, const real* sa, const Pad<NF>& b, const real* sb,
+                                     bool wall_lo, bool wall_hi, real* F) {
+    real lft[NF], rgt[NF];
+#pragma unroll
+    for (int k = 0; k < NF; ++k) {
         lft[k] = a.p[k] + (real)0.5 * sa[k];
         rgt[k] = b.p[k] - (real)0.5 * sb[k];
     }
     real gl = a.g, gr = b.g;
-    const int n = L.d + 1;
-    if (f == 0 && m.face[2 * L.d].type == FACE_WALL) {
-        for (int k = 0; k < c.nf; ++k) lft[k] = rgt[k];
+    if (wall_lo) {
+#pragma unroll
+        for (int k = 0; k < NF; ++k) lft[k] = rgt[k];
         gl = gr;
-        lft[n] = -lft[n];
+        lft[D + 1] = -lft[D + 1];
     }
-    if (f == L.n && m.face[2 * L.d + 1].type == FACE_WALL) {
-        for (int k = 0; k < c.nf; ++k) rgt[k] = lft[k];
+    if (wall_hi) {
+#pragma unroll
+        for (int k = 0; k < NF; ++k) rgt[k] = lft[k];
         gr = gl;
-        rgt[n] = -rgt[n];
+        rgt[D + 1] = -rgt[D + 1];
     }
-    rusanov(lft, rgt, gl, gr, L.d, c.nf, F);
-    for (int j = 0; j < 2; ++j) {
-        ql[j] = a.q[j];
-        qr[j] = b.q[j];
-    }
+    rusanov<NF, D>(lft, rgt, gl, gr, F);
 }
 
 __device__ __forceinline__ double product(const double* const* f, int dim, const int* idx) {
@@ -242,63 +246,78 @@ __device__ __forceinline__ double product(const double* const* f, int dim, const
     return v;
 }
 
-// Area of direction d's face `face` at the cell's other coordinates.
-__device__ __forceinline__ double area_at(const fvl::Metric& m, int dim, int d, const int* idx, int face) {
-    int j[3] = {idx[0], idx[1], idx[2]};
-    j[d] = face;
-    return product(m.af[d], dim, j);
+template <int DIM>
+__device__ __forceinline__ double volume_at(const fvl::Metric& m, const int* idx) {
+    double v = 1.0;
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) v = v * m.vf[a][idx[a]];
+    return v;
+}
+
+// Area of direction D's face `at` at the cell's other coordinates.
+template <int DIM, int D>
+__device__ __forceinline__ double area_at(const fvl::Metric& m, const int* idx, int at) {
+    double v = 1.0;
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) v = v * m.af[D][a][a == D ? at : idx[a]];
+    return v;
 }
 
 // Geometric momentum sources (torchGeometry.py:184-213) from the cell's
 // primitives and its area divergences.
-__device__ void geometric_source(const fvl::Metric& m, int dim, const real* p, double rad, double cot, real* out) {
-    const int nf = dim + 2;
-    for (int k = 0; k < nf; ++k) out[k] = 0;
-    if (m.geometry == GEO_CARTESIAN) return;
-    const real rho = p[0], pr = p[nf - 1];
+template <int DIM>
+__device__ __forceinline__ void geometric_source(int geometry, const real* p, double rad, double cot, real* out) {
+    constexpr int NF = DIM + 2;
+#pragma unroll
+    for (int k = 0; k < NF; ++k) out[k] = 0;
+    if (geometry == GEO_CARTESIAN) return;
+    const real rho = p[0], pr = p[NF - 1];
     const real radial = (real)rad;
     out[1] = pr * radial;
-    if (m.geometry == GEO_SPHERICAL) {
+    if (geometry == GEO_SPHERICAL) {
         const real inv_r = radial / (real)2;
         real s2 = 0;
-        for (int k = 2; k < nf - 1; ++k) s2 = k == 2 ? p[k] * p[k] : s2 + p[k] * p[k];
+#pragma unroll
+        for (int k = 2; k < NF - 1; ++k) s2 = k == 2 ? p[k] * p[k] : s2 + p[k] * p[k];
         out[1] = out[1] + (rho * s2) * inv_r;
-        if (dim >= 2) {
+        if (DIM >= 2) {
             const real cotr = (real)cot;
             out[2] = pr * cotr - ((rho * p[1]) * p[2]) * inv_r;
-            if (dim == 3) {
+            if (DIM == 3) {
                 out[2] = out[2] + (rho * (p[3] * p[3])) * cotr;
                 out[3] = ((-rho) * p[3]) * (p[1] * inv_r + p[2] * cotr);
             }
         }
-    } else if (dim >= 2 && !(m.geometry == GEO_CYLINDRICAL && dim == 2)) {
+    } else if (DIM >= 2 && !(geometry == GEO_CYLINDRICAL && DIM == 2)) {
         out[1] = out[1] + (rho * (p[2] * p[2])) * radial;
         out[2] = (((-rho) * p[1]) * p[2]) * radial;
     }
 }
 
 // physical_state of one state (torchSolver.py:1038-1046).
-__device__ __forceinline__ bool physical_n(const real* u, int nf) {
+template <int NF>
+__device__ __forceinline__ bool physical_n(const real* u) {
     bool finite = true;
     real s2 = 0;
-    for (int k = 0; k < nf; ++k) finite = finite & isfinite(u[k]);
-    for (int k = 1; k < nf - 1; ++k) s2 = k == 1 ? u[k] * u[k] : s2 + u[k] * u[k];
-    return finite & (u[0] > (real)1e-12) & (u[nf - 1] - ((real)0.5 * s2) / u[0] > (real)1e-12);
+#pragma unroll
+    for (int k = 0; k < NF; ++k) finite = finite & isfinite(u[k]);
+#pragma unroll
+    for (int k = 1; k < NF - 1; ++k) s2 = k == 1 ? u[k] * u[k] : s2 + u[k] * u[k];
+    return finite & (u[0] > (real)1e-12) & (u[NF - 1] - ((real)0.5 * s2) / u[0] > (real)1e-12);
 }
 
-// Deterministic block sums of `n` slots (tree in LDS) into partials.
-__device__ void block_sums(const double* v, int first_slot, int n, double* partials) {
+// Deterministic block sum of one slot (tree in LDS) into partials.
+__device__ __forceinline__ void block_sum(double v, int slot, double* partials) {
     __shared__ double red[MT];
-    for (int k = 0; k < n; ++k) {
-        red[threadIdx.x] = v[k];
-        __syncthreads();
-        for (int s = MT / 2; s > 0; s >>= 1) {
-            if ((int)threadIdx.x < s) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + s];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) partials[(long)blockIdx.x * fvl::NSLOT + first_slot + k] = red[0];
+    red[threadIdx.x] = v;
+    __syncthreads();
+#pragma unroll
+    for (int s = MT / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + s];
         __syncthreads();
     }
+    if (threadIdx.x == 0) partials[(long)blockIdx.x * fvl::NSLOT + slot] = red[0];
+    __syncthreads();
 }
 
 __device__ void block_max(double v, double* partials) {
@@ -324,20 +343,80 @@ __device__ __forceinline__ void cell_coords(const fvl::Grid& g, long i, int* idx
     idx[2] = (int)(i / ((long)g.nx * g.ny));
 }
 
-__device__ __forceinline__ Line line_of(const fvl::Grid& g, int d, long i, const int* idx) {
+template <int D>
+__device__ __forceinline__ Line line_of(const fvl::Grid& g, long i, const int* idx) {
     Line L;
-    L.d = d;
-    L.n = d == 0 ? g.nx : (d == 1 ? g.ny : g.nz);
-    L.stride = d == 0 ? 1 : (d == 1 ? (long)g.nx : (long)g.nx * g.ny);
-    L.base = i - (long)idx[d] * L.stride;
+    L.n = D == 0 ? g.nx : (D == 1 ? g.ny : g.nz);
+    L.stride = D == 0 ? 1 : (D == 1 ? (long)g.nx : (long)g.nx * g.ny);
+    L.base = i - (long)idx[D] * L.stride;
     return L;
 }
 
-template <int LIM>
-__global__ void __launch_bounds__(MT) k_metric_stage(fvl::Grid g, fvl::Metric m, fvl::MStage s) {
+// Change of one cell along direction D: the two faces of the cell from the
+// padded stencil jc-2..jc+2 (jc = at + 1), with the periodic slope wrap of
+// the boundary entries, evaluated face by face to keep few entries live;
+// accumulates -(W_hi - W_lo) / V into k, the species
+// change into kq and the boundary faces' weighted fluxes into out_sum.
+template <int DIM, int D, int LIM>
+__device__ __forceinline__ void direction(const Ctx& c, const fvl::Metric& m, const fvl::Grid& g, long i,
+                                          const int* idx, int first_order, real Vr, real* k, real* kq,
+                                          double* out_sum, int& bad) {
+    constexpr int NF = DIM + 2;
+    const Line L = line_of<D>(g, i, idx);
+    const int at = idx[D], jc = at + 1, n = L.n;
+    const bool periodic = m.face[2 * D].type == FACE_PERIODIC;
+    const bool wall_lo = at == 0 && m.face[2 * D].type == FACE_WALL;
+    const bool wall_hi = at == n - 1 && m.face[2 * D + 1].type == FACE_WALL;
+    // Stencil entries jc-1, jc, jc+1 (always inside [0, n+1]); the outer
+    // two (jc-2, jc+2) only feed a slope and are loaded where needed.
+    const Pad<NF> Pm = padded<NF, D>(c, m, L, jc - 1, bad);
+    const Pad<NF> P0 = padded<NF, D>(c, m, L, jc, bad);
+    const Pad<NF> Pp = padded<NF, D>(c, m, L, jc + 1, bad);
+    real sm[NF], s0[NF], sp[NF];
+#pragma unroll
+    for (int q = 0; q < NF; ++q) sm[q] = s0[q] = sp[q] = 0;
+    if (!first_order) {
+        slope3<NF, LIM>(Pm, P0, Pp, s0);
+        if (jc - 1 >= 1) {
+            slope3<NF, LIM>(padded<NF, D>(c, m, L, jc - 2, bad), Pm, P0, sm);
+        } else if (periodic) {  // entry 0 takes the slope of entry n
+            slope3<NF, LIM>(padded<NF, D>(c, m, L, n - 1, bad), padded<NF, D>(c, m, L, n, bad),
+                            padded<NF, D>(c, m, L, n + 1, bad), sm);
+        }
+    }
+    real Flo[NF], Fhi[NF];
+    face<NF, D>(Pm, sm, P0, s0, wall_lo, false, Flo);
+    if (!first_order) {
+        if (jc + 1 <= n) {
+            slope3<NF, LIM>(P0, Pp, padded<NF, D>(c, m, L, jc + 2, bad), sp);
+        } else if (periodic) {  // entry n + 1 takes the slope of entry 1
+            slope3<NF, LIM>(padded<NF, D>(c, m, L, 0, bad), padded<NF, D>(c, m, L, 1, bad),
+                            padded<NF, D>(c, m, L, 2, bad), sp);
+        }
+    }
+    face<NF, D>(P0, s0, Pp, sp, false, wall_hi, Fhi);
+    const real Alo = (real)area_at<DIM, D>(m, idx, at), Ahi = (real)area_at<DIM, D>(m, idx, at + 1);
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+        const real wlo = Flo[f] * Alo, whi = Fhi[f] * Ahi;
+        k[f] = k[f] - (whi - wlo) / Vr;
+        if (at == n - 1) out_sum[f] += (double)whi;
+        if (at == 0) out_sum[f] -= (double)wlo;
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        if (j >= c.nq) break;
+        const real mlo = (Flo[0] * (Flo[0] >= (real)0 ? Pm.q[j] : P0.q[j])) * Alo;
+        const real mhi = (Fhi[0] * (Fhi[0] >= (real)0 ? P0.q[j] : Pp.q[j])) * Ahi;
+        kq[j] = kq[j] - (mhi - mlo) / Vr;
+    }
+}
+
+template <int DIM, int LIM>
+__global__ void __launch_bounds__(MT, 2) k_metric_stage(fvl::Grid g, fvl::Metric m, fvl::MStage s) {
+    constexpr int NF = DIM + 2;
     const long N = (long)g.nx * g.ny * g.nz;
     const long i = blockIdx.x * (long)MT + threadIdx.x;
-    const int dim = g.dim, nf = dim + 2;
     double sums[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};  // outward x5, integrated source x5
     int bad = 0;
     if (i < N) {
@@ -346,64 +425,45 @@ __global__ void __launch_bounds__(MT) k_metric_stage(fvl::Grid g, fvl::Metric m,
         c.q[0] = (const real*)s.qS[0];
         c.q[1] = (const real*)s.qS[1];
         c.N = N;
-        c.dim = dim;
-        c.nf = nf;
         c.nq = s.nq;
         c.reactive = s.reactive;
         c.gamma = (real)s.gamma;
         int idx[3];
         cell_coords(g, i, idx);
-        const double V = product(m.vf, dim, idx);
-        const Pad me = load_cell(c, i);
+        const double V = volume_at<DIM>(m, idx);
+        const Pad<NF> me = load_cell<NF>(c, i);
         double rad = 0, cot = 0;
         if (m.geometry != GEO_CARTESIAN) {
-            rad = (area_at(m, dim, 0, idx, idx[0] + 1) - area_at(m, dim, 0, idx, idx[0])) / V;
-            if (dim >= 2) cot = (area_at(m, dim, 1, idx, idx[1] + 1) - area_at(m, dim, 1, idx, idx[1])) / V;
+            rad = (area_at<DIM, 0>(m, idx, idx[0] + 1) - area_at<DIM, 0>(m, idx, idx[0])) / V;
+            if (DIM >= 2) cot = (area_at<DIM, (DIM >= 2 ? 1 : 0)>(m, idx, idx[1] + 1) -
+                                 area_at<DIM, (DIM >= 2 ? 1 : 0)>(m, idx, idx[1])) / V;
         }
-        real src[5], k[5], kq[2] = {0, 0};
-        geometric_source(m, dim, me.p, rad, cot, src);
-        for (int f = 0; f < nf; ++f) k[f] = src[f];
-        for (int f = 0; f < nf; ++f) sums[5 + f] = (double)src[f] * V;
+        real k[NF], kq[2] = {0, 0};
+        geometric_source<DIM>(m.geometry, me.p, rad, cot, k);
+#pragma unroll
+        for (int f = 0; f < NF; ++f) sums[5 + f] = (double)k[f] * V;
         const real Vr = (real)V;
-        for (int d = 0; d < dim; ++d) {
-            const Line L = line_of(g, d, i, idx);
-            const int at = idx[d];
-            real Flo[5], Fhi[5], ql[2], qr[2], hl[2], hr[2];
-            face_flux<LIM>(c, m, L, at, s.first_order, Flo, ql, qr, bad);
-            face_flux<LIM>(c, m, L, at + 1, s.first_order, Fhi, hl, hr, bad);
-            const real Alo = (real)area_at(m, dim, d, idx, at), Ahi = (real)area_at(m, dim, d, idx, at + 1);
-            real Wlo[5], Whi[5];
-            for (int f = 0; f < nf; ++f) {
-                Wlo[f] = Flo[f] * Alo;
-                Whi[f] = Fhi[f] * Ahi;
-                k[f] = k[f] - (Whi[f] - Wlo[f]) / Vr;
-            }
-            for (int j = 0; j < s.nq; ++j) {
-                const real mlo = (Flo[0] * (Flo[0] >= (real)0 ? ql[j] : qr[j])) * Alo;
-                const real mhi = (Fhi[0] * (Fhi[0] >= (real)0 ? hl[j] : hr[j])) * Ahi;
-                kq[j] = kq[j] - (mhi - mlo) / Vr;
-            }
-            if (at == 0) {
-                // last minus first face of this line (torchFiniteVolume.py:209-217)
-                real Fl[5], a2[2], b2[2];
-                face_flux<LIM>(c, m, L, L.n, s.first_order, Fl, a2, b2, bad);
-                const real Alast = (real)area_at(m, dim, d, idx, L.n);
-                for (int f = 0; f < nf; ++f) sums[f] += (double)(Fl[f] * Alast - Wlo[f]);
-            }
-        }
-        real out[5], qo[2] = {0, 0};
+        direction<DIM, 0, LIM>(c, m, g, i, idx, s.first_order, Vr, k, kq, sums, bad);
+        if (DIM >= 2) direction<DIM, (DIM >= 2 ? 1 : 0), LIM>(c, m, g, i, idx, s.first_order, Vr, k, kq, sums, bad);
+        if (DIM >= 3) direction<DIM, (DIM >= 3 ? 2 : 0), LIM>(c, m, g, i, idx, s.first_order, Vr, k, kq, sums, bad);
+        real out[NF], qo[2] = {0, 0};
         const real* S = (const real*)s.S;
         if (s.mode == 0) {
-            for (int f = 0; f < nf; ++f) out[f] = k[f];
-            for (int j = 0; j < s.nq; ++j) qo[j] = kq[j];
+#pragma unroll
+            for (int f = 0; f < NF; ++f) out[f] = k[f];
+#pragma unroll
+            for (int j = 0; j < 2; ++j) qo[j] = kq[j];
         } else {
             const real* U0 = (const real*)s.U0;
             const real dt = (real)s.dt;
-            for (int f = 0; f < nf; ++f)
+#pragma unroll
+            for (int f = 0; f < NF; ++f)
                 out[f] = s.mode == 1 ? S[f * N + i] + dt * k[f]
                                      : (real)0.5 * ((U0[f * N + i] + S[f * N + i]) + dt * k[f]);
-            if (!physical_n(out, nf)) bad |= fvl::F_M_UNPHYS;
-            for (int j = 0; j < s.nq; ++j) {
+            if (!physical_n<NF>(out)) bad |= fvl::F_M_UNPHYS;
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                if (j >= s.nq) break;
                 const real held = U0[i] * ((const real*)s.q0[j])[i];
                 real q;
                 if (s.mode == 1) {
@@ -415,22 +475,44 @@ __global__ void __launch_bounds__(MT) k_metric_stage(fvl::Grid g, fvl::Metric m,
                 qo[j] = rmin(rmax(q, (real)0), (real)1);
             }
         }
-        for (int f = 0; f < nf; ++f) ((real*)s.out)[f * N + i] = out[f];
-        for (int j = 0; j < s.nq; ++j) ((real*)s.qout[j])[i] = qo[j];
+#pragma unroll
+        for (int f = 0; f < NF; ++f) ((real*)s.out)[f * N + i] = out[f];
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+            if (j < s.nq) ((real*)s.qout[j])[i] = qo[j];
     }
     if (bad) atomicOr(s.flags, bad);
-    block_sums(sums, fvl::S_OUT, 5, s.partials);
-    block_sums(sums + 5, S_MSRC, 5, s.partials);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) block_sum(sums[k], fvl::S_OUT + k, s.partials);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) block_sum(sums[5 + k], S_MSRC + k, s.partials);
 }
 
-// CFL rate: sum over directions of max(A_lo s_lo, A_hi s_hi) / V with face
-// speeds max(|u_d| + a) of the two sides; boundary faces use the boundary
+// Larger |u_D| + a of a face's two sides, boundary faces with the boundary
 // plane's state (torchFiniteVolume.py:66-98).
+template <int DIM, int D>
+__device__ __forceinline__ real face_rate(const Ctx& c, const fvl::Metric& m, const fvl::Grid& g, long i,
+                                          const int* idx, int& bad) {
+    constexpr int NF = DIM + 2;
+    const Line L = line_of<D>(g, i, idx);
+    const int at = idx[D];
+    auto speed = [&](int jj) {
+        const Pad<NF> e = padded<NF, D>(c, m, L, jj, bad);
+        return fabs(e.p[D + 1]) + sqrt((e.g * e.p[NF - 1]) / e.p[0]);
+    };
+    const real sc = speed(at + 1);
+    const real slo = nmax(speed(at), sc), shi = nmax(sc, speed(at + 2));
+    const real a_lo = (real)area_at<DIM, D>(m, idx, at) * slo;
+    const real a_hi = (real)area_at<DIM, D>(m, idx, at + 1) * shi;
+    return nmax(a_lo, a_hi);
+}
+
+// CFL rate: sum over directions of max(A_lo s_lo, A_hi s_hi) / V.
+template <int DIM>
 __global__ void __launch_bounds__(MT) k_metric_cfl(fvl::Grid g, fvl::Metric m, const real* U, const real* lam,
                                                    double gamma, int reactive, double* partials, int* flags) {
     const long N = (long)g.nx * g.ny * g.nz;
     const long i = blockIdx.x * (long)MT + threadIdx.x;
-    const int dim = g.dim, nf = dim + 2;
     double rate = 0;
     int bad = 0;
     if (i < N) {
@@ -439,28 +521,16 @@ __global__ void __launch_bounds__(MT) k_metric_cfl(fvl::Grid g, fvl::Metric m, c
         c.q[0] = lam;
         c.q[1] = nullptr;
         c.N = N;
-        c.dim = dim;
-        c.nf = nf;
         c.nq = 0;
         c.reactive = reactive;
         c.gamma = (real)gamma;
         int idx[3];
         cell_coords(g, i, idx);
-        const double V = product(m.vf, dim, idx);
+        const real V = (real)volume_at<DIM>(m, idx);
         real r = 0;
-        for (int d = 0; d < dim; ++d) {
-            const Line L = line_of(g, d, i, idx);
-            const int at = idx[d];
-            auto speed = [&](int jj) {
-                const Pad e = padded(c, m, L, jj, bad);
-                return fabs(e.p[d + 1]) + sqrt((e.g * e.p[nf - 1]) / e.p[0]);
-            };
-            const real sc = speed(at + 1);
-            const real slo = nmax(speed(at), sc), shi = nmax(sc, speed(at + 2));
-            const real a_lo = (real)area_at(m, dim, d, idx, at) * slo;
-            const real a_hi = (real)area_at(m, dim, d, idx, at + 1) * shi;
-            r = r + nmax(a_lo, a_hi) / (real)V;
-        }
+        r = r + face_rate<DIM, 0>(c, m, g, i, idx, bad) / V;
+        if (DIM >= 2) r = r + face_rate<DIM, (DIM >= 2 ? 1 : 0)>(c, m, g, i, idx, bad) / V;
+        if (DIM >= 3) r = r + face_rate<DIM, (DIM >= 3 ? 2 : 0)>(c, m, g, i, idx, bad) / V;
         rate = (double)r;
     }
     if (bad) atomicOr(flags, bad);
@@ -476,29 +546,45 @@ __global__ void __launch_bounds__(MT) k_metric_source(fvl::Grid g, fvl::Metric m
     const long N = (long)g.nx * g.ny * g.nz;
     const long i = blockIdx.x * (long)MT + threadIdx.x;
     const int dim = g.dim, e = dim + 1;
-    double v[1] = {0};
+    double v = 0;
     if (i < N) {
         int idx[3];
         cell_coords(g, i, idx);
         const double V = product(m.vf, dim, idx);
         const long o = (long)e * N + i;
-        v[0] = (double)((((old[o] - start[o]) + state[o]) - hydro[o]) * (real)V);
+        v = (double)((((old[o] - start[o]) + state[o]) - hydro[o]) * (real)V);
     }
-    block_sums(v, S_MSRC + e, 1, partials);
+    block_sum(v, S_MSRC + e, partials);
 }
 
 // Stand-alone Rusanov over face arrays (rusanov_flux_grid, torchFlux.py:103-122).
-__global__ void k_rusanov_faces(int nv, int dir, long mfaces, const real* l, const real* r, const real* gl,
-                                const real* gr, real* out) {
+template <int NF, int D>
+__global__ void k_rusanov_faces(long mfaces, const real* l, const real* r, const real* gl, const real* gr,
+                                real* out) {
     const long f = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (f >= mfaces) return;
-    real a[5], b[5], F[5];
-    for (int k = 0; k < nv; ++k) {
+    real a[NF], b[NF], F[NF];
+#pragma unroll
+    for (int k = 0; k < NF; ++k) {
         a[k] = l[k * mfaces + f];
         b[k] = r[k * mfaces + f];
     }
-    rusanov(a, b, gl[f], gr[f], dir, nv, F);
-    for (int k = 0; k < nv; ++k) out[k * mfaces + f] = F[k];
+    rusanov<NF, D>(a, b, gl[f], gr[f], F);
+#pragma unroll
+    for (int k = 0; k < NF; ++k) out[k * mfaces + f] = F[k];
+}
+
+template <int DIM>
+hipError_t launch_stage(const fvl::Grid& g, const fvl::Metric& m, const fvl::MStage& s, int blocks,
+                        hipStream_t st) {
+    switch (s.limiter) {
+        case VANLEER: k_metric_stage<DIM, VANLEER><<<blocks, MT, 0, st>>>(g, m, s); break;
+        case ALBADA: k_metric_stage<DIM, ALBADA><<<blocks, MT, 0, st>>>(g, m, s); break;
+        case MINBEE: k_metric_stage<DIM, MINBEE><<<blocks, MT, 0, st>>>(g, m, s); break;
+        case SUPERBEE: k_metric_stage<DIM, SUPERBEE><<<blocks, MT, 0, st>>>(g, m, s); break;
+        default: k_metric_stage<DIM, UNOS><<<blocks, MT, 0, st>>>(g, m, s); break;
+    }
+    return hipGetLastError();
 }
 
 }  // namespace
@@ -510,20 +596,19 @@ int metric_blocks(const fvl::Grid& g) {
 
 hipError_t metric_stage(const fvl::Grid& g, const fvl::Metric& m, const fvl::MStage& s, hipStream_t st) {
     const int blocks = metric_blocks(g);
-    switch (s.limiter) {
-        case VANLEER: k_metric_stage<VANLEER><<<blocks, MT, 0, st>>>(g, m, s); break;
-        case ALBADA: k_metric_stage<ALBADA><<<blocks, MT, 0, st>>>(g, m, s); break;
-        case MINBEE: k_metric_stage<MINBEE><<<blocks, MT, 0, st>>>(g, m, s); break;
-        case SUPERBEE: k_metric_stage<SUPERBEE><<<blocks, MT, 0, st>>>(g, m, s); break;
-        default: k_metric_stage<UNOS><<<blocks, MT, 0, st>>>(g, m, s); break;
-    }
-    return hipGetLastError();
+    if (g.dim == 1) return launch_stage<1>(g, m, s, blocks, st);
+    if (g.dim == 2) return launch_stage<2>(g, m, s, blocks, st);
+    return launch_stage<3>(g, m, s, blocks, st);
 }
 
 hipError_t metric_cfl(const fvl::Grid& g, const fvl::Metric& m, const void* U, const void* lam, double gamma,
                       int reactive, double* partials, int* flags, hipStream_t st) {
-    k_metric_cfl<<<metric_blocks(g), MT, 0, st>>>(g, m, (const real*)U, (const real*)lam, gamma, reactive,
-                                                  partials, flags);
+    const int blocks = metric_blocks(g);
+    const real* u = (const real*)U;
+    const real* l = (const real*)lam;
+    if (g.dim == 1) k_metric_cfl<1><<<blocks, MT, 0, st>>>(g, m, u, l, gamma, reactive, partials, flags);
+    else if (g.dim == 2) k_metric_cfl<2><<<blocks, MT, 0, st>>>(g, m, u, l, gamma, reactive, partials, flags);
+    else k_metric_cfl<3><<<blocks, MT, 0, st>>>(g, m, u, l, gamma, reactive, partials, flags);
     return hipGetLastError();
 }
 
@@ -538,8 +623,17 @@ hipError_t rusanov_faces(int nv, int dir, long mfaces, const void* l, const void
                          void* out, hipStream_t st) {
     if (mfaces <= 0) return hipSuccess;
     const int blocks = (int)((mfaces + 255) / 256);
-    k_rusanov_faces<<<blocks, 256, 0, st>>>(nv, dir, mfaces, (const real*)l, (const real*)r, (const real*)gl,
-                                            (const real*)gr, (real*)out);
+    const real *a = (const real*)l, *b = (const real*)r, *ga = (const real*)gl, *gb = (const real*)gr;
+    real* o = (real*)out;
+#define RUSANOV_CASE(NF, D) \
+    if (nv == NF && dir == D) k_rusanov_faces<NF, D><<<blocks, 256, 0, st>>>(mfaces, a, b, ga, gb, o);
+    RUSANOV_CASE(3, 0)
+    RUSANOV_CASE(4, 0)
+    RUSANOV_CASE(4, 1)
+    RUSANOV_CASE(5, 0)
+    RUSANOV_CASE(5, 1)
+    RUSANOV_CASE(5, 2)
+#undef RUSANOV_CASE
     return hipGetLastError();
 }
 
