@@ -10,9 +10,9 @@ development container only, and stores small inputs and outputs:
 - rhs (change, species change, outward flux, integrated source) and cfl_dt
   (torchFiniteVolume) for every geometry, limiter and face type, with and
   without transported fractions, second and first order;
-- SSPRK2 ``advance`` (the first-order retry has no natural trigger on
-  these grids: the slope positivity mask keeps the second-order stages
-  physical; the first-order rhs is stored instead);
+- SSPRK2 ``advance``, including a case (a rough 1D state at twice its CFL
+  step) whose second-order stages lose positivity so the first-order retry
+  is taken;
 - multi-step metric sessions (Simulation.step -> _metric_step), inert and
   reactive (Arrhenius, ZND) with walls, inflow, pressure outlets with
   backflow, stagnation inlets and periodic faces.
@@ -197,6 +197,40 @@ def rhs_cases(Simulation, rhs, cfl_dt, advance, data):
     data["rhs_cases"] = np.array(names)
 
 
+def fallback_case(Simulation, rhs, cfl_dt, advance, data):
+    """A rough 1D state advanced at twice its CFL step: the second-order
+    SSPRK2 stages lose positivity, the first-order retry succeeds. The first
+    seeded candidate (rng 3) for which that holds is kept."""
+    from cfdSolver.torchSolver import physical_state
+
+    rng = np.random.default_rng(3)
+    n = 16
+    for _ in range(400):
+        p = np.ones((3, n))
+        p[0] = np.exp(rng.uniform(-3, 1, n))
+        p[1] = rng.uniform(-3, 3, n)
+        p[2] = np.exp(rng.uniform(-6, 1, n))
+        s = Simulation(p, [1.0], t_end=1.0, face_boundaries={"x_min": "outflow", "x_max": "outflow"})
+        c, m = s.config, s.metrics
+        for mult in (2.0, 2.5):
+            dt = mult * cfl_dt(s.state, None, c, m)
+            stage = s.state + dt * rhs(s.state, [], c, m)[0]
+            second_ok = physical_state(stage) and physical_state(
+                0.5 * (s.state + stage + dt * rhs(stage, [], c, m)[0]))
+            if second_ok:
+                continue
+            try:
+                res = advance(s.state, None, None, dt, c, m)
+            except RuntimeError:
+                continue
+            data["fallback_prim"] = p
+            data["fallback_dt"] = np.array(dt)
+            data["fallback_state"] = res[0]
+            data["fallback_boundary"] = res[3]
+            return
+    raise RuntimeError("no first-order retry candidate found")
+
+
 def session_cases(Simulation, data):
     runs = []
     for dim in (1, 2, 3):
@@ -233,6 +267,7 @@ def main():
     metric_cases(Simulation, data)
     ghost_cases(ghost, data)
     rhs_cases(Simulation, rhs, cfl_dt, advance, data)
+    fallback_case(Simulation, rhs, cfl_dt, advance, data)
     session_cases(Simulation, data)
     np.savez_compressed(os.path.join(OUT, "metric.npz"), **data)
     print("wrote", len(data), "arrays")

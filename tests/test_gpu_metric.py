@@ -79,6 +79,17 @@ def test_rhs_cfl_and_advance_golden(d):
         sim.close()
 
 
+def test_first_order_retry_golden(d):
+    from torch_amd import Simulation
+
+    sim = Simulation(d["fallback_prim"], [1.0], t_end=1.0, face_boundaries={"x_min": "outflow", "x_max": "outflow"})
+    info = sim._device.metric_advance(float(d["fallback_dt"]))
+    sim._host = None
+    assert info.fallback == 1
+    close(sim.state, d["fallback_state"], what="fallback state")
+    sums_close(np.array(info.outward[:3]), d["fallback_boundary"], 1e-11, "fallback boundary")
+
+
 def test_sessions_golden(d):
     for tag in d["session_cases"]:
         sim, meta = device_session(d, tag)

@@ -94,6 +94,15 @@ def test_rhs_cfl_and_advance_golden(d):
         close(res[3], d[tag + "_adv_boundary"], rel=1e-11, what=f"{tag} boundary")
 
 
+def test_first_order_retry_golden(d):
+    """advance takes its first-order retry where the second-order stages lose positivity."""
+    s = Session(d["fallback_prim"], [1.0], t_end=1.0, face_boundaries={"x_min": "outflow", "x_max": "outflow"})
+    res = metric.advance(s.state, None, None, float(d["fallback_dt"]), s.config, s.metrics, s.faces)
+    assert res[5], "expected the first-order retry"
+    close(res[0], d["fallback_state"], what="fallback state")
+    close(res[3], d["fallback_boundary"], rel=1e-11, what="fallback boundary")
+
+
 def test_sessions_golden(d):
     for tag in d["session_cases"]:
         s, meta = session_of(d, tag)
