@@ -94,6 +94,12 @@ int torch_state_download(torch_state* state, double* U, double* lam, double* bet
 
 /* Simulation._dt on the committed state (torchSession.py:333-360). */
 int torch_state_dt(torch_state* state, double* dt_limit);
+/* The CFL maxima behind torch_state_dt, reduced on the device: 3D/2D
+ * max(|u_m| + a) per axis; 1D [max |S| of the wave-speed estimate,
+ * max(|u| + a), 0], whose second entry gives hydro_dt_1d = cfl dx / max
+ * (torchSolver.py:1049-1051) for the legacy reactive driver's CFL test
+ * (torchSolver.py:1319-1331). */
+int torch_state_wave_maxima(torch_state* state, double* maxima);
 
 /* Simulation.step (torchSession.py:374-522): CFL dt clipped to t_end - time,
  * reactive R(dt/2) with <= 60 halvings, hydro with fallback, species

@@ -1,0 +1,81 @@
+"""Legacy drivers on the device against the vectors the reference's own
+drivers wrote (tests/golden/legacy.npz): every level of the 1D histories
+(inert and reactive), the 2D/3D drivers' final states, step counts, times
+and balances; and the reference's session-vs-legacy contract
+(testSession.py:123-139)."""
+
+import numpy as np
+import pytest
+
+from conftest import golden
+from test_oracle_legacy import DRIVER_2D, DRIVER_3D, INERT, REACTIVE, check_history, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def d():
+    return golden("legacy")
+
+
+@pytest.mark.parametrize("tag", list(INERT))
+def test_run_simulation_history(d, tag):
+    from torch_amd import run_simulation
+
+    diag = {}
+    u, t = run_simulation(**INERT[tag], diagnostics=diag)
+    check_history(d, tag, u, t, diag=diag)
+
+
+@pytest.mark.parametrize("tag", list(REACTIVE))
+def test_run_simulation_wqr_history(d, tag):
+    from torch_amd import run_simulation_wqr
+
+    diag = {}
+    u, t, lam = run_simulation_wqr(**REACTIVE[tag], lambda_init=d[tag + "_lam0"], diagnostics=diag)
+    check_history(d, tag, u, t, lam, diag)
+
+
+@pytest.mark.parametrize("tag", list(DRIVER_2D))
+def test_run_torch_solver_2d(d, tag):
+    from torch_amd import run_torch_solver_2d
+
+    r = run_torch_solver_2d(**DRIVER_2D[tag])
+    assert r["steps"] == int(d[tag + "_steps"]) and r["time"] == pytest.approx(float(d[tag + "_time"]), rel=1e-14)
+    assert rel_err(r["u_final"], d[tag + "_u_final"]) <= 1e-11
+    assert r["net_boundary_mass"] == pytest.approx(float(d[tag + "_net_boundary_mass"]), rel=1e-10, abs=1e-13)
+    if tag + "_lambda_final" in d:
+        assert rel_err(r["lambda_final"], d[tag + "_lambda_final"]) <= 1e-11
+
+
+@pytest.mark.parametrize("tag", list(DRIVER_3D))
+def test_run_torch_solver_3d(d, tag):
+    from torch_amd import run_torch_solver_3d
+
+    r = run_torch_solver_3d(**DRIVER_3D[tag])
+    assert r["steps"] == int(d[tag + "_steps"]) and r["time"] == pytest.approx(float(d[tag + "_time"]), rel=1e-14)
+    assert rel_err(r["u_final"], d[tag + "_u_final"]) <= 1e-11
+    for key in ("boundary_integral", "source_integral"):
+        ref = d[f"{tag}_{key}"]
+        assert np.max(np.abs(r[key] - ref)) <= 1e-10 * max(np.max(np.abs(ref)), 1.0), (tag, key)
+    if tag + "_lambda_final" in d:
+        assert rel_err(r["lambda_final"], d[tag + "_lambda_final"]) <= 1e-11
+
+
+def test_inert_sessions_match_legacy_drivers():
+    """testSession.py:123-139 on the device."""
+    from torch_amd import Simulation, run_simulation, run_torch_solver_2d, run_torch_solver_3d
+
+    s = Simulation.preset("sod_standard", [40], t_end=0.05)
+    s.run()
+    legacy, t = run_simulation(1, 40, 1, 0, 1, 0.125, 0, 0.1, c_cfl=0.4, t_corte=0.05)
+    np.testing.assert_array_equal(s.state, legacy[:, :, -1])
+    for dim in (2, 3):
+        counts = [8, 6] if dim == 2 else [8, 6, 4]
+        s = Simulation.preset("2d_explosion" if dim == 2 else "3d_explosion", counts, t_end=0.02)
+        s.run()
+        if dim == 2:
+            r = run_torch_solver_2d(nx=8, ny=6, t_end=0.02, cfl=0.4)
+        else:
+            r = run_torch_solver_3d(nx=8, ny=6, nz=4, t_end=0.02, cfl=0.4)
+        np.testing.assert_allclose(s.state, r["u_final"], atol=1e-14)

@@ -33,7 +33,7 @@ EXPORTS = (
     "torch_batch_remove", "torch_batch_seed", "torch_batch_run", "torch_batch_info", "torch_batch_download",
     "torch_batch_tune", "torch_batch_set_tiles", "torch_batch_profile", "torch_batch_time_pass",
     "torch_state_set_metric", "torch_state_metric_dt", "torch_state_metric_rhs", "torch_state_metric_advance",
-    "torch_state_metric_step", "torch_rusanov_faces",
+    "torch_state_metric_step", "torch_rusanov_faces", "torch_state_wave_maxima",
 )
 
 GEOMETRY_CODES = {"cartesian": 0, "cylindrical": 1, "annular": 2, "spherical": 3}
@@ -128,6 +128,7 @@ def load():
                 "torch_state_upload": (ctypes.c_int, [vp, _dp, _dp, _dp]),
                 "torch_state_download": (ctypes.c_int, [vp, _dp, _dp, _dp]),
                 "torch_state_dt": (ctypes.c_int, [vp, _dp]),
+                "torch_state_wave_maxima": (ctypes.c_int, [vp, _dp]),
                 "torch_state_step": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_double,
                                                     ctypes.POINTER(StepInfo)]),
                 "torch_state_hydro": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_int,
@@ -266,6 +267,11 @@ class DeviceState:
         out = ctypes.c_double()
         check(load().torch_state_dt(self.handle, ctypes.byref(out)))
         return out.value
+
+    def wave_maxima(self):
+        out = np.zeros(3)
+        check(load().torch_state_wave_maxima(self.handle, ptr(out)))
+        return out
 
     def step(self, time, t_end):
         info = StepInfo()

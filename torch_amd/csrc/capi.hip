@@ -672,6 +672,18 @@ int torch_state_dt(torch_state* s, double* dt_limit) {
     return TORCH_OK;
 }
 
+int torch_state_wave_maxima(torch_state* s, double* maxima) {
+    if (!s || !maxima) return fail(TORCH_INVALID, "null argument");
+    std::lock_guard<std::mutex> lock(s->ctx->mu);
+    HIP_TRY(hipSetDevice(s->ctx->device));
+    if (!s->maxima_valid) {
+        int rc = compute_maxima(s);
+        if (rc) return rc;
+    }
+    for (int m = 0; m < 3; ++m) maxima[m] = s->maxima[m];
+    return TORCH_OK;
+}
+
 int torch_state_step(torch_state* s, double time, double t_end, torch_step_info* info) {
     if (!s || !info) return fail(TORCH_INVALID, "null argument");
     std::lock_guard<std::mutex> lock(s->ctx->mu);

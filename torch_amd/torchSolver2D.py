@@ -59,3 +59,59 @@ def reaction_step_2d(u_consv, lambda_val, beta_val, dt_chem, kinetic_model="arrh
     finally:
         dev.close()
     return out, new_lam.reshape(lam.shape), None if beta_val is None else new_beta.reshape(lam.shape)
+
+
+def run_torch_solver_2d(preset_name="2d_explosion", rho_init=None, u_init=None, v_init=None, p_init=None, nx=50,
+                        ny=50, lx=1.0, ly=1.0, t_end=0.2, cfl=0.45, gamma=1.4, limiter="VanLeer",
+                        bc_x="Transmissive", bc_y="Transmissive", is_reactive=False, kinetic_model="arrhenius",
+                        max_steps=100000, inlet_primitive=None, ambient_pressure=101325.0, lambda_init=None,
+                        beta_init=None, precision="fp64", device=0):
+    """Legacy 2D driver (torchSolver2D.py:670-946) on a device session."""
+    from .presets import preset_2d
+    from .torchSession import Simulation
+    from .torchSolver3D import _legacy_run
+
+    if not np.isscalar(gamma) or not np.isfinite(gamma) or gamma <= 1:
+        raise ValueError("gamma must be finite and greater than 1")
+    if not np.isfinite(ambient_pressure) or ambient_pressure < 0:
+        raise ValueError("Ambient pressure must be finite and nonnegative")
+    fields = (rho_init, u_init, v_init, p_init)
+    if any(f is not None for f in fields) and any(f is None for f in fields):
+        raise ValueError("Custom initialization requires rho_init, u_init, v_init and p_init")
+    if preset_name is None and rho_init is None:
+        raise ValueError("Provide a preset or all four initial primitive fields")
+    if preset_name is not None and rho_init is None:
+        prim, d = preset_2d(preset_name, nx, ny)
+        gamma = 1.4
+        bc_x, bc_y = d["boundaries"]
+        is_reactive = d["reactive"]
+    else:
+        prim = np.stack([np.broadcast_to(np.asarray(f, dtype=float), (ny, nx)) for f in fields])
+    legacy_names = {"Transmissive": "Open"}
+    sim = Simulation(prim, [lx, ly], t_end=t_end, cfl=cfl, gamma=gamma, limiter=limiter,
+                     boundaries=[legacy_names.get(bc_x, bc_x), legacy_names.get(bc_y, bc_y)], reactive=is_reactive,
+                     kinetic_model=kinetic_model, max_steps=max_steps, inlet_primitive=inlet_primitive,
+                     lambda_init=lambda_init, beta_init=beta_init, precision=precision, device=device)
+    dx, dy = lx / nx, ly / ny
+    try:
+        initial_mass = float(np.sum(sim.state[0]) * dx * dy)
+        time, steps = _legacy_run(sim, t_end, max_steps, {
+            "Integration stalled": "Time integration stalled before the requested final time",
+            "Could not find stable reactive step": "Could not find a CFL-stable reactive step"})
+        state, lam, beta = sim.state, sim.lam, sim.beta
+        boundary_mass = float(sim.boundary_integral[0])
+        g = sim.gamma
+    finally:
+        sim.close()
+    final_mass = float(np.sum(state[0]) * dx * dy)
+    p = state.copy()
+    p[1:3] /= state[0]
+    p[3] = (g - 1) * (state[3] - 0.5 * (state[1] ** 2 + state[2] ** 2) / state[0])
+    x = np.linspace(0.5 * dx, lx - 0.5 * dx, nx)
+    y = np.linspace(0.5 * dy, ly - 0.5 * dy, ny)
+    X, Y = np.meshgrid(x, y)
+    return {"x": x, "y": y, "X": X, "Y": Y, "u_final": state, "u_prim_final": p, "lambda_final": lam,
+            "beta_final": beta, "time": time, "steps": steps,
+            "mass_conservation_error": abs(final_mass - initial_mass + boundary_mass) / max(1e-12, initial_mass),
+            "mass_change_rel": (final_mass - initial_mass) / initial_mass, "net_boundary_mass": boundary_mass,
+            "exit_thrust_per_depth": float(dx * np.sum(p[0, -1] * p[2, -1] ** 2 + p[3, -1] - ambient_pressure))}

@@ -121,3 +121,77 @@ def reaction_step_3d(state, lam, beta, dt, model="arrhenius", execution=None, co
     finally:
         dev.close()
     return out, new_lam.reshape(lam.shape), None if beta is None else new_beta.reshape(lam.shape)
+
+
+def _legacy_run(sim, t_end, max_steps, messages):
+    """Step a device session with the legacy drivers' running time (t += dt);
+    returns (time, steps). The session's dt is min(dt_cfl, t_end - t) and
+    its stall test rejects dt <= 0 or t + dt == t: the 3D driver's rules, and
+    the 2D driver's clamp except where rounding separates the two tests."""
+    time, steps = 0.0, 0
+    while time < t_end and steps < max_steps:
+        sim.time = time
+        try:
+            sim.step()
+        except RuntimeError as exc:
+            for prefix, message in messages.items():
+                if str(exc).startswith(prefix):
+                    raise RuntimeError(message) from exc
+            raise
+        time += sim.last_dt
+        steps += 1
+    if time < t_end:
+        raise RuntimeError(f"Maximum steps reached at t={time}; requested {t_end}")
+    return time, steps
+
+
+def run_torch_solver_3d(preset_name="3d_explosion", *, primitive_init=None, nx=20, ny=20, nz=20, lx=1.0, ly=1.0,
+                        lz=1.0, t_end=None, cfl=0.4, gamma=1.4, limiter="VanLeer", bc_x="Open", bc_y="Open",
+                        bc_z="Open", is_reactive=False, kinetic_model="arrhenius", lambda_init=None, beta_init=None,
+                        max_steps=100000, precision="fp64", device=0):
+    """Legacy 3D driver (torchSolver3D.py:289-471) on a device session:
+    final state, coordinates and the five-equation balances."""
+    from .torchSession import Simulation
+
+    if t_end is None:
+        t_end = 1e-5 if preset_name == "3d_ignition" and primitive_init is None else 0.05
+    if not np.isscalar(gamma) or not np.isfinite(gamma) or gamma <= 1:
+        raise ValueError("gamma must be finite and greater than one")
+    if limiter not in LIMITERS or kinetic_model not in ("arrhenius", "znd"):
+        raise ValueError("Invalid 3D limiter or kinetic model")
+    if not isinstance(max_steps, (int, np.integer)) or max_steps < 1:
+        raise ValueError("max_steps must be a positive integer")
+    if primitive_init is None:
+        cfg = get_3d_preset(preset_name, nx, ny, nz)
+        primitive_init, is_reactive = cfg["primitive"], cfg["is_reactive"]
+    primitive = np.array(primitive_init, dtype=float, copy=True)
+    if primitive.shape != (5, nz, ny, nx) or not np.isfinite(primitive).all() or np.any(primitive[[0, 4]] <= 0):
+        raise ValueError("Require finite (5,nz,ny,nx) primitives with positive density/pressure")
+    if not is_reactive and (lambda_init is not None or beta_init is not None):
+        raise ValueError("Composition requires is_reactive=True")
+    if beta_init is not None and kinetic_model != "znd":
+        raise ValueError("beta_init requires reactive ZND kinetics")
+    sim = Simulation(primitive, [lx, ly, lz], t_end=t_end, cfl=cfl, gamma=gamma, limiter=limiter,
+                     boundaries=[bc_x, bc_y, bc_z], reactive=is_reactive, kinetic_model=kinetic_model,
+                     lambda_init=lambda_init, beta_init=beta_init, max_steps=max_steps, precision=precision,
+                     device=device)
+    try:
+        initial = sim.integral.copy()
+        time, steps = _legacy_run(sim, t_end, max_steps, {
+            "Integration stalled": "3D integration stalled before final time",
+            "Could not find stable reactive step": "Could not find a stable 3D reactive step"})
+        state, lam, beta = sim.state, sim.lam, sim.beta
+        final = sim.integral
+        boundary, source = sim.boundary_integral.copy(), sim.source_integral.copy()
+    finally:
+        sim.close()
+    g = (4.5 - lam) / (3.5 - lam) if is_reactive else gamma
+    spacing = (lx / nx, ly / ny, lz / nz)
+    x, y, z = [(np.arange(n) + 0.5) * h for n, h in zip((nx, ny, nz), spacing)]
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    residual = final - initial + boundary - source
+    return dict(u_final=state, u_prim_final=uconsv_to_uprim_3d(state, g), lambda_final=lam, beta_final=beta,
+                time=time, steps=steps, x=x, y=y, z=z, X=X, Y=Y, Z=Z, initial_integral=initial,
+                final_integral=final, boundary_integral=boundary, source_integral=source,
+                balance_residual=residual, mass_conservation_error=abs(residual[0]) / initial[0],
+                mass_change_rel=(final[0] - initial[0]) / initial[0])
