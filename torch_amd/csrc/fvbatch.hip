@@ -50,17 +50,27 @@ __device__ __forceinline__ double py_min(double a, double b) { return b < a ? b 
 // finalize: 256 strided partial results, then a tree (maxima for slots
 // 0-2, sums for the rest). Every thread of the block must call it; thread 0
 // reads the results from red[slot][0].
+//
+// Every slot of a partial row is folded in the same pass (one contiguous
+// row per block instead of one strided column per slot), and the block loop
+// is unrolled so the loads of several rows are in flight together; each
+// slot still accumulates its rows in increasing order, as finalize does.
 __device__ void fold(const double* part, int nb, int nslot, double (*red)[256]) {
     const int t = threadIdx.x;
-    for (int slot = 0; slot < nslot; ++slot) {
-        const bool is_max = slot < 3;
-        double v = 0.0;
-        for (int b = t; b < nb; b += 256) {
-            double p = part[(long)b * fvl::NSLOT + slot];
-            v = is_max ? fmax(v, p) : v + p;
+    double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll 4
+    for (int b = t; b < nb; b += 256) {
+        const double* row = part + (long)b * fvl::NSLOT;
+#pragma unroll
+        for (int slot = 0; slot < 9; ++slot) {
+            if (slot >= nslot) break;
+            const double p = row[slot];
+            v[slot] = slot < 3 ? fmax(v[slot], p) : v[slot] + p;
         }
-        red[slot][t] = v;
     }
+#pragma unroll
+    for (int slot = 0; slot < 9; ++slot)
+        if (slot < nslot) red[slot][t] = v[slot];
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
         if (t < s)
