@@ -190,6 +190,62 @@ int torch_rusanov_faces(torch_ctx* ctx, int precision, int nv, int direction, lo
                         const double* right, const double* gl, const double* gr, double* out);
 
 /* ---------------------------------------------------------------------------
+ * Adaptive leaves: the reference's dyadic AMR family (AdaptiveSimulation,
+ * torchAdaptive.py:14-206 and 400-481; LeafMesh, torchAdaptiveGeometry.py:
+ * 125-372). The leaf tree, its face list and remeshing stay host data
+ * structures (torch_amd/torchAdaptiveGeometry.py); the device holds one leaf
+ * mesh at a time as flat arrays and runs the shared-face Rusanov SSPRK2
+ * update, the CFL rate and the split chemistry over it. Arrays crossing the
+ * boundary are host float64 (dim+2, nleaf) states and nleaf fractions.
+ * ------------------------------------------------------------------------- */
+typedef struct torch_amr torch_amr;
+/* A leaf solver with the numeric options of config (dim, gamma, cfl,
+ * reactive, kinetic model and its constants, precision; counts and
+ * boundaries are unused), the geometry, the face boundaries (x_min, x_max,
+ * y_min, ...) and the reconstruction order (1 or 2) of
+ * AdaptiveSimulation.config (torchAdaptive.py:210-217, torchAdaptiveGeometry.py:12-44). */
+int torch_amr_create(torch_ctx* ctx, const torch_config* config, int geometry, const torch_face* faces, int order,
+                     torch_amr** out);
+int torch_amr_destroy(torch_amr* amr);
+/* Installs a leaf mesh (the LeafMesh attributes of torchAdaptiveGeometry.py:
+ * 170-248): per face its left and right leaf (-1 outside the domain),
+ * direction, side (-1 low or +1 high domain boundary, 0 interior or periodic
+ * seam), area, gradient weight, and (nface, dim) offsets of the face centre
+ * from the left and right leaf centres and the gradient displacement; per
+ * leaf its volume, area divergence (dim, nleaf) and inverse normal matrix
+ * (nleaf, dim, dim); the incidence lists in the order the reference
+ * accumulates (nleaf + 1 starts, then face and role, -1 left / +1 right, of
+ * each of the nentry entries). The stored state is undefined until the next
+ * upload. */
+int torch_amr_set_mesh(torch_amr* amr, long nleaf, long nface, const int* left, const int* right,
+                       const int* direction, const int* side, const double* area, const double* weight,
+                       const double* offset_left, const double* offset_right, const double* displacement,
+                       const double* volume, const double* divergence, const double* inverse, long nentry,
+                       const long* start, const int* entry_face, const int* entry_role);
+/* State (dim+2, nleaf) and, reactive, lambda and beta (NULL beta: ones). */
+int torch_amr_upload(torch_amr* amr, const double* U, const double* lam, const double* beta);
+int torch_amr_download(torch_amr* amr, double* U, double* lam, double* beta);
+/* face_values without reconstruction (torchAdaptive.py:14-57) of the
+ * committed state, as remesh reads them: left/right (dim+2, nface)
+ * primitives, gl/gr the nface gammas; NULL skips an output. */
+int torch_amr_faces(torch_amr* amr, double* left, double* right, double* gl, double* gr);
+/* cfl (torchAdaptive.py:149-166) of the committed state. */
+int torch_amr_dt(torch_amr* amr, double* dt);
+/* rhs (torchAdaptive.py:106-146) of the committed state: change (dim+2,
+ * nleaf), species change (one row per fraction), boundary and integrated
+ * geometric source (dim+2 each); NULL skips an output. */
+int torch_amr_rhs(torch_amr* amr, int first_order, double* change, double* species, double* outward,
+                  double* source);
+/* advance (torchAdaptive.py:169-206) of the committed state by dt,
+ * committing the result: outward = boundary, source = geometric source,
+ * fallback = the first-order retry was taken. */
+int torch_amr_advance(torch_amr* amr, double dt, torch_step_info* info);
+/* AdaptiveSimulation.step up to its regrid (torchAdaptive.py:436-481): cfl
+ * clipped to t_end - time, reactive R(dt/2) with the CFL test, advance,
+ * R(dt/2), up to 24 halvings; commits only on success. */
+int torch_amr_step(torch_amr* amr, double time, double t_end, torch_step_info* info);
+
+/* ---------------------------------------------------------------------------
  * Batches. No counterpart in the reference, where every Simulation steps on
  * its own (torchSession.py:374-522) and a batch of problems is a Python loop
  * over sessions. Here many instances (inert of any dimension, reactive 3D;

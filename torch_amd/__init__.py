@@ -16,6 +16,7 @@ include/torch_hip.h; there is no CPU fallback.
 from .torchExecution import DeviceExecution, Execution, validate_workers  # noqa: F401
 from .torchFlux import hllc_flux, hllc_flux_grid, rusanov_flux, rusanov_flux_grid  # noqa: F401
 from .torchIO import read_checkpoint, write_checkpoint  # noqa: F401
+from .torchAdaptive import AdaptiveSimulation  # noqa: F401
 from .torchSession import Simulation  # noqa: F401
 from .torchSolver import run_simulation, run_simulation_wqr, run_simulation_wQR  # noqa: F401
 from .torchSolver2D import muscl_hancock_step_2d, reaction_step_2d, run_torch_solver_2d  # noqa: F401

@@ -34,6 +34,8 @@ EXPORTS = (
     "torch_batch_tune", "torch_batch_set_tiles", "torch_batch_profile", "torch_batch_time_pass",
     "torch_state_set_metric", "torch_state_metric_dt", "torch_state_metric_rhs", "torch_state_metric_advance",
     "torch_state_metric_step", "torch_rusanov_faces", "torch_state_wave_maxima",
+    "torch_amr_create", "torch_amr_destroy", "torch_amr_set_mesh", "torch_amr_upload", "torch_amr_download",
+    "torch_amr_faces", "torch_amr_dt", "torch_amr_rhs", "torch_amr_advance", "torch_amr_step",
 )
 
 GEOMETRY_CODES = {"cartesian": 0, "cylindrical": 1, "annular": 2, "spherical": 3}
@@ -103,6 +105,8 @@ class BatchProgress(ctypes.Structure):
 _lib = None
 _lock = threading.Lock()
 _dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_lp = ctypes.POINTER(ctypes.c_long)
 
 
 def library_path():
@@ -165,6 +169,20 @@ def load():
                                                            ctypes.POINTER(StepInfo)]),
                 "torch_rusanov_faces": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                                        ctypes.c_long, _dp, _dp, _dp, _dp, _dp]),
+                "torch_amr_create": (ctypes.c_int, [vp, ctypes.POINTER(Config), ctypes.c_int, ctypes.POINTER(Face),
+                                                    ctypes.c_int, ctypes.POINTER(vp)]),
+                "torch_amr_destroy": (ctypes.c_int, [vp]),
+                "torch_amr_set_mesh": (ctypes.c_int, [vp, ctypes.c_long, ctypes.c_long, _ip, _ip, _ip, _ip,
+                                                      _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_long,
+                                                      _lp, _ip, _ip]),
+                "torch_amr_upload": (ctypes.c_int, [vp, _dp, _dp, _dp]),
+                "torch_amr_download": (ctypes.c_int, [vp, _dp, _dp, _dp]),
+                "torch_amr_faces": (ctypes.c_int, [vp, _dp, _dp, _dp, _dp]),
+                "torch_amr_dt": (ctypes.c_int, [vp, _dp]),
+                "torch_amr_rhs": (ctypes.c_int, [vp, ctypes.c_int, _dp, _dp, _dp, _dp]),
+                "torch_amr_advance": (ctypes.c_int, [vp, ctypes.c_double, ctypes.POINTER(StepInfo)]),
+                "torch_amr_step": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_double,
+                                                  ctypes.POINTER(StepInfo)]),
             }
             for name, (res, args) in sig.items():
                 fn = getattr(lib, name)
@@ -339,6 +357,105 @@ class DeviceState:
     def close(self):
         if getattr(self, "handle", None):
             load().torch_state_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _typed(array, dtype, ctype):
+    array = np.ascontiguousarray(array, dtype=dtype)
+    return array, array.ctypes.data_as(ctypes.POINTER(ctype))
+
+
+class DeviceLeaves:
+    """Device-resident adaptive leaves (torch_amr_*): one leaf mesh at a time
+    with its state, lambda and beta."""
+
+    def __init__(self, context, config, geometry, faces, order):
+        self.context = context
+        self.config = config
+        self.dim = config.dim
+        self.nf = config.dim + 2
+        self.nq = (2 if config.kinetic_model == 1 else 1) if config.reactive else 0
+        handle = ctypes.c_void_p()
+        check(load().torch_amr_create(context.handle, ctypes.byref(config), GEOMETRY_CODES[geometry],
+                                      faces_of(faces), int(order), ctypes.byref(handle)))
+        self.handle = handle
+        self.nleaf = 0
+        self.nface = 0
+
+    def set_mesh(self, mesh):
+        """Install a torchAdaptiveGeometry.LeafMesh (its face list, metrics,
+        gradient matrices and incidence lists)."""
+        start, faces, roles = mesh.incidence()
+        keep = []
+        ints = [_typed(a, np.int32, ctypes.c_int) for a in (mesh.left, mesh.right, mesh.directions, mesh.sides)]
+        reals = [_typed(a, np.float64, ctypes.c_double) for a in (
+            mesh.face_areas, mesh.weights, mesh.offset_left, mesh.offset_right, mesh.displacement,
+            mesh.volumes, mesh.divergence, mesh.inverse)]
+        starts = _typed(start, np.int64, ctypes.c_long)
+        entries = [_typed(a, np.int32, ctypes.c_int) for a in (faces, roles)]
+        keep.extend(ints + reals + [starts] + entries)
+        nleaf, nface = len(mesh.volumes), len(mesh.left)
+        check(load().torch_amr_set_mesh(self.handle, nleaf, nface, *(p for _, p in ints), *(p for _, p in reals),
+                                        len(faces), starts[1], *(p for _, p in entries)))
+        self.nleaf, self.nface = nleaf, nface
+
+    def upload(self, state, lam=None, beta=None):
+        state = np.ascontiguousarray(state, dtype=np.float64)
+        if state.shape != (self.nf, self.nleaf):
+            raise ValueError(f"leaf state must have shape {(self.nf, self.nleaf)}")
+        lam = None if lam is None else np.ascontiguousarray(lam, dtype=np.float64)
+        beta = None if beta is None else np.ascontiguousarray(beta, dtype=np.float64)
+        for q in (lam, beta):
+            if q is not None and q.shape != (self.nleaf,):
+                raise ValueError("leaf fractions must have one value per leaf")
+        check(load().torch_amr_upload(self.handle, ptr(state), ptr(lam), ptr(beta)))
+
+    def download(self):
+        state = np.empty((self.nf, self.nleaf))
+        lam = np.empty(self.nleaf) if self.nq > 0 else None
+        beta = np.empty(self.nleaf) if self.nq > 1 else None
+        check(load().torch_amr_download(self.handle, ptr(state), ptr(lam), ptr(beta)))
+        return state, lam, beta
+
+    def faces(self):
+        """Raw face values (left, right, gl, gr) of the committed state."""
+        left, right = np.empty((self.nf, self.nface)), np.empty((self.nf, self.nface))
+        gl, gr = np.empty(self.nface), np.empty(self.nface)
+        check(load().torch_amr_faces(self.handle, ptr(left), ptr(right), ptr(gl), ptr(gr)))
+        return left, right, gl, gr
+
+    def dt(self):
+        out = ctypes.c_double()
+        check(load().torch_amr_dt(self.handle, ctypes.byref(out)))
+        return out.value
+
+    def rhs(self, first_order=False):
+        change = np.empty((self.nf, self.nleaf))
+        species = np.empty((max(self.nq, 1), self.nleaf))
+        outward, source = np.empty(self.nf), np.empty(self.nf)
+        check(load().torch_amr_rhs(self.handle, int(bool(first_order)), ptr(change),
+                                   ptr(species) if self.nq else None, ptr(outward), ptr(source)))
+        return change, [species[j] for j in range(self.nq)], outward, source
+
+    def advance(self, dt):
+        info = StepInfo()
+        check(load().torch_amr_advance(self.handle, float(dt), ctypes.byref(info)))
+        return info
+
+    def step(self, time, t_end):
+        info = StepInfo()
+        check(load().torch_amr_step(self.handle, float(time), float(t_end), ctypes.byref(info)))
+        return info
+
+    def close(self):
+        if getattr(self, "handle", None):
+            load().torch_amr_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

@@ -23,7 +23,7 @@ LIB = os.path.join(HERE, "libtorchhip.so")
 ARCH = "gfx950"
 
 KERNELS = ("fv3d.hip", "fv2d.hip", "fv1d.hip", "fvbatch.hip", "fvmetric.hip")
-HOST = ("capi.hip", "batch.hip")
+HOST = ("capi.hip", "batch.hip", "adapt.hip")
 COMMON = [
     "--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

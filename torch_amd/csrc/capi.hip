@@ -112,7 +112,8 @@ int download_as(const Ops* ops, hipStream_t st, double* dst, const void* src, lo
         NS::batch3d,      NS::batch_llf3d,   NS::batch_cfl3d, NS::batch2d,      NS::batch_llf2d,          \
         NS::batch_cfl2d,  NS::batch1d,       NS::batch_llf1d, NS::batch_cfl1d,  NS::batch_post,           \
         NS::batch_react,  NS::batch_check,   NS::strip2d_rows, NS::step1d_tiles,                       \
-        NS::metric_stage, NS::metric_cfl,    NS::metric_source, NS::rusanov_faces, NS::metric_blocks}
+        NS::metric_stage, NS::metric_cfl,    NS::metric_source, NS::rusanov_faces, NS::metric_blocks,      \
+        NS::amr_eval,     NS::amr_update,    NS::amr_cfl,       NS::amr_source,   NS::amr_blocks}
 
 // Not const: a const table would be promoted to device memory, where the
 // host launchers it points to do not exist.

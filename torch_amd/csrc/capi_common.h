@@ -79,6 +79,14 @@ struct Ops {
     hipError_t (*rusanov_faces)(int, int, long, const void*, const void*, const void*, const void*, void*,
                                 hipStream_t);
     int (*metric_blocks)(const fvl::Grid&);
+    // adaptive leaves
+    hipError_t (*amr_eval)(int, const fvl::Amr&, const void*, const void*, const void*, int, double, int, int, int*,
+                           double*, hipStream_t);
+    hipError_t (*amr_update)(int, const fvl::Amr&, const fvl::MStage&, hipStream_t);
+    hipError_t (*amr_cfl)(int, const fvl::Amr&, double*, hipStream_t);
+    hipError_t (*amr_source)(int, const fvl::Amr&, const void*, const void*, const void*, const void*, double*,
+                             hipStream_t);
+    int (*amr_blocks)(long);
 };
 
 extern Ops kOps64;

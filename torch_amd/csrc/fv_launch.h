@@ -81,6 +81,42 @@ struct MStage {
     double* partials;         // NSLOT per block: S_OUT.. outward, S_MSRC.. integrated source
 };
 enum { S_MSRC = 8 };          // 5 integrated geometric-source sums (metric stages)
+// Adaptive (dyadic AMR) leaves as flat arrays (torchAdaptiveGeometry.LeafMesh):
+// faces with their left/right leaves (-1 outside the domain), direction,
+// boundary side, area and centre offsets; leaves with volume, area
+// divergence (dim x nleaf) and least-squares inverse matrix (nleaf x dim x
+// dim); per-leaf face incidence as CSR in the order the reference
+// accumulates (faces where the leaf is left, then where it is right; role
+// -1 / +1). Scratch arrays hold one evaluation's intermediate values.
+struct Amr {
+    long nleaf, nface;
+    const int* left;
+    const int* right;
+    const int* dir;
+    const int* side;
+    const double* area;
+    const double* offl;    // nface x dim
+    const double* offr;    // nface x dim
+    const double* disp;    // nface x dim
+    const double* weight;  // nface
+    const double* volume;  // nleaf
+    const double* div;     // dim x nleaf
+    const double* inverse; // nleaf x dim x dim
+    const long* start;     // nleaf + 1
+    const int* cface;
+    const int* crole;
+    MFace face[6];
+    int geometry;
+    void* FL;    // nf x nface face values (left side)
+    void* FR;    // nf x nface (right side)
+    void* G;     // 2 x nface: gamma left, right
+    void* Q;     // 4 x nface: fraction 0 left/right, fraction 1 left/right
+    void* grad;  // nleaf x dim x nf
+    void* W;     // nf x nface area-weighted flux
+    void* M;     // 2 x nface transported fractions
+    void* S;     // nface area-weighted face speed (CFL)
+};
+
 
 struct Step {
     const void* U;     // input state, (dim+2) fields of ncell values
@@ -324,6 +360,16 @@ struct Launch {
     hipError_t rusanov_faces(int nv, int dir, long m, const void* l, const void* r,        \
                              const void* gl, const void* gr, void* out, hipStream_t st);   \
     int metric_blocks(const fvl::Grid& g);                                                \
+    /* adaptive leaves (fvmetric.hip) */                                                  \
+    hipError_t amr_eval(int dim, const fvl::Amr& a, const void* S, const void* q0,        \
+                        const void* q1, int nq, double gamma, int reactive,               \
+                        int reconstruct, int* flags, double* partials, hipStream_t st);   \
+    hipError_t amr_update(int dim, const fvl::Amr& a, const fvl::MStage& s, hipStream_t st); \
+    hipError_t amr_cfl(int dim, const fvl::Amr& a, double* partials, hipStream_t st);      \
+    hipError_t amr_source(int dim, const fvl::Amr& a, const void* start, const void* old,  \
+                          const void* state, const void* hydro, double* partials,          \
+                          hipStream_t st);                                                 \
+    int amr_blocks(long n);                                                               \
     }
 
 FVL_DECLARE(fv64)

@@ -412,6 +412,50 @@ __device__ __forceinline__ void direction(const Ctx& c, const fvl::Metric& m, co
     }
 }
 
+// Stage output of cell i from its change k and species change kq (SSPRK2,
+// torchFiniteVolume.py:232-276 and torchAdaptive.py:169-206): mode 0 stores
+// the change, mode 1 S + dt k, mode 2 ((U0 + S) + dt k) / 2, fractions from
+// the held partial densities; flags nonphysical states and fractions
+// outside [0, 1] and stores the clipped fractions.
+template <int NF>
+__device__ __forceinline__ void stage_out(const fvl::MStage& s, long N, long i, const real* k, const real* kq,
+                                          int& bad) {
+    real out[NF], qo[2] = {0, 0};
+    const real* S = (const real*)s.S;
+    if (s.mode == 0) {
+#pragma unroll
+        for (int f = 0; f < NF; ++f) out[f] = k[f];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) qo[j] = kq[j];
+    } else {
+        const real* U0 = (const real*)s.U0;
+        const real dt = (real)s.dt;
+#pragma unroll
+        for (int f = 0; f < NF; ++f)
+            out[f] = s.mode == 1 ? S[f * N + i] + dt * k[f]
+                                 : (real)0.5 * ((U0[f * N + i] + S[f * N + i]) + dt * k[f]);
+        if (!physical_n<NF>(out)) bad |= fvl::F_M_UNPHYS;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            if (j >= s.nq) break;
+            const real held = U0[i] * ((const real*)s.q0[j])[i];
+            real q;
+            if (s.mode == 1) {
+                q = (held + dt * kq[j]) / out[0];
+            } else {
+                q = ((real)0.5 * ((held + S[i] * ((const real*)s.qS[j])[i]) + dt * kq[j])) / out[0];
+            }
+            if (q < (real)-1e-12 || q > (real)1 + (real)1e-12) bad |= fvl::F_M_SPEC;
+            qo[j] = rmin(rmax(q, (real)0), (real)1);
+        }
+    }
+#pragma unroll
+    for (int f = 0; f < NF; ++f) ((real*)s.out)[f * N + i] = out[f];
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+        if (j < s.nq) ((real*)s.qout[j])[i] = qo[j];
+}
+
 template <int DIM, int LIM>
 __global__ void __launch_bounds__(MT, 2) k_metric_stage(fvl::Grid g, fvl::Metric m, fvl::MStage s) {
     constexpr int NF = DIM + 2;
@@ -446,40 +490,7 @@ __global__ void __launch_bounds__(MT, 2) k_metric_stage(fvl::Grid g, fvl::Metric
         direction<DIM, 0, LIM>(c, m, g, i, idx, s.first_order, Vr, k, kq, sums, bad);
         if (DIM >= 2) direction<DIM, (DIM >= 2 ? 1 : 0), LIM>(c, m, g, i, idx, s.first_order, Vr, k, kq, sums, bad);
         if (DIM >= 3) direction<DIM, (DIM >= 3 ? 2 : 0), LIM>(c, m, g, i, idx, s.first_order, Vr, k, kq, sums, bad);
-        real out[NF], qo[2] = {0, 0};
-        const real* S = (const real*)s.S;
-        if (s.mode == 0) {
-#pragma unroll
-            for (int f = 0; f < NF; ++f) out[f] = k[f];
-#pragma unroll
-            for (int j = 0; j < 2; ++j) qo[j] = kq[j];
-        } else {
-            const real* U0 = (const real*)s.U0;
-            const real dt = (real)s.dt;
-#pragma unroll
-            for (int f = 0; f < NF; ++f)
-                out[f] = s.mode == 1 ? S[f * N + i] + dt * k[f]
-                                     : (real)0.5 * ((U0[f * N + i] + S[f * N + i]) + dt * k[f]);
-            if (!physical_n<NF>(out)) bad |= fvl::F_M_UNPHYS;
-#pragma unroll
-            for (int j = 0; j < 2; ++j) {
-                if (j >= s.nq) break;
-                const real held = U0[i] * ((const real*)s.q0[j])[i];
-                real q;
-                if (s.mode == 1) {
-                    q = (held + dt * kq[j]) / out[0];
-                } else {
-                    q = ((real)0.5 * ((held + S[i] * ((const real*)s.qS[j])[i]) + dt * kq[j])) / out[0];
-                }
-                if (q < (real)-1e-12 || q > (real)1 + (real)1e-12) bad |= fvl::F_M_SPEC;
-                qo[j] = rmin(rmax(q, (real)0), (real)1);
-            }
-        }
-#pragma unroll
-        for (int f = 0; f < NF; ++f) ((real*)s.out)[f * N + i] = out[f];
-#pragma unroll
-        for (int j = 0; j < 2; ++j)
-            if (j < s.nq) ((real*)s.qout[j])[i] = qo[j];
+        stage_out<NF>(s, N, i, k, kq, bad);
     }
     if (bad) atomicOr(s.flags, bad);
 #pragma unroll
@@ -574,6 +585,344 @@ __global__ void k_rusanov_faces(long mfaces, const real* l, const real* r, const
     for (int k = 0; k < NF; ++k) out[k * mfaces + f] = F[k];
 }
 
+// -- adaptive leaves ------------------------------------------------------------
+// Dyadic leaves with an explicit face list (torchAdaptive.py:14-166). Face
+// values and fluxes take one thread per face; gradients, updates and CFL
+// rates one thread per leaf, which gathers its faces through the incidence
+// list in the order the reference scatters them with np.add.at (faces where
+// the leaf is left, then where it is right), so each sum is formed in the
+// reference's order and no atomics are needed.
+
+__device__ __forceinline__ real nmin(real a, real b) { return (a < b || a != a) ? a : b; }
+
+// Boundary-plane state of a boundary face along runtime direction D.
+template <int NF, int DIM>
+__device__ __forceinline__ void ghost_along(Pad<NF>& e, const MFace& f, int side, int D, int nq, int reactive,
+                                            int& bad) {
+    if (D == 0) ghost<NF, 0>(e, f, side, nq, reactive, bad);
+    else if (DIM >= 2 && D == 1) ghost<NF, (DIM >= 2 ? 1 : 0)>(e, f, side, nq, reactive, bad);
+    else if (DIM >= 3) ghost<NF, (DIM >= 3 ? 2 : 0)>(e, f, side, nq, reactive, bad);
+}
+
+template <int NF, int DIM>
+__device__ __forceinline__ void rusanov_along(const real* l, const real* r, real gl, real gr, int D, real* F) {
+    if (D == 0) rusanov<NF, 0>(l, r, gl, gr, F);
+    else if (DIM >= 2 && D == 1) rusanov<NF, (DIM >= 2 ? 1 : 0)>(l, r, gl, gr, F);
+    else rusanov<NF, (DIM >= 3 ? 2 : 0)>(l, r, gl, gr, F);
+}
+
+// Raw face values (face_values without reconstruction, torchAdaptive.py:14-57):
+// the primitives, gamma and fractions of the leaves either side, the
+// boundary plane's state on the outer side of a boundary face.
+template <int DIM>
+__global__ void __launch_bounds__(MT) k_amr_faces(fvl::Amr a, Ctx c, int* flags) {
+    constexpr int NF = DIM + 2;
+    const long F = a.nface;
+    const long f = blockIdx.x * (long)MT + threadIdx.x;
+    int bad = 0;
+    if (f < F) {
+        const int side = a.side[f], D = a.dir[f];
+        Pad<NF> L, R;
+        if (side == -1) {
+            R = load_cell<NF>(c, a.right[f]);
+            L = R;
+            ghost_along<NF, DIM>(L, a.face[2 * D], -1, D, c.nq, c.reactive, bad);
+        } else if (side == 1) {
+            L = load_cell<NF>(c, a.left[f]);
+            R = L;
+            ghost_along<NF, DIM>(R, a.face[2 * D + 1], 1, D, c.nq, c.reactive, bad);
+        } else {
+            L = load_cell<NF>(c, a.left[f]);
+            R = load_cell<NF>(c, a.right[f]);
+        }
+        real* FL = (real*)a.FL;
+        real* FR = (real*)a.FR;
+#pragma unroll
+        for (int v = 0; v < NF; ++v) {
+            FL[v * F + f] = L.p[v];
+            FR[v * F + f] = R.p[v];
+        }
+        real* G = (real*)a.G;
+        G[f] = L.g;
+        G[F + f] = R.g;
+        real* Q = (real*)a.Q;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            if (j >= c.nq) break;
+            Q[(2 * j) * F + f] = L.q[j];
+            Q[(2 * j + 1) * F + f] = R.q[j];
+        }
+    }
+    if (bad) atomicOr(flags, bad);
+}
+
+// Bounded least-squares gradients of the primitives of one leaf
+// (torchAdaptiveGeometry.py:334-372): the weighted displacement-difference
+// vector through the inverse normal matrix, scaled per variable by the
+// largest theta in [0, 1] that keeps every face value within the range of
+// the leaf and its face neighbours.
+template <int DIM>
+__global__ void __launch_bounds__(MT) k_amr_gradient(fvl::Amr a, Ctx c) {
+    constexpr int NF = DIM + 2;
+    const long F = a.nface;
+    const long n = blockIdx.x * (long)MT + threadIdx.x;
+    if (n >= a.nleaf) return;
+    const Pad<NF> me = load_cell<NF>(c, n);
+    const real* FL = (const real*)a.FL;
+    const real* FR = (const real*)a.FR;
+    real vec[DIM][NF], lo[NF], hi[NF];
+#pragma unroll
+    for (int v = 0; v < NF; ++v) {
+        lo[v] = hi[v] = me.p[v];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) vec[d][v] = 0;
+    }
+    const long e0 = a.start[n], e1 = a.start[n + 1];
+    for (long e = e0; e < e1; ++e) {
+        const long f = a.cface[e];
+        const bool is_left = a.crole[e] < 0;
+        const real w = (real)a.weight[f];
+        real diff[NF];
+#pragma unroll
+        for (int v = 0; v < NF; ++v) {
+            const real l = FL[v * F + f], r = FR[v * F + f];
+            diff[v] = r - l;
+            const real other = is_left ? r : l;
+            lo[v] = nmin(lo[v], other);
+            hi[v] = nmax(hi[v], other);
+        }
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            const real wd = w * (real)a.disp[f * DIM + d];
+#pragma unroll
+            for (int v = 0; v < NF; ++v) vec[d][v] = vec[d][v] + wd * diff[v];
+        }
+    }
+    real grad[DIM][NF];
+    const double* inv = a.inverse + n * DIM * DIM;
+#pragma unroll
+    for (int i = 0; i < DIM; ++i)
+#pragma unroll
+        for (int v = 0; v < NF; ++v) {
+            real acc = (real)inv[i * DIM] * vec[0][v];
+#pragma unroll
+            for (int j = 1; j < DIM; ++j) acc = acc + (real)inv[i * DIM + j] * vec[j][v];
+            grad[i][v] = acc;
+        }
+    real theta[NF];
+#pragma unroll
+    for (int v = 0; v < NF; ++v) theta[v] = 1;
+    for (long e = e0; e < e1; ++e) {
+        const long f = a.cface[e];
+        const double* off = (a.crole[e] < 0 ? a.offl : a.offr) + f * DIM;
+#pragma unroll
+        for (int v = 0; v < NF; ++v) {
+            real change = (real)off[0] * grad[0][v];
+#pragma unroll
+            for (int d = 1; d < DIM; ++d) change = change + (real)off[d] * grad[d][v];
+            const real allowed = change >= (real)0 ? hi[v] - me.p[v] : lo[v] - me.p[v];
+            real ratio = fabs(change) > (real)1e-14 ? allowed / change : (real)1;
+            ratio = ratio < (real)0 ? (real)0 : (ratio > (real)1 ? (real)1 : ratio);
+            theta[v] = nmin(theta[v], ratio);
+        }
+    }
+    real* out = (real*)a.grad + n * DIM * NF;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d)
+#pragma unroll
+        for (int v = 0; v < NF; ++v) out[d * NF + v] = grad[d][v] * theta[v];
+}
+
+// Face fluxes (torchAdaptive.py:59-77, 106-146, 149-166): reconstructed
+// face values (when second order), the wall mirror after reconstruction,
+// the Rusanov flux weighted by the face area, the fractions it moves, the
+// area-weighted wave speed for the CFL rate, and the boundary faces'
+// outward sums.
+template <int DIM>
+__global__ void __launch_bounds__(MT) k_amr_flux(fvl::Amr a, int nq, int reconstruct, double* partials) {
+    constexpr int NF = DIM + 2, E = NF - 1;
+    const long F = a.nface;
+    const long f = blockIdx.x * (long)MT + threadIdx.x;
+    double out[NF];
+#pragma unroll
+    for (int v = 0; v < NF; ++v) out[v] = 0;
+    if (f < F) {
+        const int D = a.dir[f], side = a.side[f], li = a.left[f], ri = a.right[f];
+        const real* FL = (const real*)a.FL;
+        const real* FR = (const real*)a.FR;
+        const real* G = (const real*)a.G;
+        real l[NF], r[NF];
+#pragma unroll
+        for (int v = 0; v < NF; ++v) {
+            l[v] = FL[v * F + f];
+            r[v] = FR[v * F + f];
+        }
+        real gl = G[f], gr = G[F + f];
+        if (reconstruct) {
+            const real* grad = (const real*)a.grad;
+            if (li >= 0) {
+                const real* g = grad + (long)li * DIM * NF;
+                const double* off = a.offl + f * DIM;
+#pragma unroll
+                for (int v = 0; v < NF; ++v) {
+                    real change = (real)off[0] * g[v];
+#pragma unroll
+                    for (int d = 1; d < DIM; ++d) change = change + (real)off[d] * g[d * NF + v];
+                    l[v] = l[v] + change;
+                }
+            }
+            if (ri >= 0) {
+                const real* g = grad + (long)ri * DIM * NF;
+                const double* off = a.offr + f * DIM;
+#pragma unroll
+                for (int v = 0; v < NF; ++v) {
+                    real change = (real)off[0] * g[v];
+#pragma unroll
+                    for (int d = 1; d < DIM; ++d) change = change + (real)off[d] * g[d * NF + v];
+                    r[v] = r[v] + change;
+                }
+            }
+            if (side == -1 && a.face[2 * D].type == FACE_WALL) {
+#pragma unroll
+                for (int v = 0; v < NF; ++v) l[v] = r[v];
+                l[D + 1] = -l[D + 1];
+                gl = gr;
+            } else if (side == 1 && a.face[2 * D + 1].type == FACE_WALL) {
+#pragma unroll
+                for (int v = 0; v < NF; ++v) r[v] = l[v];
+                r[D + 1] = -r[D + 1];
+                gr = gl;
+            }
+        }
+        real flux[NF];
+        rusanov_along<NF, DIM>(l, r, gl, gr, D, flux);
+        const real A = (real)a.area[f];
+        real* W = (real*)a.W;
+#pragma unroll
+        for (int v = 0; v < NF; ++v) {
+            const real w = flux[v] * A;
+            W[v * F + f] = w;
+            if (side != 0) out[v] = side > 0 ? (double)w : -(double)w;
+        }
+        const real* Q = (const real*)a.Q;
+        real* M = (real*)a.M;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            if (j >= nq) break;
+            M[j * F + f] = (flux[0] * A) * (flux[0] >= (real)0 ? Q[(2 * j) * F + f] : Q[(2 * j + 1) * F + f]);
+        }
+        const real s = nmax(fabs(l[D + 1]) + sqrt((gl * l[E]) / l[0]), fabs(r[D + 1]) + sqrt((gr * r[E]) / r[0]));
+        ((real*)a.S)[f] = s * A;
+    }
+#pragma unroll
+    for (int v = 0; v < NF; ++v) block_sum(out[v], fvl::S_OUT + v, partials);
+}
+
+// Leaf update (torchAdaptive.py:80-146, 169-206): geometric source times
+// volume plus the signed weighted fluxes of the leaf's faces, over the
+// volume; then the stage output.
+template <int DIM>
+__global__ void __launch_bounds__(MT) k_amr_update(fvl::Amr a, fvl::MStage s) {
+    constexpr int NF = DIM + 2;
+    const long N = a.nleaf, F = a.nface;
+    const long n = blockIdx.x * (long)MT + threadIdx.x;
+    double sums[NF];
+#pragma unroll
+    for (int v = 0; v < NF; ++v) sums[v] = 0;
+    int bad = 0;
+    if (n < N) {
+        Ctx c;
+        c.S = (const real*)s.S;
+        c.q[0] = (const real*)s.qS[0];
+        c.q[1] = (const real*)s.qS[1];
+        c.N = N;
+        c.nq = s.nq;
+        c.reactive = s.reactive;
+        c.gamma = (real)s.gamma;
+        const Pad<NF> me = load_cell<NF>(c, n);
+        const double V = a.volume[n];
+        const real Vr = (real)V;
+        real src[NF], chg[NF], kq[2] = {0, 0};
+        geometric_source<DIM>(a.geometry, me.p, a.div[n], DIM >= 2 ? a.div[N + n] : 0.0, src);
+#pragma unroll
+        for (int v = 0; v < NF; ++v) {
+            chg[v] = src[v] * Vr;
+            sums[v] = (double)src[v] * V;
+        }
+        const real* W = (const real*)a.W;
+        const real* M = (const real*)a.M;
+        for (long e = a.start[n]; e < a.start[n + 1]; ++e) {
+            const long f = a.cface[e];
+            const bool is_left = a.crole[e] < 0;
+#pragma unroll
+            for (int v = 0; v < NF; ++v) {
+                const real w = W[v * F + f];
+                chg[v] = chg[v] + (is_left ? -w : w);
+            }
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                if (j >= s.nq) break;
+                const real m = M[j * F + f];
+                kq[j] = kq[j] + (is_left ? -m : m);
+            }
+        }
+        real k[NF];
+#pragma unroll
+        for (int v = 0; v < NF; ++v) k[v] = chg[v] / Vr;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) kq[j] = kq[j] / Vr;
+        stage_out<NF>(s, N, n, k, kq, bad);
+    }
+    if (bad) atomicOr(s.flags, bad);
+#pragma unroll
+    for (int v = 0; v < NF; ++v) block_sum(sums[v], S_MSRC + v, s.partials);
+}
+
+// CFL rate of one leaf: the area-weighted face speeds summed direction by
+// direction (faces where the leaf is left, then right), 0.5 rate / volume.
+template <int DIM>
+__global__ void __launch_bounds__(MT) k_amr_cfl(fvl::Amr a, double* partials) {
+    const long n = blockIdx.x * (long)MT + threadIdx.x;
+    double v = 0;
+    if (n < a.nleaf) {
+        const real* S = (const real*)a.S;
+        real rate = 0;
+        const long e0 = a.start[n], e1 = a.start[n + 1];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d)
+            for (long e = e0; e < e1; ++e) {
+                const long f = a.cface[e];
+                if (a.dir[f] == d) rate = rate + S[f];
+            }
+        v = (double)(((real)0.5 * rate) / (real)a.volume[n]);
+    }
+    block_max(v, partials);
+}
+
+// Reaction source integral of the energy row over the leaves:
+// sum of (((old - start) + state) - hydro) * volume (torchAdaptive.py:464-467).
+__global__ void __launch_bounds__(MT) k_amr_source(fvl::Amr a, int e, const real* start, const real* old,
+                                                   const real* state, const real* hydro, double* partials) {
+    const long N = a.nleaf;
+    const long n = blockIdx.x * (long)MT + threadIdx.x;
+    double v = 0;
+    if (n < N) {
+        const long o = (long)e * N + n;
+        v = (double)((((old[o] - start[o]) + state[o]) - hydro[o]) * (real)a.volume[n]);
+    }
+    block_sum(v, S_MSRC + e, partials);
+}
+
+template <int DIM>
+hipError_t amr_eval_dim(const fvl::Amr& a, const Ctx& c, int reconstruct, int* flags, double* partials,
+                        hipStream_t st) {
+    const int fb = (int)((a.nface + MT - 1) / MT), lb = (int)((a.nleaf + MT - 1) / MT);
+    k_amr_faces<DIM><<<fb, MT, 0, st>>>(a, c, flags);
+    if (reconstruct) k_amr_gradient<DIM><<<lb, MT, 0, st>>>(a, c);
+    k_amr_flux<DIM><<<fb, MT, 0, st>>>(a, c.nq, reconstruct, partials);
+    return hipGetLastError();
+}
+
 template <int DIM>
 hipError_t launch_stage(const fvl::Grid& g, const fvl::Metric& m, const fvl::MStage& s, int blocks,
                         hipStream_t st) {
@@ -616,6 +965,47 @@ hipError_t metric_source(const fvl::Grid& g, const fvl::Metric& m, const void* s
                          const void* state, const void* hydro, double* partials, hipStream_t st) {
     k_metric_source<<<metric_blocks(g), MT, 0, st>>>(g, m, (const real*)start, (const real*)old, (const real*)state,
                                                      (const real*)hydro, partials);
+    return hipGetLastError();
+}
+
+int amr_blocks(long n) { return (int)((n + MT - 1) / MT); }
+
+hipError_t amr_eval(int dim, const fvl::Amr& a, const void* S, const void* q0, const void* q1, int nq, double gamma,
+                    int reactive, int reconstruct, int* flags, double* partials, hipStream_t st) {
+    if (a.nface <= 0 || a.nleaf <= 0) return hipSuccess;
+    Ctx c;
+    c.S = (const real*)S;
+    c.q[0] = (const real*)q0;
+    c.q[1] = (const real*)q1;
+    c.N = a.nleaf;
+    c.nq = nq;
+    c.reactive = reactive;
+    c.gamma = (real)gamma;
+    if (dim == 1) return amr_eval_dim<1>(a, c, reconstruct, flags, partials, st);
+    if (dim == 2) return amr_eval_dim<2>(a, c, reconstruct, flags, partials, st);
+    return amr_eval_dim<3>(a, c, reconstruct, flags, partials, st);
+}
+
+hipError_t amr_update(int dim, const fvl::Amr& a, const fvl::MStage& s, hipStream_t st) {
+    const int lb = amr_blocks(a.nleaf);
+    if (dim == 1) k_amr_update<1><<<lb, MT, 0, st>>>(a, s);
+    else if (dim == 2) k_amr_update<2><<<lb, MT, 0, st>>>(a, s);
+    else k_amr_update<3><<<lb, MT, 0, st>>>(a, s);
+    return hipGetLastError();
+}
+
+hipError_t amr_cfl(int dim, const fvl::Amr& a, double* partials, hipStream_t st) {
+    const int lb = amr_blocks(a.nleaf);
+    if (dim == 1) k_amr_cfl<1><<<lb, MT, 0, st>>>(a, partials);
+    else if (dim == 2) k_amr_cfl<2><<<lb, MT, 0, st>>>(a, partials);
+    else k_amr_cfl<3><<<lb, MT, 0, st>>>(a, partials);
+    return hipGetLastError();
+}
+
+hipError_t amr_source(int dim, const fvl::Amr& a, const void* start, const void* old, const void* state,
+                      const void* hydro, double* partials, hipStream_t st) {
+    k_amr_source<<<amr_blocks(a.nleaf), MT, 0, st>>>(a, dim + 1, (const real*)start, (const real*)old,
+                                                     (const real*)state, (const real*)hydro, partials);
     return hipGetLastError();
 }
 

@@ -215,6 +215,12 @@ class Simulation:
         if getattr(self, "_execution", None) is not None:
             self._execution.close()
 
+    def to_adaptive(self, **options):
+        """An adaptive (AMR) session continuing this one (torchSession.py:185-188)."""
+        from .torchAdaptive import AdaptiveSimulation
+
+        return AdaptiveSimulation.from_uniform(self, **options)
+
     @property
     def gamma(self):
         return (4.5 - self.lam) / (3.5 - self.lam) if self.config["reactive"] else self.config["gamma"]
