@@ -26,6 +26,9 @@ workload's length). The JSON line reports:
 - metric: the metric/face scheme (curvilinear grid, face boundaries,
   SSPRK2 MUSCL-Rusanov) on a cylindrical-sector blast, FP64 and FP32, with
   its own CPU baseline;
+- adaptive: the dyadic AMR leaves (2D Sod, left half refined twice), FP64
+  and FP32 leaf-updates/s of the device step, the host remesh time, and its
+  own CPU baseline;
 - cpu_baseline: the NumPy oracle (a restatement of the reference's CPU path)
   timed on this host on a bounded sample of the same workload.
 
@@ -317,6 +320,82 @@ def cpu_metric_baseline(budget_s=8.0, n=24):
             "sample": f"cylindrical-sector blast {n}^3, {steps} metric steps of the NumPy oracle, {wall:.1f} s"}
 
 
+def adaptive_problem(nx=512, ny=64):
+    """A 2D Sod tube on dyadic leaves: nx x ny roots over [0, 1] x [0, ny/nx]
+    (square cells), (rho, u, v, p) = (1, 0, 0, 1) for x < 0.5 and
+    (0.125, 0, 0, 0.1) beyond, open x ends, reflective y walls, CFL 0.4,
+    second-order reconstruction; the left half refined twice (level 2)."""
+    prim = np.zeros((4, ny, nx))
+    right = (np.arange(nx) + 0.5) / nx >= 0.5
+    prim[0] = np.where(right, 0.125, 1.0)[None, :]
+    prim[3] = np.where(right, 0.1, 1.0)[None, :]
+    opts = dict(boundaries=["Open", "Reflective"], t_end=10.0, cfl=0.4)
+    return prim, [1.0, ny / nx], opts
+
+
+def adaptive_session(precision, nx, ny, regrid_every=10 ** 9):
+    from torch_amd import Simulation
+
+    prim, lengths, opts = adaptive_problem(nx, ny)
+    s = Simulation(prim, lengths, precision=precision, **opts).to_adaptive(
+        max_level=2, regrid_every=regrid_every, max_cells=2_000_000)
+    for _ in range(2):
+        s.regrid(refine=s.mesh.centers[:, 0] < 0.5, coarsen=False)
+    return s
+
+
+def run_adaptive(precision, steps, warmup, nx, ny):
+    s = adaptive_session(precision, nx, ny)
+    leaves, faces = len(s.mesh.keys), len(s.mesh.left)
+    for _ in range(warmup):
+        s.step()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s.step()
+    wall = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    s.regrid()
+    regrid_s = time.perf_counter() - t1
+    out = dict(value=round(leaves * steps / wall, 1), ms_per_step=round(1e3 * wall / steps, 4), leaves=leaves,
+               faces=faces, regrid_s=round(regrid_s, 3), leaves_after_regrid=len(s.mesh.keys),
+               dtype="f64" if precision == "fp64" else "f32")
+    s.close()
+    return out
+
+
+def cpu_adaptive_baseline(budget_s=8.0, nx=128, ny=16):
+    """The NumPy oracle's adaptive cfl + advance on the same workload at
+    nx x ny roots (left half at level 2), single thread."""
+    os.environ.setdefault("OPENBLAS_NUM_THREADS", "1")
+    from oracle import adaptive as oa
+    from torch_amd import torchAdaptiveGeometry as tag
+    from torch_amd.torchAdaptive import remesh
+    from torch_amd.torchGeometry import normalize_faces
+
+    prim, lengths, opts = adaptive_problem(nx, ny)
+    c = dict(dimension=2, counts_xyz=[nx, ny], lengths_xyz=lengths, boundaries_xyz=opts["boundaries"],
+             origins_xyz=[0.0, 0.0], geometry="cartesian", gamma=1.4, cfl=opts["cfl"], reactive=False,
+             kinetic_model="arrhenius")
+    c["face_boundaries"] = normalize_faces(c, None)
+    c["amr"] = tag.settings(max_level=2, max_cells=2_000_000)
+    mesh = tag.LeafMesh(c)
+    state = prim.reshape(4, -1).copy()
+    state[-1] = prim[3].ravel() / 0.4
+    for _ in range(2):
+        faces = oa.face_values(mesh, state, [], c, False)[:4]
+        mesh, state, _, _, _ = remesh(mesh, state, [], c, faces, mesh.centers[:, 0] < 0.5, False)
+    t0 = time.perf_counter()
+    steps = 0
+    while steps < 2 or time.perf_counter() - t0 < budget_s:
+        dt = oa.cfl(mesh, state, [], c)
+        state = oa.advance(mesh, state, [], dt, c)[0]
+        steps += 1
+    wall = time.perf_counter() - t0
+    return {"value": round(len(mesh.keys) * steps / wall, 1), "unit": "leaf-updates/s", "cores": 1, "kind": "port",
+            "sample": f"2D Sod on {nx}x{ny} roots, left half at level 2 ({len(mesh.keys)} leaves), {steps} "
+                      f"cfl+advance steps of the NumPy oracle, {wall:.1f} s"}
+
+
 def copy_calibration(mib=1024, reps=3):
     """Device-to-device copies of a known size, for scaling rocprof's
     FETCH_SIZE/WRITE_SIZE counters (each copy reads and writes `mib` MiB)."""
@@ -339,7 +418,12 @@ def copy_calibration(mib=1024, reps=3):
     print(json.dumps({"copy_mib": mib, "reps": reps, "GBps_rw": round(2 * n * reps / wall / 1e9, 1)}))
 
 
-LEGS = ("fp64", "fp32", "batched", "error", "uneven", "served", "metric", "cpu")
+LEGS = ("fp64", "fp32", "batched", "error", "uneven", "served", "metric", "adaptive", "cpu")
+
+
+def _pair(text):
+    a, b = (int(v) for v in text.split(","))
+    return a, b
 
 
 def _tile(text):
@@ -377,6 +461,8 @@ def main(argv=None):
     ap.add_argument("--serve-closed-only", action="store_true", help="skip the open-loop (Poisson) run")
     ap.add_argument("--metric-n", type=int, default=96, help="cells per axis of the metric/face workload")
     ap.add_argument("--metric-steps", type=int, default=20)
+    ap.add_argument("--amr-roots", type=_pair, default=(512, 64), help="NX,NY roots of the adaptive workload")
+    ap.add_argument("--amr-steps", type=int, default=20)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device copies of MIB MiB (counter calibration) and exit")
     args = ap.parse_args(argv)
@@ -444,6 +530,20 @@ def main(argv=None):
             say(f"# metric {precision}: {metric[precision]['ms_per_step']} ms/step")
         if "cpu" in legs:
             metric["cpu_baseline"] = cpu_metric_baseline()
+    adaptive = None
+    if "adaptive" in legs:
+        ax, ay = args.amr_roots
+        adaptive = {"workload": f"adaptive (dyadic AMR) leaves: 2D Sod tube on {ax}x{ay} roots, left half "
+                                "refined to level 2, open x / reflective y, CFL 0.4, second-order bounded "
+                                "least-squares reconstruction, Rusanov faces, SSPRK2; full AdaptiveSimulation.step "
+                                "without regrids (regrid_s: one host remesh of the final state)",
+                    "steps": args.amr_steps, "unit": "leaf-updates/s"}
+        for precision in ("fp64", "fp32"):
+            adaptive[precision] = run_adaptive(precision, args.amr_steps, 2, ax, ay)
+            say(f"# adaptive {precision}: {adaptive[precision]['ms_per_step']} ms/step, "
+                f"{adaptive[precision]['leaves']} leaves")
+        if "cpu" in legs:
+            adaptive["cpu_baseline"] = cpu_adaptive_baseline()
     if "cpu" in legs:
         cpu = cpu_baseline()
     head = r32 or r32s
@@ -471,6 +571,7 @@ def main(argv=None):
         "batched": uneven,
         "served": served,
         "metric": metric,
+        "adaptive": adaptive,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

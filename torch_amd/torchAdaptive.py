@@ -20,9 +20,9 @@ import numpy as np
 
 from . import _abi
 from . import torchGeometry as _geometry
-from .torchAdaptiveGeometry import LeafMesh, children, settings, volume
+from .torchAdaptiveGeometry import LeafMesh, canonical_order, settings, volume
 from .torchExecution import DeviceExecution
-from .torchSession import Simulation, _physical
+from .torchSession import Simulation
 
 
 def _conservative(p, gamma):
@@ -59,24 +59,40 @@ def _balanced(mesh, tag, max_level):
 
 def _coarsenable(mesh, tag, indicator, options, i, j):
     """Complete sibling groups below the coarsening threshold whose merge
-    keeps the 2:1 balance with the (tagged) neighbours."""
-    groups = {}
-    for idx, key in enumerate(mesh.keys):
-        if key[0] <= options["min_level"] or tag[idx] or indicator[idx] >= options["coarsen_threshold"]:
-            continue
-        parent = (int(key[0]) - 1, *(int(v) // 2 for v in key[1:]))
-        groups.setdefault(parent, []).append(idx)
-    accepted = {}
-    for parent, ids in groups.items():
-        if len(ids) != 2 ** mesh.dimension:
-            continue
-        member = np.zeros(len(tag), dtype=bool)
-        member[ids] = True
-        outside = np.concatenate((j[member[i] & ~member[j]], i[member[j] & ~member[i]]))
-        if np.any(mesh.levels[outside] + tag[outside] > parent[0] + 1):
-            continue
-        accepted[parent] = ids
-    return accepted
+    keeps the 2:1 balance with the (tagged) neighbours: (parents (G, d+1),
+    member leaves (G, 2^d) in leaf order)."""
+    d = mesh.dimension
+    empty = (np.zeros((0, d + 1), dtype=np.int64), np.zeros((0, 2 ** d), dtype=np.int64))
+    ids = np.flatnonzero((mesh.levels > options["min_level"]) & ~tag & (indicator < options["coarsen_threshold"]))
+    if not len(ids):
+        return empty
+    parent = np.column_stack((mesh.levels[ids] - 1, mesh.keys[ids, 1:] // 2))
+    codes = np.empty(len(ids), dtype=np.int64)
+    for level in np.unique(parent[:, 0]):
+        at = parent[:, 0] == level
+        codes[at] = mesh._codes(level, parent[at, 1:])
+    _, first, inverse, count = np.unique(codes, return_index=True, return_inverse=True, return_counts=True)
+    inverse = inverse.ravel()
+    complete = np.flatnonzero(count == 2 ** d)
+    if not len(complete):
+        return empty
+    slot = np.full(len(count), -1)
+    slot[complete] = np.arange(len(complete))
+    member = ids[np.argsort(inverse, kind="stable")].reshape(-1)
+    starts = np.concatenate(([0], np.cumsum(count)))
+    groups = member[starts[complete][:, None] + np.arange(2 ** d)]
+    # A group is rejected when a leaf across one of its faces, outside the
+    # group, would end up more than one level finer than the merged parent.
+    group = np.full(len(tag), -1)
+    group[ids] = slot[inverse]
+    limit = parent[first[complete], 0] + 1
+    finer = mesh.levels + tag
+    rejected = np.zeros(len(complete), dtype=bool)
+    for a, b in ((i, j), (j, i)):
+        crossing = (group[a] >= 0) & (group[a] != group[b])
+        rejected[group[a][crossing][finer[b][crossing] > limit[group[a][crossing]]]] = True
+    keep = ~rejected
+    return parent[first[complete]][keep], groups[keep]
 
 
 def remesh(mesh, state, fractions, config, faces, refine=None, coarsen=True):
@@ -88,52 +104,67 @@ def remesh(mesh, state, fractions, config, faces, refine=None, coarsen=True):
     ``mesh``. Returns (mesh, state, fractions, budget_limited, changed)."""
     left, right, gl, gr = faces
     options = config["amr"]
+    d = mesh.dimension
+    nchild = 2 ** d
     indicator = _indicator(mesh, left, right)
     tag = indicator > options["refine_threshold"] if refine is None else np.asarray(refine, dtype=bool)
     if tag.shape != indicator.shape:
         raise ValueError("Refinement mask must match leaves")
     tag = (tag | (mesh.levels < options["min_level"])) & (mesh.levels < options["max_level"])
     tag, i, j = _balanced(mesh, tag, options["max_level"])
-    accepted = _coarsenable(mesh, tag, indicator, options, i, j) if coarsen else {}
-    removed = {idx for ids in accepted.values() for idx in ids}
-    grow = 2 ** mesh.dimension - 1
-    budget_limited = len(tag) + int(tag.sum()) * grow - len(accepted) * grow > options["max_cells"]
+    if coarsen:
+        parents, groups = _coarsenable(mesh, tag, indicator, options, i, j)
+    else:
+        parents, groups = np.zeros((0, d + 1), dtype=np.int64), np.zeros((0, nchild), dtype=np.int64)
+    budget_limited = len(tag) + int(tag.sum()) * (nchild - 1) - len(parents) * (nchild - 1) > options["max_cells"]
     if budget_limited:
         tag[:] = False  # refinements wait; coarsening alone keeps the mesh valid
-    if not np.any(tag) and not accepted:
+    if not np.any(tag) and not len(parents):
         return mesh, state, fractions, budget_limited, False
-    data = {}
-    for parent, ids in accepted.items():
-        w = mesh.volumes[ids]
-        mean = np.sum(state[:, ids] * w, axis=1) / np.sum(w)
-        data[parent] = (mean, [np.sum(state[0, ids] * q[ids] * w) / np.sum(state[0, ids] * w) for q in fractions])
-    gradient = None
-    if np.any(tag) and options["order"] == 2:
-        gradient = mesh.gradients(state, _conservative(left, gl), _conservative(right, gr))
-    lengths, counts = np.array(config["lengths_xyz"]), np.array(config["counts_xyz"])
-    for idx, key_array in enumerate(mesh.keys):
-        if idx in removed:
-            continue
-        key = tuple(int(v) for v in key_array)
-        if not tag[idx]:
-            data[key] = (state[:, idx], [q[idx] for q in fractions])
-            continue
-        group = children(key)
-        low = mesh.origin + np.array([k[1:] for k in group]) * lengths / counts / 2 ** (key[0] + 1)
-        high = low + lengths / counts / 2 ** (key[0] + 1)
-        w = volume(config, low, high)
-        values = np.repeat(state[:, idx, None], len(group), axis=1)
-        if gradient is not None:
-            change = (0.5 * (low + high) - mesh.centers[idx]) @ gradient[idx]
-            change -= np.sum(change * w[:, None], axis=0) / np.sum(w)
-            trial = values + change.T
-            if _physical(trial):
-                values = trial
-        for k, value in zip(group, values.T):
-            data[k] = (value, [q[idx] for q in fractions])
-    candidate = LeafMesh(config, np.array(list(data), dtype=np.int64))
-    out = np.stack([data[tuple(key)][0] for key in candidate.keys], axis=1)
-    qs = [np.array([data[tuple(key)][1][a] for key in candidate.keys]) for a in range(len(fractions))]
+    removed = np.zeros(len(tag), dtype=bool)
+    removed[groups.ravel()] = True
+    keep = ~removed & ~tag
+    keys, values, qs = [mesh.keys[keep]], [state[:, keep]], [[q[keep]] for q in fractions]
+    if len(parents):
+        # Conservative restriction: volume-weighted means, mass-weighted fractions.
+        w = mesh.volumes[groups]
+        keys.append(parents)
+        values.append(np.sum(state[:, groups] * w, axis=2) / np.sum(w, axis=1))
+        rho = state[0, groups]
+        for out, q in zip(qs, fractions):
+            out.append(np.sum(rho * q[groups] * w, axis=1) / np.sum(rho * w, axis=1))
+    t = np.flatnonzero(tag)
+    if len(t):
+        # Prolongation: each tagged leaf's state on its children, plus the
+        # limited conservative gradient at the child centres with the
+        # volume-weighted mean removed, kept where every child stays physical.
+        level = mesh.levels[t] + 1
+        bits = np.array([[(c >> a) & 1 for a in range(d)] for c in range(nchild)])
+        index = 2 * mesh.keys[t, 1:][:, None, :] + bits[None]
+        lengths, counts = np.array(config["lengths_xyz"]), np.array(config["counts_xyz"])
+        scale = (2.0 ** level)[:, None, None]
+        low = mesh.origin + index * lengths / counts / scale
+        high = low + lengths / counts / scale
+        w = volume(config, low.reshape(-1, d), high.reshape(-1, d)).reshape(len(t), nchild)
+        fine = np.repeat(state[:, t, None], nchild, axis=2)
+        if options["order"] == 2:
+            gradient = mesh.gradients(state, _conservative(left, gl), _conservative(right, gr))[t]
+            change = np.matmul(0.5 * (low + high) - mesh.centers[t][:, None, :], gradient)
+            change -= np.sum(change * w[:, :, None], axis=1, keepdims=True) / np.sum(w, axis=1)[:, None, None]
+            trial = fine + change.transpose(2, 0, 1)
+            internal = trial[-1] - 0.5 * np.sum(trial[1:-1] ** 2, axis=0) / trial[0]
+            ok = (np.isfinite(trial).all(axis=(0, 2)) & np.all(trial[0] > 1e-12, axis=1)
+                  & np.all(internal > 1e-12, axis=1))
+            fine = np.where(ok[None, :, None], trial, fine)
+        keys.append(np.column_stack((np.repeat(level, nchild), index.reshape(-1, d))))
+        values.append(fine.reshape(state.shape[0], -1))
+        for out, q in zip(qs, fractions):
+            out.append(np.repeat(q[t], nchild))
+    keys = np.concatenate(keys).astype(np.int64)
+    order = canonical_order(keys)
+    candidate = LeafMesh(config, keys[order])
+    out = np.concatenate(values, axis=1)[:, order]
+    qs = [np.concatenate(parts)[order] for parts in qs]
     changed = not np.array_equal(mesh.keys, candidate.keys)
     return (candidate if changed else mesh), out, qs, budget_limited, changed
 

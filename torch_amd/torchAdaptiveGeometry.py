@@ -47,10 +47,16 @@ def children(key):
     return [(level + 1, *(2 * i + b for i, b in zip(index, bits[::-1]))) for bits in product((0, 1), repeat=len(index))]
 
 
-def canonical(keys):
-    """Leaves sorted by level, then by the last index down to the first."""
+def canonical_order(keys):
+    """Permutation sorting leaves by level, then by the last index down to the first."""
     keys = np.asarray(keys, dtype=np.int64)
-    return keys[np.lexsort(tuple(keys[:, k] for k in range(1, keys.shape[1])) + (keys[:, 0],))]
+    return np.lexsort(tuple(keys[:, k] for k in range(1, keys.shape[1])) + (keys[:, 0],))
+
+
+def canonical(keys):
+    """Leaves in canonical order."""
+    keys = np.asarray(keys, dtype=np.int64)
+    return keys[canonical_order(keys)]
 
 
 def volume(c, low, high):
@@ -125,8 +131,16 @@ class LeafMesh:
             raise ValueError("Leaf level is outside configured bounds")
         if np.any(self.keys[:, 1:] < 0) or np.any(self.keys[:, 1:] >= counts * (2 ** self.levels[:, None])):
             raise ValueError("Leaf indices are outside the domain")
-        self.lookup = {tuple(int(v) for v in key): i for i, key in enumerate(self.keys)}
-        if len(self.lookup) != len(self.keys):
+        self.counts = counts
+        sizes = [int(np.prod(counts)) * 2 ** (d * level) for level in range(self.max_level + 1)]
+        self._offsets = np.concatenate(([0], np.cumsum(sizes)[:-1])).astype(np.int64)
+        codes = np.empty(len(self.keys), dtype=np.int64)
+        for level in np.unique(self.levels):
+            at = self.levels == level
+            codes[at] = self._codes(level, self.keys[at, 1:])
+        self._code_order = np.argsort(codes, kind="stable")
+        self._sorted_codes = codes[self._code_order]
+        if np.any(self._sorted_codes[1:] == self._sorted_codes[:-1]):
             raise ValueError("Duplicate leaf keys")
         if validate:
             self._check_tree(counts)
@@ -149,82 +163,124 @@ class LeafMesh:
         self._topology()
 
     def _check_tree(self, counts):
-        """Every root is covered exactly once (complete sibling groups merge upward)."""
-        remaining = set(self.lookup)
+        """Every root is covered exactly once: complete sibling groups merge
+        upward level by level (a merged parent that is itself a leaf means
+        overlapping cells), and what remains must be the roots."""
+        d = self.dimension
+        levels, idx = self.levels.copy(), self.keys[:, 1:].copy()
         for level in range(self.max_level, 0, -1):
-            parents = {(level - 1, *(v // 2 for v in key[1:])) for key in remaining if key[0] == level}
-            for parent in parents:
-                group = set(children(parent))
-                if group <= remaining:
-                    if parent in remaining:
-                        raise ValueError("Overlapping adaptive cells")
-                    remaining.difference_update(group)
-                    remaining.add(parent)
-        if remaining != {tuple(int(v) for v in key) for key in _roots(counts)}:
+            at = np.flatnonzero(levels == level)
+            if not len(at):
+                continue
+            parents = idx[at] // 2
+            codes = self._codes(level - 1, parents)
+            uniq, first, inverse, count = np.unique(codes, return_index=True, return_inverse=True,
+                                                   return_counts=True)
+            full = count == 2 ** d
+            present = levels == level - 1
+            if present.any() and np.isin(uniq[full], self._codes(level - 1, idx[present])).any():
+                raise ValueError("Overlapping adaptive cells")
+            keep = np.ones(len(levels), dtype=bool)
+            keep[at[full[inverse.ravel()]]] = False
+            levels = np.concatenate((levels[keep], np.full(int(full.sum()), level - 1)))
+            idx = np.concatenate((idx[keep], parents[first[full]]))
+        if not (np.all(levels == 0) and len(levels) == int(np.prod(counts))):
             raise ValueError("Adaptive leaves overlap or leave domain holes")
 
-    def _find(self, point):
+    def _codes(self, level, idx):
+        """Integer codes of the keys (level, idx...), idx (n, d) in x, y, z order."""
+        dims = self.counts * 2 ** level
+        return self._offsets[level] + np.ravel_multi_index(tuple(idx.T[::-1]), tuple(dims[::-1]))
+
+    def _find(self, points):
+        """Leaf containing each point (n, d), in finest-level integer units;
+        the coarsest level holding a point wins (the reference's _find order)."""
+        found = np.full(len(points), -1, dtype=np.int64)
         for level in range(self.max_level + 1):
-            size = 2 ** (self.max_level - level)
-            found = self.lookup.get((level, *(int(v) for v in np.floor(point / size))))
-            if found is not None:
-                return found
-        raise ValueError("Adaptive mesh has an uncovered neighbor")
+            pending = np.flatnonzero(found < 0)
+            if not len(pending):
+                break
+            idx = np.floor(points[pending] / 2 ** (self.max_level - level)).astype(np.int64)
+            codes = self._codes(level, idx)
+            pos = np.minimum(np.searchsorted(self._sorted_codes, codes), len(self._sorted_codes) - 1)
+            hit = self._sorted_codes[pos] == codes
+            found[pending[hit]] = self._code_order[pos[hit]]
+        if np.any(found < 0):
+            raise ValueError("Adaptive mesh has an uncovered neighbor")
+        return found
+
+    def _face_list(self):
+        """The reference's face order (torchAdaptiveGeometry.py:197-240),
+        built with array operations: per leaf in canonical order and per
+        direction, the high face (a domain-boundary face, the shared face
+        with the neighbour, or its subfaces in depth-first order where the
+        neighbour is finer), then the low domain-boundary face."""
+        c, d = self.config, self.dimension
+        periodic = [c["face_boundaries"][2 * k]["type"] == "periodic" for k in range(d)]
+        width = max(2 ** (d - 1), 1)
+        depth_cap = self.max_level + 1
+        parts = []
+
+        def emit(owner, k, part, left, right, lo, hi, position, side, code):
+            parts.append(dict(owner=owner, k=np.full(len(owner), k), part=np.full(len(owner), part), left=left,
+                              right=right, lo=lo, hi=hi, position=position, side=np.full(len(owner), side),
+                              code=code))
+
+        for k in range(d):
+            others = np.arange(d) != k
+            transverse = [a for a in range(d) if a != k]
+            position = self.integer_high[:, k]
+            edge = (position == self.total[k]) & (not periodic[k])
+            ib = np.flatnonzero(edge)
+            emit(ib, k, 0, ib, np.full(len(ib), -1), self.integer_low[ib], self.integer_high[ib], position[ib], 1,
+                 np.zeros(len(ib), dtype=np.int64))
+            owner = np.flatnonzero(~edge)
+            lo, hi, pos = self.integer_low[owner].copy(), self.integer_high[owner].copy(), position[owner]
+            code = np.zeros(len(owner), dtype=np.int64)
+            for depth in range(depth_cap + 1):
+                if not len(owner):
+                    break
+                if depth == depth_cap:
+                    raise ValueError("Adaptive mesh has an uncovered neighbor")
+                point = 0.5 * (lo + hi)
+                point[:, k] = (pos + 0.5) % self.total[k]
+                j = self._find(point)
+                covers = (np.all(self.integer_low[j][:, others] <= lo[:, others], axis=1)
+                          & np.all(self.integer_high[j][:, others] >= hi[:, others], axis=1))
+                emit(owner[covers], k, 0, owner[covers], j[covers], lo[covers], hi[covers], pos[covers], 0,
+                     code[covers] * width ** (depth_cap - depth))
+                split = ~covers
+                owner, lo, hi, pos, code = owner[split], lo[split], hi[split], pos[split], code[split]
+                middle = (lo + hi) // 2
+                subs = []
+                for b, bits in enumerate(product((0, 1), repeat=len(transverse))):
+                    sub_lo, sub_hi = lo.copy(), hi.copy()
+                    for a, bit in zip(transverse, bits):
+                        if bit:
+                            sub_lo[:, a] = middle[:, a]
+                        else:
+                            sub_hi[:, a] = middle[:, a]
+                    subs.append((owner, sub_lo, sub_hi, pos, code * width + b))
+                owner, lo, hi, pos, code = (np.concatenate([s[f] for s in subs]) for f in range(5))
+            il = np.flatnonzero((self.integer_low[:, k] == 0) & (not periodic[k]))
+            emit(il, k, 1, np.full(len(il), -1), il, self.integer_low[il], self.integer_high[il],
+                 np.zeros(len(il), dtype=np.int64), -1, np.zeros(len(il), dtype=np.int64))
+        cat = {f: np.concatenate([p[f] for p in parts]) for f in parts[0]}
+        order = np.lexsort((cat["code"], cat["part"], cat["k"], cat["owner"]))
+        return {f: v[order] for f, v in cat.items()}
 
     def _topology(self):
         c, d = self.config, self.dimension
-        periodic = [c["face_boundaries"][2 * k]["type"] == "periodic" for k in range(d)]
-        left, right, dirs, lows, highs, positions, sides = ([] for _ in range(7))
-
-        def emit(i, j, k, lo, hi, position, side=0):
-            left.append(i)
-            right.append(j)
-            dirs.append(k)
-            lows.append(lo.copy())
-            highs.append(hi.copy())
-            positions.append(position)
-            sides.append(side)
-
-        others = [np.arange(d) != k for k in range(d)]
-
-        def match(i, k, lo, hi, position):
-            point = 0.5 * (lo + hi)
-            point[k] = (position + 0.5) % self.total[k]
-            j = self._find(point)
-            o = others[k]
-            if np.all(self.integer_low[j, o] <= lo[o]) and np.all(self.integer_high[j, o] >= hi[o]):
-                emit(i, j, k, lo, hi, position)
-                return
-            # The neighbour is finer: split the face into its subfaces.
-            transverse = [a for a in range(d) if a != k]
-            middle = (lo + hi) // 2
-            for bits in product((0, 1), repeat=len(transverse)):
-                sub_lo, sub_hi = lo.copy(), hi.copy()
-                for a, bit in zip(transverse, bits):
-                    if bit:
-                        sub_lo[a] = middle[a]
-                    else:
-                        sub_hi[a] = middle[a]
-                match(i, k, sub_lo, sub_hi, position)
-
-        for i in range(len(self.keys)):
-            for k in range(d):
-                lo, hi = self.integer_low[i].copy(), self.integer_high[i].copy()
-                position = hi[k]
-                if position == self.total[k] and not periodic[k]:
-                    emit(i, -1, k, lo, hi, position, 1)
-                else:
-                    match(i, k, lo, hi, position)
-                if lo[k] == 0 and not periodic[k]:
-                    emit(-1, i, k, lo, hi, 0, -1)
-        self.left, self.right = np.array(left), np.array(right)
-        self.directions, self.sides = np.array(dirs), np.array(sides)
+        faces = self._face_list()
+        self.left, self.right = faces["left"].astype(np.int64), faces["right"].astype(np.int64)
+        self.directions, self.sides = faces["k"].astype(np.int64), faces["side"].astype(np.int64)
+        lows, highs, positions = faces["lo"], faces["hi"], faces["position"]
         self.face_low = self.origin + np.array(lows) * self.unit
         self.face_high = self.origin + np.array(highs) * self.unit
         self.face_positions = self.origin[self.directions] + np.array(positions) * self.unit[self.directions]
-        self.face_areas = np.empty(len(left))
+        self.face_areas = np.empty(len(self.left))
         center = 0.5 * (self.face_low + self.face_high)
-        center[np.arange(len(left)), self.directions] = self.face_positions
+        center[np.arange(len(self.left)), self.directions] = self.face_positions
         self.offset_left = center - self.centers[np.maximum(self.left, 0)]
         self.offset_right = center - self.centers[np.maximum(self.right, 0)]
         for k in range(d):
