@@ -17,6 +17,13 @@ Modules and the reference code they restate:
 - ``solver1d`` torchSolver.py:71-142, 164-404, 412-508, 534-700, 708-789,
                1038-1132 (legacy 1D scheme and kinetics)
 - ``session``  torchSession.py:333-522 (Strang-split step driver)
+- ``metric``   torchFiniteVolume.py:16-276, torchGeometry.py:93-213,
+               torchBoundaries.py:17-246, torchFlux.py:80-122 (metric/face scheme)
+- ``legacy``   torchSolver.py:918-1418, torchSolver2D.py:670-946,
+               torchSolver3D.py:289-471 (stand-alone drivers, time histories)
+- ``adaptive`` torchAdaptive.py:14-206 (adaptive leaves: face values, bounded
+               gradients, rhs, cfl, advance)
+- ``presets``  the reference's named initial conditions
 - ``exact``    an exact Riemann solver (Toro, "Riemann Solvers and
                Numerical Methods for Fluid Dynamics", ch. 4); test oracle
                for the 1D/3D Sod profiles, independent of the reference.

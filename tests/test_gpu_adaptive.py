@@ -178,6 +178,32 @@ def test_rest_and_metric_partition_in_all_geometries():
             s.close()
 
 
+def test_adaptive_sod_along_every_axis_against_the_exact_solution():
+    """testAdaptive.py:89-109 with the exact Riemann solution (oracle.exact)."""
+    from oracle import exact
+    from torch_amd import Simulation
+
+    for dim in (1, 2, 3):
+        for axis in range(dim):
+            counts = [2] * dim
+            counts[axis] = 32
+            p = np.ones((dim + 2, *counts[::-1]))
+            p[1:-1] = 0
+            shape = [1] * dim
+            shape[dim - 1 - axis] = 32
+            right = np.broadcast_to((np.arange(32) >= 16).reshape(shape), counts[::-1])
+            p[0, right] = 0.125
+            p[-1, right] = 0.1
+            s = Simulation(p, [1.0] * dim, t_end=0.08).to_adaptive(max_level=1, regrid_every=2)
+            s.run()
+            rho = exact.sample(s.mesh.centers[:, axis], s.time)[0]
+            error = np.sum(abs(s.state[0] - rho) * s.mesh.volumes)
+            assert error < 0.025, (dim, axis, error)
+            assert s.mesh.levels.max() > 0
+            np.testing.assert_allclose(s.balance_residual, 0, atol=1e-13)
+            s.close()
+
+
 def test_refined_rest_at_radial_origin_and_spherical_poles():
     from torch_amd import Simulation
 
