@@ -10,8 +10,11 @@ development container only, and stores small inputs and outputs:
 - rhs (both orders), cfl and SSPRK2 advance (torchAdaptive.py:14-206) on
   those meshes with a nonuniform state, inert and with fractions;
 - multi-step AdaptiveSimulation runs with regrids (Sod along x and y,
-  an entropy wave on a periodic 1D mesh, a reactive spherical case): the
-  leaf keys, state, fractions, time and balances after every step.
+  an entropy wave on a periodic 1D mesh, a reactive spherical case, ZND
+  with induction fractions, first-order reconstruction, a wave periodic in
+  x and y, inflow / pressure-outlet / wall / outflow faces in 2D, a 3D
+  stagnation inlet with periodic y and z): the leaf keys, state, fractions,
+  time and balances after every step.
 
 Each case stores what the tests need to rebuild it without the reference.
 Run: PYTHONDONTWRITEBYTECODE=1 python tests/golden/make_adaptive_fixtures.py
@@ -116,6 +119,42 @@ def main():
     p[-1] = np.where((np.arange(8) + 0.5) / 8 < 0.5, 1e6, 101325.0)[None, :]
     runs.append(("react2d", p, dict(geometry="spherical", origins=[0.4, 0.3], lengths=[0.8, 0.6], t_end=2e-5,
                                     reactive=True, kinetic_model="znd"), dict(max_level=1, regrid_every=2), 5))
+    # ZND with induction fractions, cartesian.
+    p = np.ones((4, 4, 8))
+    p[1:-1] = 0.1
+    p[-1] = np.where((np.arange(8) + 0.5) / 8 < 0.5, 1e6, 101325.0)[None, :]
+    runs.append(("znd2d", p, dict(lengths=[0.8, 0.4], t_end=2e-5, reactive=True, kinetic_model="znd"),
+                 dict(max_level=1, regrid_every=2), 5))
+    # First-order reconstruction.
+    runs.append(("order1", runs[0][1], dict(t_end=0.08), dict(max_level=1, regrid_every=2, order=1), 8))
+    # An entropy wave periodic in x and y: refinement reaches both seams.
+    x = (np.arange(16) + 0.5) / 16
+    y = (np.arange(8) + 0.5) / 8
+    p = np.ones((4, 8, 16))
+    wrap_x = np.minimum(x, 1 - x)  # periodic distance from the x seam
+    p[0] = 1 + 0.3 * np.exp(-60 * wrap_x[None, :] ** 2 - 20 * (y[:, None] - 0.5) ** 2)
+    p[1], p[2] = 0.5, 0.25
+    runs.append(("periodic2d", p, dict(boundaries=["Periodic", "Periodic"], t_end=0.1),
+                 dict(max_level=2, refine_threshold=0.05, coarsen_threshold=0.01, regrid_every=3), 8))
+    # Inflow, pressure outlet with backflow, wall and outflow faces.
+    p = np.ones((4, 4, 16))
+    p[0] = 1 + 0.3 * np.exp(-40 * (x[None, :] - 0.4) ** 2)
+    p[1] = 0.2
+    faces = {"x_min": {"type": "inflow", "primitive": [1.2, 0.3, 0.0, 1.2]},
+             "x_max": {"type": "pressure_outlet", "pressure": 0.9, "backflow": [1.0, -0.1, 0.0, 0.9]},
+             "y_min": "wall", "y_max": "outflow"}
+    runs.append(("faces2d", p, dict(t_end=0.05, face_boundaries=faces),
+                 dict(max_level=1, refine_threshold=0.02, coarsen_threshold=0.005, regrid_every=2), 8))
+    # 3D stagnation inlet and pressure outlet, periodic in y and z.
+    p = np.ones((5, 4, 4, 8))
+    wrap_y = np.minimum((np.arange(4) + 0.5) / 4, 1 - (np.arange(4) + 0.5) / 4)
+    p[0] = 1 + 0.3 * np.exp(-30 * wrap_y ** 2)[None, :, None] * np.exp(-30 * (x[::2] - 0.5) ** 2)[None, None, :]
+    faces = {"x_min": {"type": "stagnation_inlet", "total_pressure": 1.3, "total_temperature": 300.0,
+                       "gas_constant": 1.0 / 300.0},
+             "x_max": {"type": "pressure_outlet", "pressure": 1.0, "backflow": [1.0, -0.1, 0.0, 0.0, 1.0]},
+             "y_min": "periodic", "y_max": "periodic", "z_min": "periodic", "z_max": "periodic"}
+    runs.append(("stag3d", p, dict(t_end=0.02, face_boundaries=faces),
+                 dict(max_level=1, refine_threshold=0.05, coarsen_threshold=0.01, regrid_every=2), 5))
     run_names = []
     for tag, p, opts, amr, steps in runs:
         dim = p.ndim - 1
@@ -134,6 +173,8 @@ def main():
             data[f"{tag}_time{k + 1}"] = np.array(s.time)
             if s.lam is not None:
                 data[f"{tag}_lam{k + 1}"] = s.lam
+            if s.beta is not None:
+                data[f"{tag}_beta{k + 1}"] = s.beta
             if s.completed:
                 break
         data[tag + "_steps"] = np.array(s.steps)

@@ -33,8 +33,9 @@ def run_config(d, tag):
              origins_xyz=list(opts.get("origins", [0.0] * dim)), geometry=opts.get("geometry", "cartesian"),
              t_end=opts["t_end"], cfl=0.4, gamma=1.4, limiter="VanLeer", reactive=opts.get("reactive", False),
              kinetic_model=opts.get("kinetic_model", "arrhenius"), max_steps=100000)
-    if c["geometry"] != "cartesian":
-        c["face_boundaries"] = geo.session_faces(c, None)
+    spec_faces = opts.get("face_boundaries")
+    if c["geometry"] != "cartesian" or spec_faces is not None:
+        c["face_boundaries"] = geo.session_faces(c, spec_faces)
     c["face_boundaries"] = geo.normalize_faces(c, c.get("face_boundaries"))
     c["amr"] = tag_geo.settings(**spec["amr"])
     return c, p
@@ -51,6 +52,8 @@ class HostLeaves:
         dim = c["dimension"]
         lam = np.ones(p[0].size) if c["reactive"] else None
         self.fractions = [lam] if lam is not None else []
+        if c["reactive"] and c["kinetic_model"] == "znd":
+            self.fractions.append(np.ones(p[0].size))
         g = one.gamma_calc(lam) if c["reactive"] else c["gamma"]
         prim = p.reshape(dim + 2, -1)
         state = prim.copy()
@@ -74,11 +77,13 @@ class HostLeaves:
         self.regrids += 1
 
     def react(self, state, fractions, dt):
+        """torchAdaptive.py:418-434."""
         normal = np.stack((state[0], state[1], state[-1] - 0.5 * np.sum(state[2:-1] ** 2, axis=0) / state[0]))
-        lam, reacted, _ = one.kinetics(fractions[0], dt, normal, self.c["kinetic_model"])
+        beta = fractions[1] if len(fractions) > 1 else None
+        lam, reacted, beta = one.kinetics(fractions[0], dt, normal, self.c["kinetic_model"], beta)
         out = state.copy()
         out[-1] += reacted[-1] - normal[-1]
-        return out, [lam]
+        return out, [lam] + ([beta] if len(fractions) > 1 else [])
 
     def step(self):
         from torch_amd.torchAdaptive import remesh
@@ -129,7 +134,8 @@ def d():
     return golden("adaptive")
 
 
-@pytest.mark.parametrize("tag", ["sod1d", "sod2d", "wave1d", "react2d"])
+@pytest.mark.parametrize("tag", ["sod1d", "sod2d", "wave1d", "react2d", "znd2d", "order1", "periodic2d", "faces2d",
+                                 "stag3d"])
 def test_remesh_reproduces_the_reference_runs(d, tag):
     c, p = run_config(d, tag)
     run = HostLeaves(c, p)
@@ -142,6 +148,8 @@ def test_remesh_reproduces_the_reference_runs(d, tag):
         assert abs(run.time - float(d[f"{tag}_time{k}"])) <= 1e-12 * float(d[f"{tag}_time{k}"])
         if f"{tag}_lam{k}" in d:
             assert rel(run.fractions[0], d[f"{tag}_lam{k}"]) <= 1e-11
+        if f"{tag}_beta{k}" in d:
+            assert rel(run.fractions[1], d[f"{tag}_beta{k}"]) <= 1e-11
     assert run.regrids == int(d[tag + "_regrids"])
     for got, want in ((run.boundary, d[tag + "_boundary"]), (run.source, d[tag + "_source"])):
         assert np.max(np.abs(got - want)) <= 1e-11 * max(np.max(np.abs(want)), 1.0), tag

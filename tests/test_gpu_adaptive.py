@@ -4,7 +4,9 @@ the reference's own adaptive invariants.
 Every case runs libtorchhip's adaptive kernels through the C ABI
 (torch_amr_*). FP64 bars: 1e-12 relative per field for rhs, cfl and
 advance on the reference's meshes; 1e-11 after multi-step runs with
-regrids (leaf keys exact); integrals to 1e-11 of their scale. The
+regrids (leaf keys exact; Sod, entropy waves periodic in 1D and 2D,
+reactive Arrhenius/ZND, first order, every face type, 3D stagnation inlet);
+integrals to 1e-11 of their scale. The
 invariant tests follow python/tests/testAdaptive.py:50-229 (rest states in
 every geometry, poles and the radial origin, 2:1 closure across a periodic
 seam, capacity and minimum level, second-order entropy advection).
@@ -124,7 +126,8 @@ def test_fp32_leaves_follow_the_reference(d, ctx):
             dev.close()
 
 
-@pytest.mark.parametrize("tag", ["sod1d", "sod2d", "sod3d", "wave1d", "react2d"])
+@pytest.mark.parametrize("tag", ["sod1d", "sod2d", "sod3d", "wave1d", "react2d", "znd2d", "order1", "periodic2d",
+                                 "faces2d", "stag3d"])
 def test_adaptive_runs_match_the_reference(d, tag):
     from torch_amd import Simulation
 
@@ -143,6 +146,8 @@ def test_adaptive_runs_match_the_reference(d, tag):
             assert abs(s.time - float(d[f"{tag}_time{k}"])) <= 1e-12 * float(d[f"{tag}_time{k}"])
             if f"{tag}_lam{k}" in d:
                 assert rel(s.lam, d[f"{tag}_lam{k}"]) <= 1e-11, (tag, k)
+            if f"{tag}_beta{k}" in d:
+                assert rel(s.beta, d[f"{tag}_beta{k}"]) <= 1e-11, (tag, k)
         assert s.regrids == int(d[tag + "_regrids"])
         assert sums_ok(s.boundary_integral, d[tag + "_boundary"], 1e-11)
         assert sums_ok(s.source_integral, d[tag + "_source"], 1e-11)
