@@ -301,3 +301,59 @@ def test_mesh_validation_and_order_of_calls(d, ctx):
             dev.set_mesh(bad)
     finally:
         dev.close()
+
+
+def test_exact_restart_with_a_changing_mesh(tmp_path):
+    """testAdaptive.py:230-270: save after a step, resume, run both to the end."""
+    from torch_amd import Simulation
+
+    path = tmp_path / "restart.npz"
+    for dim in (1, 2, 3):
+        for model in ("inert", "arrhenius", "znd"):
+            print("restart", dim, model, flush=True)
+            p = np.ones((dim + 2, *([4] * dim)))
+            p[1:-1] = 0.1
+            p[0, ..., 2:] = 0.7
+            reactive = model != "inert"
+            s = Simulation(p, [1] * dim, t_end=0.08, reactive=reactive,
+                           kinetic_model=model if reactive else "arrhenius").to_adaptive(max_level=1, regrid_every=1)
+            s.step()
+            s.save(path)
+            loaded = Simulation.load(path)
+            loaded.set_workers(2)
+            s.run()
+            loaded.run()
+            for attr in ("state", "lam", "beta", "boundary_integral", "source_integral"):
+                a, b = getattr(loaded, attr), getattr(s, attr)
+                if b is None:
+                    assert a is None
+                else:
+                    np.testing.assert_array_equal(a, b)
+            np.testing.assert_array_equal(loaded.mesh.keys, s.mesh.keys)
+            assert (loaded.regrids, loaded.mesh_revision) == (s.regrids, s.mesh_revision)
+            assert s.mesh_revision > 0
+            loaded.close()
+            s.close()
+
+
+def test_a_failed_regrid_leaves_the_step_uncommitted(monkeypatch):
+    """testAdaptive.py:272-291."""
+    from torch_amd import torchAdaptive
+
+    s = uniform()
+    state, keys, time = s.state.copy(), s.mesh.keys.copy(), s.time
+
+    def fail(*args, **kwargs):
+        raise ValueError("failure")
+
+    monkeypatch.setattr(torchAdaptive, "remesh", fail)
+    s.config["amr"]["regrid_every"] = 1
+    with pytest.raises(ValueError):
+        s.step()
+    np.testing.assert_array_equal(s.state, state)
+    np.testing.assert_array_equal(s.mesh.keys, keys)
+    assert s.time == time
+    monkeypatch.undo()
+    s.step()
+    assert s.steps == 1
+    s.close()

@@ -21,7 +21,7 @@ import numpy as np
 from . import _abi
 from . import torchGeometry as _geometry
 from .torchAdaptiveGeometry import LeafMesh, canonical_order, settings, volume
-from .torchExecution import DeviceExecution
+from .torchExecution import DeviceExecution, validate_workers
 from .torchSession import Simulation
 
 
@@ -328,14 +328,13 @@ class AdaptiveSimulation(Simulation):
         self.last_dt = info.dt
         return info
 
-    def run(self, *, step_limit=None, checkpoint=None, checkpoint_every=0, output_prefix=None, output_every=1,
-            formats=("vtk", "xdmf")):
-        if checkpoint or output_prefix:
-            raise NotImplementedError("Adaptive checkpoints and field output are not part of the device backend")
-        return super().run(step_limit=step_limit)
-
-    def save(self, path):
-        raise NotImplementedError("Adaptive checkpoints are not part of the device backend")
+    def _attach(self, workers, device):
+        """Give a host-side shell (a validated schema-3 checkpoint) its device leaves."""
+        state, lam, beta = self._host
+        self._execution = DeviceExecution(validate_workers(workers), device=device, precision=self.precision)
+        self._device, self._leaves, self._leaves_key, self._mesh_on_device = None, None, None, None
+        self._install(self.mesh, state, lam, beta)
+        return self
 
     def __deepcopy__(self, memo):
         state, lam, beta = (None if a is None else a.copy() for a in self._fetch())

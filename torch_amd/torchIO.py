@@ -1,4 +1,5 @@
-"""Checkpoints of device sessions in the reference's schema-2 layout.
+"""Checkpoints of device sessions in the reference's schema-2 layout, and of
+adaptive sessions in its schema-3 leaf layout.
 
 Follows python/cfdSolver/torchIO.py: the metadata record and its checks
 (:66-119), the datasets written per session (:192-214: ``metadata`` as a
@@ -31,6 +32,8 @@ import numpy as np
 
 SCHEMA_VERSION = 2
 SOLVER_VERSION = "1.4"
+ADAPTIVE_SCHEMA = 3
+ADAPTIVE_SOLVER = "2.0"
 # (schema_version, solver_version) pairs the reader accepts (torchIO.py:272-273).
 FAMILIES = ((1, "1.1"), (2, "1.2"), (2, "1.4"))
 REQUIRED = {"config", "time", "steps", "run_id", "initial_integral", "boundary_integral", "source_integral",
@@ -115,8 +118,69 @@ def metadata(session):
     )
 
 
+def _adaptive(session):
+    from .torchAdaptive import AdaptiveSimulation
+
+    return isinstance(session, AdaptiveSimulation)
+
+
+def adaptive_metadata(session):
+    """The schema-3 record of an adaptive session, after the reference's
+    checks (torchAdaptiveIO.py:86-153)."""
+    from .torchSession import _physical
+
+    session._validate_config()
+    c = session.config
+    state = session.state
+    if state.shape != (c["dimension"] + 2, len(session.mesh.keys)) or not _physical(state):
+        raise ValueError("Invalid adaptive state")
+    if (not np.isfinite(session.time) or not 0 <= session.time <= c["t_end"]
+            or not isinstance(session.steps, int) or not 0 <= session.steps <= c["max_steps"]):
+        raise ValueError("Invalid adaptive counters")
+    want_lam = bool(c["reactive"])
+    want_beta = want_lam and c["kinetic_model"] == "znd"
+    if (session.lam is not None) != want_lam or (session.beta is not None) != want_beta:
+        raise ValueError("Adaptive composition/model mismatch")
+    for key in ("steps", "regrids", "mesh_revision"):
+        value = getattr(session, key)
+        if not isinstance(value, int) or isinstance(value, bool) or value < 0:
+            raise ValueError("Invalid adaptive counters")
+    if session.mesh_revision > session.regrids or not isinstance(session.budget_limited, bool):
+        raise ValueError("Invalid adaptive regrid metadata")
+    if not isinstance(session.run_id, str) or not session.run_id or session.initial_integral[0] <= 0:
+        raise ValueError("Invalid adaptive run identity")
+    if session.lam is not None:
+        session._fraction(session.lam, "lambda")
+    if session.beta is not None:
+        session._fraction(session.beta, "beta")
+    for value in (session.initial_integral, session.boundary_integral, session.source_integral):
+        if np.shape(value) != (state.shape[0],) or not np.isfinite(value).all():
+            raise ValueError("Invalid adaptive conservation balances")
+    return dict(
+        format="TORCH checkpoint", schema_version=ADAPTIVE_SCHEMA, solver_version=ADAPTIVE_SOLVER,
+        geometry=c["geometry"], units="SI", layout="leaf-level-z-y-x", config=c, run_id=session.run_id,
+        time=session.time, steps=session.steps, initial_integral=session.initial_integral,
+        boundary_integral=session.boundary_integral, source_integral=session.source_integral,
+        current_integral=session.integral, cell_count=len(session.mesh.keys), regrids=session.regrids,
+        mesh_revision=session.mesh_revision, budget_limited=bool(session.budget_limited),
+        device=dict(precision=session.precision, constants=session.constants, checked=session.checked),
+    )
+
+
 def content(session):
-    """Dataset name -> array of a session's checkpoint (torchIO.py:192-214)."""
+    """Dataset name -> array of a session's checkpoint (torchIO.py:192-214;
+    adaptive sessions: torchAdaptiveIO.py:156-185, leaf levels and indices
+    as little-endian int64)."""
+    if _adaptive(session):
+        data = {"metadata": np.frombuffer(_json(adaptive_metadata(session)).encode("utf-8"), dtype=np.uint8)}
+        for name, row in zip(component_names(session.config), session.state):
+            data["state/" + name] = row.astype("<f8")
+        for name, value in (("lambda", session.lam), ("beta", session.beta)):
+            if value is not None:
+                data["state/" + name] = value.astype("<f8")
+        data["mesh/levels"] = session.mesh.levels.astype("<i8")
+        data["mesh/indices"] = session.mesh.keys[:, 1:].ravel().astype("<i8")
+        return data
     data = {"metadata": np.frombuffer(_json(metadata(session)).encode("utf-8"), dtype=np.uint8)}
     for name, row in zip(component_names(session.config), session.state):
         data["state/" + name] = row.ravel().astype("<f8")
@@ -155,8 +219,9 @@ def load_content(path):
 
             with h5py.File(path, "r") as handle:
                 out = {"metadata": np.asarray(handle["metadata"])}
-                for name in handle["state"]:
-                    out["state/" + name] = np.asarray(handle["state/" + name])
+                for group in ("state", "mesh"):
+                    for name in handle.get(group, ()):
+                        out[group + "/" + name] = np.asarray(handle[group + "/" + name])
                 return out
         with np.load(path, allow_pickle=False) as archive:
             return {name: archive[name] for name in archive.files}
@@ -180,8 +245,8 @@ def record(data):
         meta = json.loads(raw.tobytes().decode("utf-8"))
         if not isinstance(meta, dict) or not REQUIRED.issubset(meta):
             raise ValueError("Incomplete checkpoint metadata")
-        if meta.get("schema_version") == 3:
-            raise ValueError("Adaptive-mesh checkpoints (schema 3) are not part of the device backend")
+        if meta.get("schema_version") == ADAPTIVE_SCHEMA:
+            return adaptive_record(data, meta)
         expected = dict(format="TORCH checkpoint", units="SI", layout="x-fastest")
         if any(meta.get(k) != v for k, v in expected.items()):
             raise ValueError("Unsupported checkpoint schema, solver version, geometry or units")
@@ -256,6 +321,85 @@ def record(data):
         sim._fraction(beta, "beta")
     sim.fallbacks = 0
     sim._host = (state, lam, beta)
+    return sim
+
+
+def adaptive_record(data, meta):
+    """Validate an adaptive (schema 3) checkpoint like the reference's reader
+    (torchAdaptiveIO.py:188-300); returns a host-side AdaptiveSimulation
+    shell (mesh, state, counters and balances; no device state)."""
+    from .torchAdaptive import AdaptiveSimulation
+    from .torchAdaptiveGeometry import LeafMesh, settings
+
+    required = {"cell_count", "regrids", "mesh_revision", "budget_limited", "config", "time", "steps", "run_id",
+                "initial_integral", "boundary_integral", "source_integral", "current_integral"}
+    if not required <= set(meta) or any(meta.get(k) != v for k, v in dict(
+            format="TORCH checkpoint", solver_version=ADAPTIVE_SOLVER, units="SI",
+            layout="leaf-level-z-y-x").items()):
+        raise ValueError("Unsupported adaptive checkpoint metadata")
+    sim = AdaptiveSimulation.__new__(AdaptiveSimulation)
+    sim._device = sim._leaves = sim._leaves_key = sim._mesh_on_device = sim._execution = None
+    sim.config = c = meta["config"]
+    if not isinstance(c, dict) or meta.get("geometry") != c.get("geometry") or "amr" not in c \
+            or "face_boundaries" not in c:
+        raise ValueError("Adaptive geometry/config mismatch")
+    device = meta.get("device") or {}
+    sim.precision = str(device.get("precision", "fp64")).lower()
+    sim.checked = bool(device.get("checked", True))
+    sim.constants = dict(device.get("constants") or {})
+    sim.tile = (0, 0, 0)
+    c["amr"] = settings(**c["amr"])
+    for key in ("counts_xyz", "lengths_xyz", "origins_xyz", "boundaries_xyz"):
+        c[key] = list(np.atleast_1d(c[key]))
+    sim._validate_config()
+    n, d = meta["cell_count"], c["dimension"]
+    if not isinstance(n, int) or isinstance(n, bool) or not 0 < n <= c["amr"]["max_cells"]:
+        raise ValueError("Invalid adaptive cell count")
+
+    def array(path, length, kind):
+        value = np.asarray(data[path])
+        if value.shape != (length,) or value.dtype.kind != kind or value.dtype.itemsize != 8:
+            raise ValueError("Invalid adaptive dataset layout or precision")
+        if not np.isfinite(value).all():
+            raise ValueError("Nonfinite adaptive dataset")
+        return value
+
+    levels = array("mesh/levels", n, "i")
+    indices = array("mesh/indices", n * d, "i").reshape(n, d)
+    keys = np.column_stack((levels, indices)).astype(np.int64)
+    sim.mesh = LeafMesh(c, keys)
+    if not np.array_equal(sim.mesh.keys, keys) or np.any(levels < c["amr"]["min_level"]):
+        raise ValueError("Noncanonical or undersized adaptive leaves")
+    names = component_names(c)
+    expected = set(names) | ({"lambda"} if c["reactive"] else set()) | (
+        {"beta"} if c["reactive"] and c["kinetic_model"] == "znd" else set())
+    if {k[len("state/"):] for k in data if k.startswith("state/")} != expected:
+        raise ValueError("Unexpected adaptive state fields")
+    state = np.stack([array("state/" + name, n, "f").astype(float) for name in names])
+    lam = array("state/lambda", n, "f").astype(float) if c["reactive"] else None
+    beta = array("state/beta", n, "f").astype(float) if "beta" in expected else None
+    for key in ("steps", "regrids", "mesh_revision"):
+        value = meta[key]
+        if not isinstance(value, int) or isinstance(value, bool) or value < 0:
+            raise ValueError("Invalid adaptive step/regrid counters")
+        setattr(sim, key, value)
+    sim.time, sim.run_id, sim.budget_limited = meta["time"], meta["run_id"], meta["budget_limited"]
+    if not isinstance(sim.budget_limited, bool) or sim.mesh_revision > sim.regrids:
+        raise ValueError("Invalid adaptive regrid metadata")
+    for key in ("initial_integral", "boundary_integral", "source_integral"):
+        value = np.asarray(meta[key], dtype=float)
+        if value.shape != (d + 2,) or not np.isfinite(value).all():
+            raise ValueError("Invalid adaptive balances")
+        setattr(sim, key, value)
+    if not isinstance(sim.run_id, str) or not sim.run_id or sim.initial_integral[0] <= 0:
+        raise ValueError("Invalid adaptive run identity")
+    sim.fallbacks = 0
+    sim._host = (state, lam, beta)
+    adaptive_metadata(sim)
+    current = np.asarray(meta["current_integral"], dtype=float)
+    if current.shape != (d + 2,) or not np.isfinite(current).all() or not np.allclose(
+            current, sim.integral, rtol=1e-12, atol=1e-12):
+        raise ValueError("Adaptive state/integral mismatch")
     return sim
 
 
