@@ -25,6 +25,11 @@ struct torch_amr {
     long cap_leaf = 0, cap_face = 0, cap_entry = 0;
     bool has_mesh = false, has_state = false;
     int cur = 0;
+    // The scratch arrays hold the checked evaluation (face values, gradients,
+    // fluxes, outward sums) of slot cache_slot with reconstruction
+    // cache_rec; -1 when they hold none. A stage whose input is that slot
+    // reuses it (the step's CFL evaluation is its first stage's).
+    int cache_slot = -1, cache_rec = -1;
     void* U[4] = {nullptr, nullptr, nullptr, nullptr};
     void* q[2][4] = {{nullptr, nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr, nullptr}};
     // mesh
@@ -204,13 +209,26 @@ fvl::Grid flat_grid(const torch_amr* a) {
 void* frac(torch_amr* a, int slot, int j) { return j < a->nq ? a->q[j][slot] : nullptr; }
 
 // Face values, gradients and fluxes of slot u; outward sums into scalars[0].
+// Nothing to launch when the scratch already holds this evaluation.
 int evaluate(torch_amr* a, int u, int reconstruct) {
+    if (a->cache_slot == u && a->cache_rec == reconstruct) return TORCH_OK;
+    a->cache_slot = -1;
     hipStream_t st = a->ctx->stream;
     const fvl::Amr m = mesh_of(a);
     HIP_TRY(a->ops->amr_eval(a->cfg.dim, m, a->U[u], frac(a, u, 0), frac(a, u, 1), a->nq, a->cfg.gamma,
                              a->cfg.reactive, reconstruct, a->flags, region(a, 0), st));
     HIP_TRY(fvl::finalize(region(a, 0), a->ops->amr_blocks(a->nface), a->scalars, st));
     return TORCH_OK;
+}
+
+void remember(torch_amr* a, int u, int reconstruct) {
+    a->cache_slot = u;
+    a->cache_rec = reconstruct;
+}
+
+// A slot is about to be overwritten: its cached evaluation goes stale.
+void overwrite(torch_amr* a, int slot) {
+    if (a->cache_slot == slot) a->cache_slot = -1;
 }
 
 // cfl (torchAdaptive.py:149-166) of slot u.
@@ -223,6 +241,7 @@ int leaf_dt(torch_amr* a, int u, double* dt) {
     HIP_TRY(fvl::finalize(region(a, 2), a->ops->amr_blocks(a->nleaf), a->scalars + 2 * fvl::NSLOT, st));
     if ((rc = sync_read(a))) return rc;
     if ((rc = boundary_error(host_flags(a)))) return rc;
+    remember(a, u, a->order == 2);
     *dt = a->cfg.cfl / a->host[2 * fvl::NSLOT];
     return TORCH_OK;
 }
@@ -249,14 +268,18 @@ int stage(torch_amr* a, int mode, int S, int U0, int out, double dt, int first_o
     m.flags = a->flags;
     m.partials = region(a, 1);
     hipStream_t st = a->ctx->stream;
+    const int rec = a->order == 2 && !first_order;
     HIP_TRY(clear(a));
-    int rc = evaluate(a, S, a->order == 2 && !first_order);
+    int rc = evaluate(a, S, rec);
     if (rc) return rc;
+    overwrite(a, out);
     HIP_TRY(a->ops->amr_update(a->cfg.dim, mesh_of(a), m, st));
     HIP_TRY(fvl::finalize(region(a, 1), a->ops->amr_blocks(a->nleaf), a->scalars + fvl::NSLOT, st));
     if ((rc = sync_read(a))) return rc;
     *flags = host_flags(a);
-    return boundary_error(*flags);
+    if ((rc = boundary_error(*flags))) return rc;
+    remember(a, S, rec);
+    return TORCH_OK;
 }
 
 // advance (torchAdaptive.py:169-206): slot `in` -> `out` through `mid`.
@@ -286,6 +309,7 @@ int advance(torch_amr* a, int in, int mid, int out, double dt, double* boundary,
 // Chemistry half-step of slot `in` into `out` (torchAdaptive.py:418-434):
 // the leaves as one flat array of cells.
 int react(torch_amr* a, int in, int out, double dt) {
+    overwrite(a, out);
     fvl::React r;
     memset(&r, 0, sizeof(r));
     r.U = a->U[in];
@@ -462,6 +486,7 @@ int torch_amr_set_mesh(torch_amr* a, long nleaf, long nface, const int* left, co
     a->nface = nface;
     a->nentry = nentry;
     a->cur = 0;
+    a->cache_slot = -1;
     a->has_mesh = true;
     return TORCH_OK;
 }
@@ -473,6 +498,7 @@ int torch_amr_upload(torch_amr* a, const double* U, const double* lam, const dou
     std::lock_guard<std::mutex> lock(a->ctx->mu);
     HIP_TRY(hipSetDevice(a->ctx->device));
     hipStream_t st = a->ctx->stream;
+    overwrite(a, a->cur);
     int rc = upload_as(a->ops, st, a->U[a->cur], U, a->nf * a->nleaf);
     if (!rc && a->nq > 0) rc = upload_as(a->ops, st, a->q[0][a->cur], lam, a->nleaf);
     if (!rc && a->nq > 1) {
@@ -510,6 +536,7 @@ int torch_amr_faces(torch_amr* a, double* left, double* right, double* gl, doubl
     if ((rc = evaluate(a, a->cur, 0))) return rc;
     if ((rc = sync_read(a))) return rc;
     if ((rc = boundary_error(host_flags(a)))) return rc;
+    remember(a, a->cur, 0);
     hipStream_t st = a->ctx->stream;
     const long F = a->nface;
     const size_t e = a->ops->elem;
