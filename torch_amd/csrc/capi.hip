@@ -203,6 +203,7 @@ struct torch_state {
     bool has_metric = false;
     fvl::Metric metric;
     double* mfactors = nullptr;
+    void* mprim = nullptr;       // primitives + gamma scratch of the metric kernels
 };
 
 namespace {
@@ -552,6 +553,7 @@ int torch_state_destroy(torch_state* s) {
     }
     for (int d = 0; d < 3; ++d) (void)hipFree(s->F[d]);
     (void)hipFree(s->mfactors);
+    (void)hipFree(s->mprim);
     (void)hipFree(s->scratch1d);
     (void)hipFree(s->partials);
     if (s->ev0) (void)hipEventDestroy(s->ev0);
@@ -869,6 +871,10 @@ int torch_state_set_metric(torch_state* s, int geometry, const double* factors, 
         int rc = alloc((void**)&s->mfactors, need * sizeof(double));
         if (rc) return rc;
     }
+    if (!s->mprim) {
+        int rc = alloc(&s->mprim, (s->nf + 1) * s->ncell * e);
+        if (rc) return rc;
+    }
     HIP_TRY(hipMemcpyAsync(s->mfactors, factors, need * sizeof(double), hipMemcpyHostToDevice, s->ctx->stream));
     HIP_TRY(hipStreamSynchronize(s->ctx->stream));
     fvl::Metric& m = s->metric;
@@ -884,6 +890,7 @@ int torch_state_set_metric(torch_state* s, int geometry, const double* factors, 
             at += n[a] + (a == d);
         }
     m.geometry = geometry;
+    m.prim = s->mprim;
     for (int f = 0; f < 2 * dim; ++f) {
         fvl::MFace& o = m.face[f];
         const torch_face& i = faces[f];

@@ -63,6 +63,7 @@ struct Metric {
     const double* af[3][3];   // area factors of direction d over axis a (n_a, or n_d + 1 when a == d)
     MFace face[6];            // x_min, x_max, y_min, y_max, z_min, z_max
     int geometry;
+    void* prim;               // (dim+3) x ncell scratch: primitives, then gamma, of the state in use
 };
 // One SSPRK2 stage of the metric scheme (torchFiniteVolume.py:232-276).
 // mode 0 writes the semi-discrete change k (and species change) itself;

@@ -61,6 +61,7 @@ struct Ctx {
     long N;
     int nq, reactive;
     real gamma;
+    const real* P = nullptr;  // primitives then gamma per cell (k_metric_prim), for load_prim
 };
 
 template <int NF>
@@ -100,6 +101,30 @@ __device__ __forceinline__ Pad<NF> load_cell(const Ctx& c, long i) {
     e.q[0] = c.nq > 0 ? c.q[0][i] : (real)0;
     e.q[1] = c.nq > 1 ? c.q[1][i] : (real)0;
     return e;
+}
+
+// The same entry from the precomputed primitives (c.P).
+template <int NF>
+__device__ __forceinline__ Pad<NF> load_prim(const Ctx& c, long i) {
+    Pad<NF> e;
+#pragma unroll
+    for (int k = 0; k < NF; ++k) e.p[k] = c.P[k * c.N + i];
+    e.g = c.P[NF * c.N + i];
+    e.q[0] = c.nq > 0 ? c.q[0][i] : (real)0;
+    e.q[1] = c.nq > 1 ? c.q[1][i] : (real)0;
+    return e;
+}
+
+// Primitives and gamma of every cell, once per state, for the stencil
+// kernels (each cell is an entry of up to fifteen padded stencils).
+template <int NF>
+__global__ void __launch_bounds__(256) k_metric_prim(Ctx c, real* out) {
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= c.N) return;
+    const Pad<NF> e = load_cell<NF>(c, i);
+#pragma unroll
+    for (int k = 0; k < NF; ++k) out[k * c.N + i] = e.p[k];
+    out[NF * c.N + i] = e.g;
 }
 
 // Boundary-plane state from the edge cell (torchBoundaries.py:177-246);
@@ -171,11 +196,11 @@ __device__ __forceinline__ void ghost(Pad<NF>& e, const MFace& f, int side, int 
 // Padded entry jj in [0, n + 1] of a line (torchFiniteVolume.py:33-63).
 template <int NF, int D>
 __device__ __forceinline__ Pad<NF> padded(const Ctx& c, const fvl::Metric& m, const Line& L, int jj, int& bad) {
-    if (jj >= 1 && jj <= L.n) return load_cell<NF>(c, L.base + (jj - 1) * L.stride);
+    if (jj >= 1 && jj <= L.n) return load_prim<NF>(c, L.base + (jj - 1) * L.stride);
     const bool low = jj == 0;
     const MFace& f = m.face[2 * D + (low ? 0 : 1)];
-    if (f.type == FACE_PERIODIC) return load_cell<NF>(c, L.base + (low ? L.n - 1 : 0) * L.stride);
-    Pad<NF> e = load_cell<NF>(c, L.base + (low ? 0 : L.n - 1) * L.stride);
+    if (f.type == FACE_PERIODIC) return load_prim<NF>(c, L.base + (low ? L.n - 1 : 0) * L.stride);
+    Pad<NF> e = load_prim<NF>(c, L.base + (low ? 0 : L.n - 1) * L.stride);
     ghost<NF, D>(e, f, low ? -1 : 1, c.nq, c.reactive, bad);
     return e;
 }
@@ -472,10 +497,11 @@ __global__ void __launch_bounds__(MT, 2) k_metric_stage(fvl::Grid g, fvl::Metric
         c.nq = s.nq;
         c.reactive = s.reactive;
         c.gamma = (real)s.gamma;
+        c.P = (const real*)m.prim;
         int idx[3];
         cell_coords(g, i, idx);
         const double V = volume_at<DIM>(m, idx);
-        const Pad<NF> me = load_cell<NF>(c, i);
+        const Pad<NF> me = load_prim<NF>(c, i);
         double rad = 0, cot = 0;
         if (m.geometry != GEO_CARTESIAN) {
             rad = (area_at<DIM, 0>(m, idx, idx[0] + 1) - area_at<DIM, 0>(m, idx, idx[0])) / V;
@@ -535,6 +561,7 @@ __global__ void __launch_bounds__(MT) k_metric_cfl(fvl::Grid g, fvl::Metric m, c
         c.nq = 0;
         c.reactive = reactive;
         c.gamma = (real)gamma;
+        c.P = (const real*)m.prim;
         int idx[3];
         cell_coords(g, i, idx);
         const real V = (real)volume_at<DIM>(m, idx);
@@ -943,8 +970,28 @@ int metric_blocks(const fvl::Grid& g) {
     return (int)((N + MT - 1) / MT);
 }
 
+hipError_t metric_prims(const fvl::Grid& g, const fvl::Metric& m, const void* S, const void* q0, const void* q1,
+                        int nq, double gamma, int reactive, hipStream_t st) {
+    Ctx c;
+    c.S = (const real*)S;
+    c.q[0] = (const real*)q0;
+    c.q[1] = (const real*)q1;
+    c.N = (long)g.nx * g.ny * g.nz;
+    c.nq = nq;
+    c.reactive = reactive;
+    c.gamma = (real)gamma;
+    const int blocks = (int)((c.N + 255) / 256);
+    real* out = (real*)m.prim;
+    if (g.dim == 1) k_metric_prim<3><<<blocks, 256, 0, st>>>(c, out);
+    else if (g.dim == 2) k_metric_prim<4><<<blocks, 256, 0, st>>>(c, out);
+    else k_metric_prim<5><<<blocks, 256, 0, st>>>(c, out);
+    return hipGetLastError();
+}
+
 hipError_t metric_stage(const fvl::Grid& g, const fvl::Metric& m, const fvl::MStage& s, hipStream_t st) {
     const int blocks = metric_blocks(g);
+    hipError_t e = metric_prims(g, m, s.S, s.qS[0], s.qS[1], s.nq, s.gamma, s.reactive, st);
+    if (e != hipSuccess) return e;
     if (g.dim == 1) return launch_stage<1>(g, m, s, blocks, st);
     if (g.dim == 2) return launch_stage<2>(g, m, s, blocks, st);
     return launch_stage<3>(g, m, s, blocks, st);
@@ -955,6 +1002,8 @@ hipError_t metric_cfl(const fvl::Grid& g, const fvl::Metric& m, const void* U, c
     const int blocks = metric_blocks(g);
     const real* u = (const real*)U;
     const real* l = (const real*)lam;
+    hipError_t e = metric_prims(g, m, U, lam, nullptr, 0, gamma, reactive, st);
+    if (e != hipSuccess) return e;
     if (g.dim == 1) k_metric_cfl<1><<<blocks, MT, 0, st>>>(g, m, u, l, gamma, reactive, partials, flags);
     else if (g.dim == 2) k_metric_cfl<2><<<blocks, MT, 0, st>>>(g, m, u, l, gamma, reactive, partials, flags);
     else k_metric_cfl<3><<<blocks, MT, 0, st>>>(g, m, u, l, gamma, reactive, partials, flags);
