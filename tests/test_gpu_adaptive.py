@@ -299,6 +299,16 @@ def test_mesh_validation_and_order_of_calls(d, ctx):
         bad.volumes[0] = 0.0
         with pytest.raises(ValueError):
             dev.set_mesh(bad)
+        bad.volumes = mesh.volumes.copy()
+        start, faces, roles = mesh.incidence()
+        cut = start[1] - 1  # the first leaf's last face side goes missing
+        short = start.copy()
+        short[1:] -= 1
+        bad.incidence = lambda: (short, np.delete(faces, cut), np.delete(roles, cut))
+        with pytest.raises(ValueError, match="misses"):
+            dev.set_mesh(bad)
+        bad.incidence = mesh.incidence
+        dev.set_mesh(bad)  # the repaired copy installs
     finally:
         dev.close()
 

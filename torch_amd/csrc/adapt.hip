@@ -105,13 +105,12 @@ void release_buffers(torch_amr* a) {
         release(&a->q[0][i]);
         release(&a->q[1][i]);
     }
-    release_t(&a->left), release_t(&a->right), release_t(&a->dir), release_t(&a->side);
-    release_t(&a->cface), release_t(&a->crole), release_t(&a->start);
-    release_t(&a->area), release_t(&a->weight), release_t(&a->offl), release_t(&a->offr), release_t(&a->disp);
-    release_t(&a->volume), release_t(&a->div), release_t(&a->inverse);
-    release(&a->FL), release(&a->FR), release(&a->G), release(&a->Q), release(&a->grad), release(&a->W);
-    release(&a->M), release(&a->S);
-    release_t(&a->partials);
+    for (int** p : {&a->left, &a->right, &a->dir, &a->side, &a->cface, &a->crole}) release_t(p);
+    release_t(&a->start);
+    for (double** p : {&a->area, &a->weight, &a->offl, &a->offr, &a->disp, &a->volume, &a->div, &a->inverse,
+                       &a->partials})
+        release_t(p);
+    for (void** p : {&a->FL, &a->FR, &a->G, &a->Q, &a->grad, &a->W, &a->M, &a->S}) release(p);
     a->cap_leaf = a->cap_face = a->cap_entry = 0;
 }
 
@@ -448,14 +447,23 @@ int torch_amr_set_mesh(torch_amr* a, long nleaf, long nface, const int* left, co
     for (long i = 0; i < nleaf; ++i)
         if (!(volume[i] > 0)) return fail(TORCH_INVALID, "adaptive leaf volumes must be positive");
     if (start[0] != 0 || start[nleaf] != nentry) return fail(TORCH_INVALID, "adaptive incidence starts do not match");
+    // Every entry names a face of its leaf in that role, and every side of
+    // every face that has a leaf appears exactly once.
+    std::vector<unsigned char> seen(nface, 0);
     for (long i = 0; i < nleaf; ++i) {
         if (start[i + 1] < start[i]) return fail(TORCH_INVALID, "adaptive incidence starts must not decrease");
         for (long e = start[i]; e < start[i + 1]; ++e) {
             const int f = cface[e], role = crole[e];
             if (f < 0 || f >= nface || (role != -1 && role != 1) || (role < 0 ? left[f] : right[f]) != i)
                 return fail(TORCH_INVALID, "adaptive incidence entry does not match the face list");
+            const unsigned char bit = role < 0 ? 1 : 2;
+            if (seen[f] & bit) return fail(TORCH_INVALID, "adaptive incidence lists a face side twice");
+            seen[f] |= bit;
         }
     }
+    for (long f = 0; f < nface; ++f)
+        if (seen[f] != ((left[f] >= 0 ? 1 : 0) | (right[f] >= 0 ? 2 : 0)))
+            return fail(TORCH_INVALID, "adaptive incidence misses a face side");
     std::lock_guard<std::mutex> lock(a->ctx->mu);
     HIP_TRY(hipSetDevice(a->ctx->device));
     HIP_TRY(hipStreamSynchronize(a->ctx->stream));

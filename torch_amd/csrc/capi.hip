@@ -204,6 +204,7 @@ struct torch_state {
     fvl::Metric metric;
     double* mfactors = nullptr;
     void* mprim = nullptr;       // primitives + gamma scratch of the metric kernels
+    void* mfaces = nullptr;      // face fluxes of a metric stage
 };
 
 namespace {
@@ -554,6 +555,7 @@ int torch_state_destroy(torch_state* s) {
     for (int d = 0; d < 3; ++d) (void)hipFree(s->F[d]);
     (void)hipFree(s->mfactors);
     (void)hipFree(s->mprim);
+    (void)hipFree(s->mfaces);
     (void)hipFree(s->scratch1d);
     (void)hipFree(s->partials);
     if (s->ev0) (void)hipEventDestroy(s->ev0);
@@ -875,6 +877,14 @@ int torch_state_set_metric(torch_state* s, int geometry, const double* factors, 
         int rc = alloc(&s->mprim, (s->nf + 1) * s->ncell * e);
         if (rc) return rc;
     }
+    // Face fluxes of one stage, per direction: (dim+2) weighted fluxes and
+    // two moved fractions per face.
+    long nfaces[3] = {0, 0, 0}, total = 0;
+    for (int d = 0; d < dim; ++d) total += (nfaces[d] = face_count(s->grid, d));
+    if (!s->mfaces) {
+        int rc = alloc(&s->mfaces, (s->nf + 2) * total * e);
+        if (rc) return rc;
+    }
     HIP_TRY(hipMemcpyAsync(s->mfactors, factors, need * sizeof(double), hipMemcpyHostToDevice, s->ctx->stream));
     HIP_TRY(hipStreamSynchronize(s->ctx->stream));
     fvl::Metric& m = s->metric;
@@ -891,6 +901,8 @@ int torch_state_set_metric(torch_state* s, int geometry, const double* factors, 
         }
     m.geometry = geometry;
     m.prim = s->mprim;
+    for (long d = 0, off = 0; d < dim; off += (s->nf + 2) * nfaces[d] * (long)e, ++d)
+        m.wf[d] = (char*)s->mfaces + off;
     for (int f = 0; f < 2 * dim; ++f) {
         fvl::MFace& o = m.face[f];
         const torch_face& i = faces[f];

@@ -64,6 +64,7 @@ struct Metric {
     MFace face[6];            // x_min, x_max, y_min, y_max, z_min, z_max
     int geometry;
     void* prim;               // (dim+3) x ncell scratch: primitives, then gamma, of the state in use
+    void* wf[3];              // per direction: (dim+4) x nface weighted fluxes, then moved fractions
 };
 // One SSPRK2 stage of the metric scheme (torchFiniteVolume.py:232-276).
 // mode 0 writes the semi-discrete change k (and species change) itself;

@@ -4,20 +4,22 @@
 // (python/cfdSolver/torchFiniteVolume.py:16-276, torchGeometry.py:93-213,
 // torchBoundaries.py:177-246, torchFlux.py:80-122).
 //
-// One thread per cell. For each direction a thread loads the five-entry
-// padded stencil of its cell and evaluates the cell's two faces: each
-// face's left/right states come from the padded line
-// (interior cells plus one boundary plane per end), with limited slopes of
-// the primitives (zeroed where half a slope would take density or pressure
-// to <= 1e-12; wrapped on periodic axes) and mirrored states on walls. A
-// face flux is a pure function of the line, so the two cells that share a
-// face compute it bit for bit alike, and the area-weighted divergence is
-// the reference's diff / volume. The outward boundary flux sums the
-// weighted fluxes of each line's last face minus its first (the threads of
-// the end cells contribute them). Kernels are instantiated per dimension so
-// every component index is a compile-time constant. Volumes, areas and the
-// geometric sources come from per-axis factor tables (outer products in the
-// reference's order).
+// A pass converts its state to primitives once per cell. A stage then takes
+// one thread per face along each direction: the face's left/right states
+// come from the padded line (interior cells plus one boundary plane per
+// end), with limited slopes of the primitives (zeroed where half a slope
+// would take density or pressure to <= 1e-12; wrapped on periodic axes) and
+// mirrored states on walls; the Rusanov flux times the face area is stored
+// once, and one thread per cell forms the reference's diff / volume from
+// its stored faces, the geometric source and the SSPRK2 stage. The outward
+// boundary flux sums the weighted fluxes of each line's last face minus its
+// first (the threads of the end cells contribute them). Kernels are
+// instantiated per dimension so every component index is a compile-time
+// constant. Volumes, areas and the geometric sources come from per-axis
+// factor tables (outer products in the reference's order).
+//
+// The adaptive leaves (further down) reuse the face states, boundary planes,
+// Rusanov flux and stage output on a leaf mesh with an explicit face list.
 #include "fv_device.h"
 #include "fv_launch.h"
 
@@ -377,66 +379,6 @@ __device__ __forceinline__ Line line_of(const fvl::Grid& g, long i, const int* i
     return L;
 }
 
-// Change of one cell along direction D: the two faces of the cell from the
-// padded stencil jc-2..jc+2 (jc = at + 1), with the periodic slope wrap of
-// the boundary entries, evaluated face by face to keep few entries live;
-// accumulates -(W_hi - W_lo) / V into k, the species
-// change into kq and the boundary faces' weighted fluxes into out_sum.
-template <int DIM, int D, int LIM>
-__device__ __forceinline__ void direction(const Ctx& c, const fvl::Metric& m, const fvl::Grid& g, long i,
-                                          const int* idx, int first_order, real Vr, real* k, real* kq,
-                                          double* out_sum, int& bad) {
-    constexpr int NF = DIM + 2;
-    const Line L = line_of<D>(g, i, idx);
-    const int at = idx[D], jc = at + 1, n = L.n;
-    const bool periodic = m.face[2 * D].type == FACE_PERIODIC;
-    const bool wall_lo = at == 0 && m.face[2 * D].type == FACE_WALL;
-    const bool wall_hi = at == n - 1 && m.face[2 * D + 1].type == FACE_WALL;
-    // Stencil entries jc-1, jc, jc+1 (always inside [0, n+1]); the outer
-    // two (jc-2, jc+2) only feed a slope and are loaded where needed.
-    const Pad<NF> Pm = padded<NF, D>(c, m, L, jc - 1, bad);
-    const Pad<NF> P0 = padded<NF, D>(c, m, L, jc, bad);
-    const Pad<NF> Pp = padded<NF, D>(c, m, L, jc + 1, bad);
-    real sm[NF], s0[NF], sp[NF];
-#pragma unroll
-    for (int q = 0; q < NF; ++q) sm[q] = s0[q] = sp[q] = 0;
-    if (!first_order) {
-        slope3<NF, LIM>(Pm, P0, Pp, s0);
-        if (jc - 1 >= 1) {
-            slope3<NF, LIM>(padded<NF, D>(c, m, L, jc - 2, bad), Pm, P0, sm);
-        } else if (periodic) {  // entry 0 takes the slope of entry n
-            slope3<NF, LIM>(padded<NF, D>(c, m, L, n - 1, bad), padded<NF, D>(c, m, L, n, bad),
-                            padded<NF, D>(c, m, L, n + 1, bad), sm);
-        }
-    }
-    real Flo[NF], Fhi[NF];
-    face<NF, D>(Pm, sm, P0, s0, wall_lo, false, Flo);
-    if (!first_order) {
-        if (jc + 1 <= n) {
-            slope3<NF, LIM>(P0, Pp, padded<NF, D>(c, m, L, jc + 2, bad), sp);
-        } else if (periodic) {  // entry n + 1 takes the slope of entry 1
-            slope3<NF, LIM>(padded<NF, D>(c, m, L, 0, bad), padded<NF, D>(c, m, L, 1, bad),
-                            padded<NF, D>(c, m, L, 2, bad), sp);
-        }
-    }
-    face<NF, D>(P0, s0, Pp, sp, false, wall_hi, Fhi);
-    const real Alo = (real)area_at<DIM, D>(m, idx, at), Ahi = (real)area_at<DIM, D>(m, idx, at + 1);
-#pragma unroll
-    for (int f = 0; f < NF; ++f) {
-        const real wlo = Flo[f] * Alo, whi = Fhi[f] * Ahi;
-        k[f] = k[f] - (whi - wlo) / Vr;
-        if (at == n - 1) out_sum[f] += (double)whi;
-        if (at == 0) out_sum[f] -= (double)wlo;
-    }
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-        if (j >= c.nq) break;
-        const real mlo = (Flo[0] * (Flo[0] >= (real)0 ? Pm.q[j] : P0.q[j])) * Alo;
-        const real mhi = (Fhi[0] * (Fhi[0] >= (real)0 ? P0.q[j] : Pp.q[j])) * Ahi;
-        kq[j] = kq[j] - (mhi - mlo) / Vr;
-    }
-}
-
 // Stage output of cell i from its change k and species change kq (SSPRK2,
 // torchFiniteVolume.py:232-276 and torchAdaptive.py:169-206): mode 0 stores
 // the change, mode 1 S + dt k, mode 2 ((U0 + S) + dt k) / 2, fractions from
@@ -481,8 +423,120 @@ __device__ __forceinline__ void stage_out(const fvl::MStage& s, long N, long i, 
         if (j < s.nq) ((real*)s.qout[j])[i] = qo[j];
 }
 
-template <int DIM, int LIM>
-__global__ void __launch_bounds__(MT, 2) k_metric_stage(fvl::Grid g, fvl::Metric m, fvl::MStage s) {
+// Faces along direction D: their counts per axis and the face index of a
+// cell's low (at) or high (at + 1) face.
+template <int D>
+__device__ __forceinline__ long face_count_of(const fvl::Grid& g) {
+    return (long)(g.nx + (D == 0)) * (g.ny + (D == 1)) * (g.nz + (D == 2));
+}
+
+template <int D>
+__device__ __forceinline__ long face_index(const fvl::Grid& g, const int* idx, int at) {
+    const int fx = D == 0 ? at : idx[0], fy = D == 1 ? at : idx[1], fz = D == 2 ? at : idx[2];
+    return ((long)fz * (g.ny + (D == 1)) + fy) * (g.nx + (D == 0)) + fx;
+}
+
+// Limited slope of padded entry jj of a line (torchFiniteVolume.py:143-160):
+// interior entries from their neighbours; the boundary entries take the
+// slope of the entry they wrap to on a periodic axis, none otherwise.
+template <int NF, int D, int LIM>
+__device__ __forceinline__ void entry_slope(const Ctx& c, const fvl::Metric& m, const Line& L, int jj,
+                                            const Pad<NF>& e, const Pad<NF>* prev, const Pad<NF>* next, real* s,
+                                            int& bad) {
+    const int n = L.n;
+#pragma unroll
+    for (int q = 0; q < NF; ++q) s[q] = 0;
+    if (jj >= 1 && jj <= n) {
+        const Pad<NF> a = prev ? *prev : padded<NF, D>(c, m, L, jj - 1, bad);
+        const Pad<NF> z = next ? *next : padded<NF, D>(c, m, L, jj + 1, bad);
+        slope3<NF, LIM>(a, e, z, s);
+    } else if (m.face[2 * D].type == FACE_PERIODIC) {
+        const int w = jj == 0 ? n : 1;
+        slope3<NF, LIM>(padded<NF, D>(c, m, L, w - 1, bad), padded<NF, D>(c, m, L, w, bad),
+                        padded<NF, D>(c, m, L, w + 1, bad), s);
+    }
+}
+
+// One face along D (torchFiniteVolume.py:163-229): the padded entries either
+// side (fa, fa + 1) with their limited slopes, the wall mirror on a wall
+// face, the Rusanov flux times the face area, and the fractions it moves.
+// Both cells of a face read the same stored values.
+template <int DIM, int D, int LIM>
+__global__ void __launch_bounds__(MT) k_metric_faces(fvl::Grid g, fvl::Metric m, fvl::MStage s) {
+    constexpr int NF = DIM + 2;
+    const long nfD = face_count_of<D>(g);
+    const long f = blockIdx.x * (long)MT + threadIdx.x;
+    int bad = 0;
+    if (f < nfD) {
+        Ctx c;
+        c.S = (const real*)s.S;
+        c.q[0] = (const real*)s.qS[0];
+        c.q[1] = (const real*)s.qS[1];
+        c.N = (long)g.nx * g.ny * g.nz;
+        c.nq = s.nq;
+        c.reactive = s.reactive;
+        c.gamma = (real)s.gamma;
+        c.P = (const real*)m.prim;
+        const int wx = g.nx + (D == 0), wy = g.ny + (D == 1);
+        int idx[3] = {(int)(f % wx), (int)((f / wx) % wy), (int)(f / ((long)wx * wy))};
+        const int fa = idx[D];
+        idx[D] = 0;
+        const Line L = line_of<D>(g, (long)idx[0] + (long)g.nx * (idx[1] + (long)g.ny * idx[2]), idx);
+        const int n = L.n;
+        const Pad<NF> A = padded<NF, D>(c, m, L, fa, bad);
+        const Pad<NF> B = padded<NF, D>(c, m, L, fa + 1, bad);
+        real sa[NF], sb[NF];
+#pragma unroll
+        for (int q = 0; q < NF; ++q) sa[q] = sb[q] = 0;
+        if (!s.first_order) {
+            entry_slope<NF, D, LIM>(c, m, L, fa, A, nullptr, &B, sa, bad);
+            entry_slope<NF, D, LIM>(c, m, L, fa + 1, B, &A, nullptr, sb, bad);
+        }
+        const bool wall_lo = fa == 0 && m.face[2 * D].type == FACE_WALL;
+        const bool wall_hi = fa == n && m.face[2 * D + 1].type == FACE_WALL;
+        real F[NF];
+        face<NF, D>(A, sa, B, sb, wall_lo, wall_hi, F);
+        const real area = (real)area_at<DIM, D>(m, idx, fa);
+        real* W = (real*)m.wf[D];
+#pragma unroll
+        for (int v = 0; v < NF; ++v) W[v * nfD + f] = F[v] * area;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            if (j >= s.nq) break;
+            W[(NF + j) * nfD + f] = (F[0] * (F[0] >= (real)0 ? A.q[j] : B.q[j])) * area;
+        }
+    }
+    if (bad) atomicOr(s.flags, bad);
+}
+
+// Change of one cell along D from its two stored faces: -(W_hi - W_lo) / V
+// into k, the species change into kq, the boundary faces into out_sum.
+template <int DIM, int D>
+__device__ __forceinline__ void gather(const fvl::Metric& m, const fvl::Grid& g, const int* idx, int nq, real Vr,
+                                       real* k, real* kq, double* out_sum) {
+    constexpr int NF = DIM + 2;
+    const long nfD = face_count_of<D>(g);
+    const int at = idx[D], n = D == 0 ? g.nx : (D == 1 ? g.ny : g.nz);
+    const long lo = face_index<D>(g, idx, at), hi = face_index<D>(g, idx, at + 1);
+    const real* W = (const real*)m.wf[D];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+        const real wlo = W[f * nfD + lo], whi = W[f * nfD + hi];
+        k[f] = k[f] - (whi - wlo) / Vr;
+        if (at == n - 1) out_sum[f] += (double)whi;
+        if (at == 0) out_sum[f] -= (double)wlo;
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        if (j >= nq) break;
+        kq[j] = kq[j] - (W[(NF + j) * nfD + hi] - W[(NF + j) * nfD + lo]) / Vr;
+    }
+}
+
+// Cell update (torchFiniteVolume.py:125-229, 232-276): geometric source,
+// the stored faces direction by direction, then the SSPRK2 stage.
+template <int DIM>
+__global__ void __launch_bounds__(MT) k_metric_update(fvl::Grid g, fvl::Metric m, fvl::MStage s) {
     constexpr int NF = DIM + 2;
     const long N = (long)g.nx * g.ny * g.nz;
     const long i = blockIdx.x * (long)MT + threadIdx.x;
@@ -490,13 +544,10 @@ __global__ void __launch_bounds__(MT, 2) k_metric_stage(fvl::Grid g, fvl::Metric
     int bad = 0;
     if (i < N) {
         Ctx c;
-        c.S = (const real*)s.S;
-        c.q[0] = (const real*)s.qS[0];
-        c.q[1] = (const real*)s.qS[1];
         c.N = N;
         c.nq = s.nq;
-        c.reactive = s.reactive;
-        c.gamma = (real)s.gamma;
+        c.q[0] = (const real*)s.qS[0];
+        c.q[1] = (const real*)s.qS[1];
         c.P = (const real*)m.prim;
         int idx[3];
         cell_coords(g, i, idx);
@@ -513,9 +564,9 @@ __global__ void __launch_bounds__(MT, 2) k_metric_stage(fvl::Grid g, fvl::Metric
 #pragma unroll
         for (int f = 0; f < NF; ++f) sums[5 + f] = (double)k[f] * V;
         const real Vr = (real)V;
-        direction<DIM, 0, LIM>(c, m, g, i, idx, s.first_order, Vr, k, kq, sums, bad);
-        if (DIM >= 2) direction<DIM, (DIM >= 2 ? 1 : 0), LIM>(c, m, g, i, idx, s.first_order, Vr, k, kq, sums, bad);
-        if (DIM >= 3) direction<DIM, (DIM >= 3 ? 2 : 0), LIM>(c, m, g, i, idx, s.first_order, Vr, k, kq, sums, bad);
+        gather<DIM, 0>(m, g, idx, s.nq, Vr, k, kq, sums);
+        if (DIM >= 2) gather<DIM, (DIM >= 2 ? 1 : 0)>(m, g, idx, s.nq, Vr, k, kq, sums);
+        if (DIM >= 3) gather<DIM, (DIM >= 3 ? 2 : 0)>(m, g, idx, s.nq, Vr, k, kq, sums);
         stage_out<NF>(s, N, i, k, kq, bad);
     }
     if (bad) atomicOr(s.flags, bad);
@@ -950,16 +1001,30 @@ hipError_t amr_eval_dim(const fvl::Amr& a, const Ctx& c, int reconstruct, int* f
     return hipGetLastError();
 }
 
+template <int DIM, int D, int LIM>
+void launch_faces(const fvl::Grid& g, const fvl::Metric& m, const fvl::MStage& s, hipStream_t st) {
+    const long nf = (long)(g.nx + (D == 0)) * (g.ny + (D == 1)) * (g.nz + (D == 2));
+    k_metric_faces<DIM, D, LIM><<<(int)((nf + MT - 1) / MT), MT, 0, st>>>(g, m, s);
+}
+
+template <int DIM, int LIM>
+void launch_all_faces(const fvl::Grid& g, const fvl::Metric& m, const fvl::MStage& s, hipStream_t st) {
+    launch_faces<DIM, 0, LIM>(g, m, s, st);
+    if (DIM >= 2) launch_faces<DIM, (DIM >= 2 ? 1 : 0), LIM>(g, m, s, st);
+    if (DIM >= 3) launch_faces<DIM, (DIM >= 3 ? 2 : 0), LIM>(g, m, s, st);
+}
+
 template <int DIM>
 hipError_t launch_stage(const fvl::Grid& g, const fvl::Metric& m, const fvl::MStage& s, int blocks,
                         hipStream_t st) {
     switch (s.limiter) {
-        case VANLEER: k_metric_stage<DIM, VANLEER><<<blocks, MT, 0, st>>>(g, m, s); break;
-        case ALBADA: k_metric_stage<DIM, ALBADA><<<blocks, MT, 0, st>>>(g, m, s); break;
-        case MINBEE: k_metric_stage<DIM, MINBEE><<<blocks, MT, 0, st>>>(g, m, s); break;
-        case SUPERBEE: k_metric_stage<DIM, SUPERBEE><<<blocks, MT, 0, st>>>(g, m, s); break;
-        default: k_metric_stage<DIM, UNOS><<<blocks, MT, 0, st>>>(g, m, s); break;
+        case VANLEER: launch_all_faces<DIM, VANLEER>(g, m, s, st); break;
+        case ALBADA: launch_all_faces<DIM, ALBADA>(g, m, s, st); break;
+        case MINBEE: launch_all_faces<DIM, MINBEE>(g, m, s, st); break;
+        case SUPERBEE: launch_all_faces<DIM, SUPERBEE>(g, m, s, st); break;
+        default: launch_all_faces<DIM, UNOS>(g, m, s, st); break;
     }
+    k_metric_update<DIM><<<blocks, MT, 0, st>>>(g, m, s);
     return hipGetLastError();
 }
 
