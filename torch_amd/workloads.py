@@ -3,6 +3,9 @@
 * ``sod_3d``: the 3D Cartesian Sod tube along x (``3d_sod_x``,
   torchSolver3D.py:271-274): (rho, p) = (1, 1) for cell centres x <= 0.5,
   (0.125, 0.1) beyond, at rest; open x ends, reflective y/z walls.
+* ``quadrant_2d`` / ``sphere_3d``: the seeded 2D quadrant Riemann problem and
+  the 3D pressure sphere (rho, p) = (1, 10) within radius 0.1 of the centre,
+  (0.125, 0.1) outside, at rest.
 * ``uneven_problems``: seeded mixes of 1D (N in 256..8192), 2D (64^2..512^2)
   and 3D (32^3..160^3) boxes, dimension cycling 1, 2, 3, each with random
   two-state planar or spherical Riemann data (rho in [0.1, 2], |u| <= 1 in a
@@ -26,6 +29,37 @@ def sod_3d(n):
     prim = np.zeros((5, n, n, n))
     prim[0] = np.where(left, 1.0, 0.125)
     prim[4] = np.where(left, 1.0, 0.1)
+    return prim
+
+
+def quadrant_2d(n, seed=20261203):
+    """Primitive (4, n, n) of the seeded 2D quadrant Riemann problem: four
+    constant states in the quadrants of the unit square split at (0.5, 0.5),
+    drawn in the order upper right, upper left, lower left, lower right, each
+    as (rho, u, v, p) with rho in [0.1, 1.5], u and v in [-1, 1] and p in
+    [0.03, 1.5]."""
+    rng = np.random.default_rng(seed)
+    states = [(rng.uniform(0.1, 1.5), rng.uniform(-1, 1), rng.uniform(-1, 1), rng.uniform(0.03, 1.5))
+              for _ in range(4)]
+    c = (np.arange(n) + 0.5) / n
+    right, upper = (c > 0.5)[None, :], (c > 0.5)[:, None]
+    prim = np.zeros((4, n, n))
+    for q, (hi_y, hi_x) in enumerate(((True, True), (True, False), (False, False), (False, True))):
+        mask = (upper == hi_y) & (right == hi_x)
+        for k in range(4):
+            prim[k] = np.where(mask, states[q][k], prim[k])
+    return prim
+
+
+def sphere_3d(n):
+    """Primitive (5, n, n, n) of the 3D pressure sphere: rho = 1, p = 10 for
+    cell centres within 0.1 of (0.5, 0.5, 0.5), rho = 0.125, p = 0.1 outside,
+    at rest."""
+    x, y, z = _centres([n, n, n])
+    inside = (x - 0.5) ** 2 + (y - 0.5) ** 2 + (z - 0.5) ** 2 <= 0.1 ** 2
+    prim = np.zeros((5, n, n, n))
+    prim[0] = np.where(inside, 1.0, 0.125)
+    prim[4] = np.where(inside, 10.0, 0.1)
     return prim
 
 
