@@ -367,3 +367,36 @@ def test_a_failed_regrid_leaves_the_step_uncommitted(monkeypatch):
     s.step()
     assert s.steps == 1
     s.close()
+
+
+def test_steps_through_first_order_retries_and_halvings():
+    """A rough 1D state at CFL 1.5 (unchecked): some steps retry at first
+    order, some halve dt; the device run follows the oracle's step for step
+    (the host-driven retry path behind the queued step)."""
+    import test_adaptive_host as host
+    from torch_amd import Simulation
+    from torch_amd import torchAdaptiveGeometry as tag_geo
+    from torch_amd import torchGeometry as geo
+
+    rng = np.random.default_rng(0)
+    n = 32
+    prim = np.array([0.2 + 1.8 * rng.random(n), 2 * rng.standard_normal(n), 0.1 + 4.9 * rng.random(n)])
+    c = dict(dimension=1, counts_xyz=[n], lengths_xyz=[1.0], boundaries_xyz=["Open"], origins_xyz=[0.0],
+             geometry="cartesian", t_end=10.0, cfl=1.5, gamma=1.4, limiter="VanLeer", reactive=False,
+             kinetic_model="arrhenius", max_steps=100000)
+    c["face_boundaries"] = geo.normalize_faces(c, None)
+    c["amr"] = tag_geo.settings(max_level=1, regrid_every=2)
+    ref = host.HostLeaves(c, prim)
+    s = Simulation(prim, [1.0], t_end=10.0, cfl=1.5, checked=False).to_adaptive(max_level=1, regrid_every=2)
+    np.testing.assert_array_equal(s.mesh.keys, ref.mesh.keys)
+    halvings = 0
+    for k in range(8):
+        ref.step()
+        info = s.step()
+        halvings += info.retries
+        np.testing.assert_array_equal(s.mesh.keys, ref.mesh.keys, err_msg=f"step {k}")
+        assert rel(s.state, ref.state) <= 1e-11, k
+        assert s.time == pytest.approx(ref.time, rel=1e-12)
+    assert s.fallbacks >= 1 and halvings >= 1
+    assert sums_ok(s.boundary_integral, ref.boundary, 1e-11)
+    s.close()

@@ -267,3 +267,26 @@ def test_fp32_metric_session_tracks_fp64(d):
         err = np.abs(b.state - a.state).reshape(a.state.shape[0], -1).sum(axis=1) / np.abs(a.state).reshape(
             a.state.shape[0], -1).sum(axis=1)
         assert np.all(err < 1e-5), err
+
+
+def test_session_step_through_the_first_order_retry():
+    """A rough cylindrical state at CFL 0.9 (unchecked): some steps lose
+    positivity at second order and retry at first order inside the step;
+    every step equals the oracle's, fallbacks included."""
+    from torch_amd import Simulation
+
+    rng = np.random.default_rng(1)
+    n = 64
+    prim = np.array([0.2 + 1.8 * rng.random(n), 2 * rng.standard_normal(n), 0.1 + 4.9 * rng.random(n)])
+    opts = dict(geometry="cylindrical", origins=[0.5], t_end=10.0, cfl=0.9, checked=False)
+    dev = Simulation(prim, [1.0], **opts)
+    ref = OracleSession(prim, [1.0], **opts)
+    for k in range(10):
+        ref.step()
+        dev.step()
+        assert dev.fallbacks == ref.fallbacks, k
+        assert dev.time == pytest.approx(ref.time, rel=1e-12)
+        close(dev.state, ref.state, rel=1e-11, what=f"step {k}")
+    assert dev.fallbacks > 0
+    sums_close(dev.boundary_integral, ref.boundary_integral, rel=1e-11)
+    dev.close()

@@ -42,14 +42,18 @@ struct torch_amr {
          *S = nullptr;
     double* partials = nullptr;  // 4 regions: faces, leaves, CFL, chemistry
     long pblocks = 0;
-    double* scalars = nullptr;   // 4 x NSLOT finalized values
+    double* scalars = nullptr;   // kRows x NSLOT finalized values
     int* flags = nullptr;
-    double* host = nullptr;      // pinned readback: 4 x NSLOT doubles + flags
+    double* host = nullptr;      // pinned readback: kRows x NSLOT doubles + flags
 };
 
 namespace {
 
 const int kSlots = 4;
+// Finalized rows: 0 outward sums of an evaluation, 1 leaf sums of a stage,
+// 2 the CFL rate (and the queued step's dt in its last slot), 3 unused,
+// 4 and 5 the second stage's sums in a queued step.
+const int kRows = 6;
 
 __global__ void k_amr_clear(int* p, int n) {
     for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
@@ -64,15 +68,15 @@ double* region(torch_amr* a, int r) { return a->partials + (long)r * a->pblocks 
 
 int sync_read(torch_amr* a) {
     hipStream_t st = a->ctx->stream;
-    HIP_TRY(hipMemcpyAsync(a->host, a->scalars, 4 * fvl::NSLOT * sizeof(double), hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(a->host + 4 * fvl::NSLOT, a->flags, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(a->host, a->scalars, kRows * fvl::NSLOT * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(a->host + kRows * fvl::NSLOT, a->flags, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     return TORCH_OK;
 }
 
 int host_flags(const torch_amr* a) {
     int f;
-    memcpy(&f, a->host + 4 * fvl::NSLOT, sizeof(int));
+    memcpy(&f, a->host + kRows * fvl::NSLOT, sizeof(int));
     return f;
 }
 
@@ -328,6 +332,78 @@ int react(torch_amr* a, int in, int out, double dt) {
     return TORCH_OK;
 }
 
+__global__ void k_amr_dt(const double* max_rate, double cfl, double time, double t_end, double* dt) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        const double d = cfl / max_rate[0];
+        const double rest = t_end - time;
+        dt[0] = rest < d ? rest : d;
+    }
+}
+
+// The common inert step without host round trips (torchAdaptive.py:436-481
+// when nothing is retried): the evaluation of the committed state, cfl, dt
+// on the device, the first stage on that evaluation, the second stage's
+// evaluation and update, one readback. Returns TORCH_OK with the step
+// committed, kSlowPath when a stage lost positivity (nothing committed; the
+// host-driven step with its retries takes over), or an error.
+const int kSlowPath = -1;
+
+int fast_step(torch_amr* a, double time, double t_end, torch_step_info* info) {
+    hipStream_t st = a->ctx->stream;
+    const int A = a->cur, B = (A + 1) % kSlots, C = (A + 2) % kSlots;
+    const int rec = a->order == 2;
+    const fvl::Amr mesh = mesh_of(a);
+    const int lb = a->ops->amr_blocks(a->nleaf), fb = a->ops->amr_blocks(a->nface);
+    double* row[kRows];
+    for (int r = 0; r < kRows; ++r) row[r] = a->scalars + r * fvl::NSLOT;
+    double* dtp = row[2] + fvl::NSLOT - 1;
+    a->cache_slot = -1;
+    HIP_TRY(clear(a));
+    HIP_TRY(a->ops->amr_eval(a->cfg.dim, mesh, a->U[A], nullptr, nullptr, 0, a->cfg.gamma, 0, rec, a->flags,
+                             region(a, 0), st));
+    HIP_TRY(fvl::finalize(region(a, 0), fb, row[0], st));
+    HIP_TRY(a->ops->amr_cfl(a->cfg.dim, mesh, region(a, 2), st));
+    HIP_TRY(fvl::finalize(region(a, 2), lb, row[2], st));
+    k_amr_dt<<<1, 64, 0, st>>>(row[2], a->cfg.cfl, time, t_end, dtp);
+    HIP_TRY(hipGetLastError());
+    for (int k = 0; k < 2; ++k) {
+        if (k == 1) {
+            HIP_TRY(a->ops->amr_eval(a->cfg.dim, mesh, a->U[B], nullptr, nullptr, 0, a->cfg.gamma, 0, rec,
+                                     a->flags + 1, region(a, 0), st));
+            HIP_TRY(fvl::finalize(region(a, 0), fb, row[4], st));
+        }
+        fvl::MStage m;
+        memset(&m, 0, sizeof(m));
+        m.S = a->U[k == 0 ? A : B];
+        m.U0 = a->U[A];
+        m.out = a->U[k == 0 ? B : C];
+        m.dtp = dtp;
+        m.gamma = a->cfg.gamma;
+        m.mode = k + 1;
+        m.flags = a->flags + k;
+        m.partials = region(a, 1);
+        HIP_TRY(a->ops->amr_update(a->cfg.dim, mesh, m, st));
+        HIP_TRY(fvl::finalize(region(a, 1), lb, row[k == 0 ? 1 : 5], st));
+    }
+    int rc = sync_read(a);
+    if (rc) return rc;
+    int flags[2];
+    memcpy(flags, a->host + kRows * fvl::NSLOT, sizeof(flags));
+    if ((rc = boundary_error(flags[0] | flags[1]))) return rc;
+    const double* h = a->host;
+    const double dt = h[2 * fvl::NSLOT + fvl::NSLOT - 1];
+    if (!isfinite(dt) || dt <= 0 || time + dt == time) return fail(TORCH_STALLED, "Adaptive integration stalled");
+    if ((flags[0] | flags[1]) & (fvl::F_M_UNPHYS | fvl::F_M_SPEC)) return kSlowPath;
+    memset(info, 0, sizeof(*info));
+    info->dt = dt;
+    for (int f = 0; f < a->nf; ++f) {
+        info->outward[f] = (0.5 * dt) * (h[fvl::S_OUT + f] + h[4 * fvl::NSLOT + fvl::S_OUT + f]);
+        info->source[f] = (0.5 * dt) * (h[fvl::NSLOT + fvl::S_MSRC + f] + h[5 * fvl::NSLOT + fvl::S_MSRC + f]);
+    }
+    a->cur = C;
+    return TORCH_OK;
+}
+
 int ready(torch_amr* a) {
     if (!a->has_mesh) return fail(TORCH_INVALID, "adaptive solver has no mesh (torch_amr_set_mesh)");
     if (!a->has_state) return fail(TORCH_INVALID, "adaptive solver has no state for this mesh (torch_amr_upload)");
@@ -390,9 +466,9 @@ int torch_amr_create(torch_ctx* ctx, const torch_config* cfg, int geometry, cons
     std::lock_guard<std::mutex> lock(ctx->mu);
     int rc = TORCH_OK;
     if (hipSetDevice(ctx->device) != hipSuccess) rc = fail(TORCH_HIP_ERROR, "hipSetDevice failed");
-    if (!rc) rc = alloc((void**)&a->scalars, 4 * fvl::NSLOT * sizeof(double));
+    if (!rc) rc = alloc((void**)&a->scalars, kRows * fvl::NSLOT * sizeof(double));
     if (!rc) rc = alloc((void**)&a->flags, 16 * sizeof(int));
-    if (!rc && hipHostMalloc((void**)&a->host, (4 * fvl::NSLOT + 2) * sizeof(double)) != hipSuccess)
+    if (!rc && hipHostMalloc((void**)&a->host, (kRows * fvl::NSLOT + 2) * sizeof(double)) != hipSuccess)
         rc = fail(TORCH_HIP_ERROR, "hipHostMalloc failed");
     if (rc) {
         release_buffers(a);
@@ -607,6 +683,11 @@ int torch_amr_step(torch_amr* a, double time, double t_end, torch_step_info* inf
     if (rc) return rc;
     HIP_TRY(hipSetDevice(a->ctx->device));
     const bool reactive = a->cfg.reactive != 0;
+    if (!reactive) {
+        rc = fast_step(a, time, t_end, info);
+        if (rc != kSlowPath) return rc;
+        memset(info, 0, sizeof(*info));
+    }
     const int A = a->cur;
     const int B = (A + 1) % kSlots, C = (A + 2) % kSlots, D = (A + 3) % kSlots;
     hipStream_t st = a->ctx->stream;

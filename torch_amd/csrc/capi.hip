@@ -497,6 +497,68 @@ int metric_advance(torch_state* s, int in, int stage, int out, double dt, double
     return fail(TORCH_NONPHYSICAL, "Metric/face update lost positivity; reduce CFL");
 }
 
+// dt of a metric step on the device: cfl / max rate clipped to t_end - time
+// (torchSession.py:531-533), into *dt; the host checks it after the step.
+__global__ void k_metric_dt(const double* max_rate, double cfl, double time, double t_end, double* dt) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        const double d = cfl / max_rate[0];
+        const double rest = t_end - time;
+        dt[0] = rest < d ? rest : d;
+    }
+}
+
+// The common inert metric step without host round trips: cfl, dt and both
+// second-order stages are queued back to back and read once. Returns
+// TORCH_OK with the step committed, kSlowPath when a stage lost positivity
+// (nothing committed; the host-driven step with its retries takes over), or
+// an error.
+const int kSlowPath = -1;
+
+int metric_fast_step(torch_state* s, double time, double t_end, torch_step_info* info) {
+    hipStream_t st = s->ctx->stream;
+    const int A = s->cur, B = (A + 1) % kMetricSlots, C = (A + 2) % kMetricSlots;
+    double* rate = s->scalars + 3 * fvl::NSLOT;
+    double* dtp = s->scalars + 3 * fvl::NSLOT + fvl::NSLOT - 1;
+    HIP_TRY(clear_flags(s->flags, st));
+    HIP_TRY(s->ops->metric_cfl(s->grid, s->metric, s->U[A], nullptr, s->cfg.gamma, 0, region(s, 3), s->flags, st));
+    HIP_TRY(fvl::finalize(region(s, 3), s->ops->metric_blocks(s->grid), rate, st));
+    k_metric_dt<<<1, 64, 0, st>>>(rate, s->cfg.cfl, time, t_end, dtp);
+    HIP_TRY(hipGetLastError());
+    for (int k = 0; k < 2; ++k) {
+        fvl::MStage m;
+        memset(&m, 0, sizeof(m));
+        m.S = s->U[k == 0 ? A : B];
+        m.U0 = s->U[A];
+        m.out = s->U[k == 0 ? B : C];
+        m.dtp = dtp;
+        m.gamma = s->cfg.gamma;
+        m.mode = k + 1;
+        m.limiter = s->cfg.limiter;
+        m.flags = s->flags + k;
+        m.partials = region(s, k);
+        HIP_TRY(s->ops->metric_stage(s->grid, s->metric, m, st));
+        HIP_TRY(fvl::finalize(region(s, k), s->ops->metric_blocks(s->grid), s->scalars + k * fvl::NSLOT, st));
+    }
+    int rc = sync_read(s);
+    if (rc) return rc;
+    int flags[2];
+    memcpy(flags, s->host + 4 * fvl::NSLOT, sizeof(flags));
+    if ((rc = boundary_error(flags[0] | flags[1]))) return rc;
+    const double dt = s->host[3 * fvl::NSLOT + fvl::NSLOT - 1];
+    if (!isfinite(dt) || dt <= 0 || time + dt == time)
+        return fail(TORCH_STALLED, "Integration stalled before requested final time");
+    if ((flags[0] | flags[1]) & (fvl::F_M_UNPHYS | fvl::F_M_SPEC)) return kSlowPath;
+    memset(info, 0, sizeof(*info));
+    info->dt = dt;
+    for (int f = 0; f < s->nf; ++f) {
+        info->outward[f] = (0.5 * dt) * (s->host[fvl::S_OUT + f] + s->host[fvl::NSLOT + fvl::S_OUT + f]);
+        info->source[f] = (0.5 * dt) * (s->host[fvl::S_MSRC + f] + s->host[fvl::NSLOT + fvl::S_MSRC + f]);
+    }
+    s->cur = C;
+    s->maxima_valid = false;
+    return TORCH_OK;
+}
+
 int need_metric(const torch_state* s) {
     if (!s->has_metric) return fail(TORCH_INVALID, "state has no metric (torch_state_set_metric)");
     return TORCH_OK;
@@ -972,6 +1034,11 @@ int torch_state_metric_step(torch_state* s, double time, double t_end, torch_ste
     int rc = need_metric(s);
     if (rc) return rc;
     const bool reactive = s->cfg.reactive;
+    if (!reactive) {
+        rc = metric_fast_step(s, time, t_end, info);
+        if (rc != kSlowPath) return rc;
+        memset(info, 0, sizeof(*info));
+    }
     const int A = s->cur;
     const int B = (A + 1) % kMetricSlots, C = (A + 2) % kMetricSlots, D = (A + 3) % kMetricSlots;
     double dt;

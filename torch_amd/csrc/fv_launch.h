@@ -78,6 +78,7 @@ struct MStage {
     void* out;
     void* qout[2];
     double dt, gamma;
+    const double* dtp;        // dt in device memory, used instead of dt when set
     int mode, first_order, limiter, nq, reactive;
     int* flags;
     double* partials;         // NSLOT per block: S_OUT.. outward, S_MSRC.. integrated source

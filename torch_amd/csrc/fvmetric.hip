@@ -396,7 +396,7 @@ __device__ __forceinline__ void stage_out(const fvl::MStage& s, long N, long i, 
         for (int j = 0; j < 2; ++j) qo[j] = kq[j];
     } else {
         const real* U0 = (const real*)s.U0;
-        const real dt = (real)s.dt;
+        const real dt = (real)(s.dtp ? *s.dtp : s.dt);
 #pragma unroll
         for (int f = 0; f < NF; ++f)
             out[f] = s.mode == 1 ? S[f * N + i] + dt * k[f]
