@@ -26,9 +26,9 @@ workload's length). The JSON line reports:
 - metric: the metric/face scheme (curvilinear grid, face boundaries,
   SSPRK2 MUSCL-Rusanov) on a cylindrical-sector blast, FP64 and FP32, with
   its own CPU baseline;
-- adaptive: the dyadic AMR leaves (2D Sod, left half refined twice), FP64
-  and FP32 leaf-updates/s of the device step, the host remesh time, and its
-  own CPU baseline;
+- adaptive: the dyadic AMR leaves (2D and 3D Sod, left half refined twice),
+  FP64 and FP32 leaf-updates/s of the device step, the host remesh time, and
+  its own CPU baseline;
 - cpu_baseline: the NumPy oracle (a restatement of the reference's CPU path)
   timed on this host on a bounded sample of the same workload.
 
@@ -320,23 +320,26 @@ def cpu_metric_baseline(budget_s=8.0, n=24):
             "sample": f"cylindrical-sector blast {n}^3, {steps} metric steps of the NumPy oracle, {wall:.1f} s"}
 
 
-def adaptive_problem(nx=512, ny=64):
-    """A 2D Sod tube on dyadic leaves: nx x ny roots over [0, 1] x [0, ny/nx]
-    (square cells), (rho, u, v, p) = (1, 0, 0, 1) for x < 0.5 and
-    (0.125, 0, 0, 0.1) beyond, open x ends, reflective y walls, CFL 0.4,
-    second-order reconstruction; the left half refined twice (level 2)."""
-    prim = np.zeros((4, ny, nx))
+def adaptive_problem(nx=512, ny=64, nz=None):
+    """A Sod tube on dyadic leaves: nx x ny (x nz) roots over [0, 1] x
+    [0, ny/nx] (x [0, nz/nx]; cubic cells), (rho, u, p) = (1, 0, 1) for
+    x < 0.5 and (0.125, 0, 0.1) beyond, open x ends, reflective walls
+    elsewhere, CFL 0.4, second-order reconstruction; the left half refined
+    twice (level 2)."""
+    dim = 2 if nz is None else 3
+    counts = [nx, ny] + ([nz] if nz else [])
+    prim = np.zeros((dim + 2, *counts[::-1]))
     right = (np.arange(nx) + 0.5) / nx >= 0.5
-    prim[0] = np.where(right, 0.125, 1.0)[None, :]
-    prim[3] = np.where(right, 0.1, 1.0)[None, :]
-    opts = dict(boundaries=["Open", "Reflective"], t_end=10.0, cfl=0.4)
-    return prim, [1.0, ny / nx], opts
+    prim[0] = np.where(right, 0.125, 1.0)
+    prim[-1] = np.where(right, 0.1, 1.0)
+    opts = dict(boundaries=["Open"] + ["Reflective"] * (dim - 1), t_end=10.0, cfl=0.4)
+    return prim, [c / nx for c in counts], opts
 
 
-def adaptive_session(precision, nx, ny, regrid_every=10 ** 9):
+def adaptive_session(precision, nx, ny, nz=None, regrid_every=10 ** 9):
     from torch_amd import Simulation
 
-    prim, lengths, opts = adaptive_problem(nx, ny)
+    prim, lengths, opts = adaptive_problem(nx, ny, nz)
     s = Simulation(prim, lengths, precision=precision, **opts).to_adaptive(
         max_level=2, regrid_every=regrid_every, max_cells=2_000_000)
     for _ in range(2):
@@ -344,8 +347,8 @@ def adaptive_session(precision, nx, ny, regrid_every=10 ** 9):
     return s
 
 
-def run_adaptive(precision, steps, warmup, nx, ny):
-    s = adaptive_session(precision, nx, ny)
+def run_adaptive(precision, steps, warmup, nx, ny, nz=None):
+    s = adaptive_session(precision, nx, ny, nz)
     leaves, faces = len(s.mesh.keys), len(s.mesh.left)
     for _ in range(warmup):
         s.step()
@@ -380,7 +383,7 @@ def cpu_adaptive_baseline(budget_s=8.0, nx=128, ny=16):
     c["amr"] = tag.settings(max_level=2, max_cells=2_000_000)
     mesh = tag.LeafMesh(c)
     state = prim.reshape(4, -1).copy()
-    state[-1] = prim[3].ravel() / 0.4
+    state[-1] = prim[-1].ravel() / 0.4
     for _ in range(2):
         faces = oa.face_values(mesh, state, [], c, False)[:4]
         mesh, state, _, _, _ = remesh(mesh, state, [], c, faces, mesh.centers[:, 0] < 0.5, False)
@@ -463,6 +466,8 @@ def main(argv=None):
     ap.add_argument("--metric-steps", type=int, default=20)
     ap.add_argument("--amr-roots", type=_pair, default=(512, 64), help="NX,NY roots of the adaptive workload")
     ap.add_argument("--amr-steps", type=int, default=20)
+    ap.add_argument("--amr-roots-3d", type=lambda t: tuple(int(v) for v in t.split(",")), default=(64, 16, 16),
+                    help="NX,NY,NZ roots of the 3D adaptive workload")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device copies of MIB MiB (counter calibration) and exit")
     args = ap.parse_args(argv)
@@ -542,6 +547,13 @@ def main(argv=None):
             adaptive[precision] = run_adaptive(precision, args.amr_steps, 2, ax, ay)
             say(f"# adaptive {precision}: {adaptive[precision]['ms_per_step']} ms/step, "
                 f"{adaptive[precision]['leaves']} leaves")
+        bx, by, bz = args.amr_roots_3d
+        adaptive["workload_3d"] = (f"the same on {bx}x{by}x{bz} roots in 3D (left half at level 2, reflective "
+                                   "y and z walls)")
+        for precision in ("fp64", "fp32"):
+            adaptive[precision + "_3d"] = run_adaptive(precision, args.amr_steps, 2, bx, by, bz)
+            say(f"# adaptive 3D {precision}: {adaptive[precision + '_3d']['ms_per_step']} ms/step, "
+                f"{adaptive[precision + '_3d']['leaves']} leaves")
         if "cpu" in legs:
             adaptive["cpu_baseline"] = cpu_adaptive_baseline()
     if "cpu" in legs:
