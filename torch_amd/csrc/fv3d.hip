@@ -201,6 +201,10 @@ __device__ __forceinline__ void step3d_tile(const Step& s, const G3& g, int tbx,
     real qp[2] = {0, 0}, qc[2] = {0, 0}, qn[2] = {0, 0};
     real zr[5] = {0, 0, 0, 0, 0}, Fzp[5] = {0, 0, 0, 0, 0}, sfzp[2] = {0, 0};
     real R[5] = {0, 0, 0, 0, 0}, rq[2] = {0, 0};
+    // The y part of a plane's partial update is applied after the next
+    // plane's first barrier, from the y fluxes left in sF (one barrier fewer
+    // per plane).
+    bool pend = false;
     double acc[5] = {0, 0, 0, 0, 0}, mx[3];
     real mxr[3] = {0, 0, 0};
     int bad = 0;
@@ -233,6 +237,16 @@ __device__ __forceinline__ void step3d_tile(const Step& s, const G3& g, int tbx,
             for (int c = 0; c < 5; ++c) sA[c][ry + 2][rx + 2] = r[c];
         }
         __syncthreads();
+
+        // y part of the previous plane's partial update: its y fluxes were
+        // stored in sF before the barrier above, and sF is next written after
+        // this plane's second barrier.
+        if (pend) {
+#pragma unroll
+            for (int c = 0; c < 5; ++c) R[c] = R[c] - c1 * (sF[c][row + 1][lane] - sF[c][row][lane]);
+            for (int j = 0; j < NQ; ++j)
+                if (j < nq) rq[j] = rq[j] - c1 * (sF[5 + j][row + 1][lane] - sF[5 + j][row][lane]);
+        }
 
         real sx[5], sy[5], sz[5];
 #pragma unroll
@@ -327,7 +341,6 @@ __device__ __forceinline__ void step3d_tile(const Step& s, const G3& g, int tbx,
 #pragma unroll
         for (int c = 0; c < 5; ++c) sF[c][row][lane] = Fy[c];
         for (int j = 0; j < NQ; ++j) sF[5 + j][row][lane] = sfy[j];
-        __syncthreads();
 
         real FxR[5], sfxR[2] = {0, 0};
 #pragma unroll
@@ -351,20 +364,14 @@ __device__ __forceinline__ void step3d_tile(const Step& s, const G3& g, int tbx,
                 ((real*)s.qout[j])[i] = rmin(rmax(f, (real)0), (real)1);
             }
         }
-        // Partial update of plane k with the x and y faces.
-        if (owns && plane_out) {
-            real FyT[5];
+        // Partial update of plane k with the x faces; the y faces follow
+        // after the next plane's first barrier.
+        pend = owns && plane_out;
+        if (pend) {
 #pragma unroll
-            for (int c = 0; c < 5; ++c) FyT[c] = sF[c][row + 1][lane];
-#pragma unroll
-            for (int c = 0; c < 5; ++c) {
-                R[c] = Ac[c] - c0 * (FxR[c] - Fx[c]);
-                R[c] = R[c] - c1 * (FyT[c] - Fy[c]);
-            }
-            for (int j = 0; j < NQ; ++j) if (j < nq) {
-                rq[j] = Ac[0] * qc[j] - c0 * (sfxR[j] - sfx[j]);
-                rq[j] = rq[j] - c1 * (sF[5 + j][row + 1][lane] - sfy[j]);
-            }
+            for (int c = 0; c < 5; ++c) R[c] = Ac[c] - c0 * (FxR[c] - Fx[c]);
+            for (int j = 0; j < NQ; ++j)
+                if (j < nq) rq[j] = Ac[0] * qc[j] - c0 * (sfxR[j] - sfx[j]);
         }
         // Outward boundary fluxes and optional face dumps.
         if (plane_out && xface) {
