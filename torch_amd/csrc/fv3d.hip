@@ -131,8 +131,9 @@ constexpr int occupancy_of(int threads, bool spec = false) {
 }
 
 // LDS footprint of one tile: the A plane with a two-cell halo, the y face
-// states and the y fluxes (separate buffers, so a plane needs three barriers:
-// A plane written, y face states written, y fluxes written).
+// states and the y fluxes (separate buffers, so a plane needs two barriers:
+// A plane written, y face states written; the y fluxes written after the
+// second are read after the next plane's first).
 template <int BX, int BY, bool SPEC>
 struct Tile3 {
     static constexpr int NQ = SPEC ? 2 : 0;
