@@ -247,6 +247,20 @@ def fp32_error(n, warmup, steps, dts64):
     return err
 
 
+def host_cpu():
+    """The host's CPU model and logical CPU count, for the CPU baselines."""
+    model = "unknown CPU"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return f"{model}, {os.cpu_count()} logical CPUs"
+
+
 def cpu_baseline(budget_s=12.0, n=40):
     """The NumPy oracle on the same workload at n^3, single thread."""
     os.environ.setdefault("OPENBLAS_NUM_THREADS", "1")
@@ -262,7 +276,7 @@ def cpu_baseline(budget_s=12.0, n=40):
     wall = time.perf_counter() - t0
     return {"value": round(n ** 3 * steps / wall, 1), "unit": "cell-updates/s", "cores": 1, "kind": "port",
             "sample": f"3D Sod {n}^3 MINBEE CFL 0.8 walls, {steps} steps of the NumPy oracle "
-                      f"(restated reference CPU path, 1 thread), {wall:.1f} s"}
+                      f"(restated reference CPU path, 1 thread), {wall:.1f} s on {host_cpu()}"}
 
 
 def metric_problem(n):
