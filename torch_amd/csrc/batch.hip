@@ -399,6 +399,7 @@ int rebuild(torch_batch* b) {
         p->ctl.nllf = p->nllf;
         p->ctl.ncfl = p->ncfl;
         p->ctl.nreact = p->nreact;
+        p->ctl.ticket = 0;
         b->host_ctl[i] = p->ctl;
     }
     hipStream_t st = b->ctx->stream;

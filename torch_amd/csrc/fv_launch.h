@@ -267,7 +267,12 @@ struct BatchCtl {
     int active;          // 1 stepping, 2 waiting for its first dt, 0 stopped
     int cur;             // buffer holding the committed state
     int reactive;
+    int ticket;          // post blocks done with their slot (0 between launches)
+    double folded[9];    // slot totals of the step being committed
 };
+
+// Blocks of the post kernel per instance: one per folded partial slot.
+constexpr int kPostSlots = 9;
 
 // Instance of block b in a work list: first[i] <= b < first[i+1].
 __device__ __forceinline__ int owner(const int* first, int n, int b) {

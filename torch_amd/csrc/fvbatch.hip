@@ -15,7 +15,10 @@
 // partials in the order of the single-session finalize, checks the step's
 // flags, commits (time, step count, boundary and source integrals, buffer
 // rotation) and derives the next dt and step coefficients from the new wave
-// maxima.
+// maxima. An instance gets kPostSlots blocks, one per partial slot, so the
+// folds run side by side on as many CUs; each block stores its slot's total
+// in BatchCtl.folded and takes a ticket, and the block that takes the last
+// ticket commits.
 #include "fv_device.h"
 #include "fv_launch.h"
 
@@ -82,6 +85,31 @@ __device__ void fold(const double* part, int nb, int nslot, double (*red)[256]) 
     }
 }
 
+// One slot of nb per-block partials folded by a 256-thread block in the
+// single-session finalize's order (k_finalize): lane t accumulates the rows
+// t, t+256, ... in increasing order, then a tree over the 256 lanes.
+// Every thread must call it; the total is returned to thread 0.
+__device__ double fold_slot(const double* part, int nb, int slot, double* buf) {
+    const int t = threadIdx.x;
+    const bool is_max = slot < 3;
+    double v = 0.0;
+#pragma unroll 8
+    for (int b = t; b < nb; b += 256) {
+        const double p = part[(long)b * fvl::NSLOT + slot];
+        v = is_max ? fmax(v, p) : v + p;
+    }
+    buf[t] = v;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (t < s) {
+            double a = buf[t], b = buf[t + s];
+            buf[t] = is_max ? fmax(a, b) : a + b;
+        }
+        __syncthreads();
+    }
+    return buf[0];
+}
+
 // Step coefficients for dt (the host driver's base_step expressions).
 __device__ __forceinline__ void set_coef(fvl::BatchCtl& c, double dt) {
     for (int d = 0; d < 3; ++d) {
@@ -135,37 +163,47 @@ __global__ void __launch_bounds__(256) k_batch_check(fvl::BatchItem* items, fvl:
 
 __global__ void __launch_bounds__(256) k_batch_post(fvl::BatchItem* items, fvl::BatchCtl* ctl, int prime) {
 #pragma clang fp contract(off)
-    __shared__ double red[9][256];
-    const int i = blockIdx.x, t = threadIdx.x;
+    __shared__ double buf[256];
+    __shared__ int last;
+    const int i = blockIdx.x / fvl::kPostSlots, k = blockIdx.x % fvl::kPostSlots, t = threadIdx.x;
     fvl::BatchCtl& c = ctl[i];
+    // Every block of an instance takes the same exit here (nothing below
+    // changes these fields before the last ticket is taken).
     if (prime ? c.active != 2 : c.active != 1) return;
     // A reactive attempt rejected by k_batch_check: nothing to commit yet.
     if (!prime && c.reactive && c.coef[fvl::K_HYDRO] == 0.0) return;
     fvl::BatchItem& it = items[i];
     const int flags = prime ? 0 : *it.step.flags;
     const bool fb = (flags & (fvl::F_PRED_BAD | fvl::F_UPD_BAD)) != 0;
+    // Slot k's pass: the CFL pass (first dt, 1D), else the second chemistry
+    // half-step for the maxima and the source integral of a reactive
+    // instance, else the hydro pass whose state was kept.
     const double* part = it.step.partials;
-    double mx[3], out[5] = {0, 0, 0, 0, 0}, src = 0;
-    if (prime) {
-        fold(part + (long)it.cfl_first * fvl::NSLOT, c.ncfl, 3, red);
-        for (int m = 0; m < 3; ++m) mx[m] = red[m][0];
-    } else if (c.dim == 1) {
-        fold(part + (long)it.cfl_first * fvl::NSLOT, c.ncfl, 3, red);
-        for (int m = 0; m < 3; ++m) mx[m] = red[m][0];
-    } else {
-        // Boundary sums (and, inert, maxima) of the pass whose state was kept.
-        fold(part + (long)(fb ? it.llf_first : 0) * fvl::NSLOT, fb ? c.nllf : c.nblk, 8, red);
-        for (int m = 0; m < 3; ++m) mx[m] = red[m][0];
-        for (int k = 0; k < 5; ++k) out[k] = red[fvl::S_OUT + k][0];
-        if (c.reactive) {
-            __syncthreads();
-            // Maxima and source integral of the second chemistry half-step.
-            fold(part + (long)it.r2_first * fvl::NSLOT, c.nreact, 9, red);
-            for (int m = 0; m < 3; ++m) mx[m] = red[m][0];
-            src = red[fvl::S_SRC][0];
-        }
+    long first = -1;
+    int nb = 0;
+    if (prime || c.dim == 1) {
+        if (k < 3) first = it.cfl_first, nb = c.ncfl;
+    } else if (c.reactive && (k < 3 || k == fvl::S_SRC)) {
+        first = it.r2_first, nb = c.nreact;
+    } else if (k < fvl::S_SRC) {
+        first = fb ? it.llf_first : 0, nb = fb ? c.nllf : c.nblk;
     }
-    if (t != 0) return;
+    if (first >= 0) {
+        const double v = fold_slot(part + first * fvl::NSLOT, nb, k, buf);
+        if (t == 0) c.folded[k] = v;
+    }
+    if (t == 0)
+        last = __hip_atomic_fetch_add(&c.ticket, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) ==
+               fvl::kPostSlots - 1;
+    __syncthreads();
+    if (!last || t != 0) return;
+    c.ticket = 0;
+    double mx[3], out[5] = {0, 0, 0, 0, 0}, src = 0;
+    for (int m = 0; m < 3; ++m) mx[m] = c.folded[m];
+    if (!prime && c.dim >= 2) {
+        for (int j = 0; j < 5; ++j) out[j] = c.folded[fvl::S_OUT + j];
+        if (c.reactive) src = c.folded[fvl::S_SRC];
+    }
     if (c.dim == 2 && it.grid.bc[1] == BC_INFLOW) {
         // The inlet ghost rows belong to the padded state the 2D CFL covers.
         real u[4];
@@ -224,7 +262,7 @@ __global__ void __launch_bounds__(256) k_batch_post(fvl::BatchItem* items, fvl::
 
 hipError_t batch_post(fvl::BatchItem* items, fvl::BatchCtl* ctl, int n, int prime, hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    k_batch_post<<<n, 256, 0, st>>>(items, ctl, prime);
+    k_batch_post<<<n * fvl::kPostSlots, 256, 0, st>>>(items, ctl, prime);
     return hipGetLastError();
 }
 
