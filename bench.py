@@ -12,6 +12,7 @@ workload's length). The JSON line reports:
   arithmetic, FP64 CFL/balance reductions; the instance stepped by graph
   replays of the batch engine with the library's default tile);
 - fp64_plain: the FP64 plain method through the session step loop;
+- fp64_batched: the FP64 plain method as a graph-replayed batch of one;
 - roofline: the 3D step kernel's algorithmic HBM bytes (40 B per FP32
   cell-update: one read and one write of the five conserved fields) over
   its average launch time measured with HIP events on the batch's stream,
@@ -29,6 +30,9 @@ workload's length). The JSON line reports:
 - adaptive: the dyadic AMR leaves (2D and 3D Sod, left half refined twice),
   FP64 and FP32 leaf-updates/s of the device step, the host remesh time, and
   its own CPU baseline;
+- legacy: the legacy 1D driver with its full time history, with its own
+  CPU baseline;
+- checkpoint: save and load rates of a 256^3 FP64 session's checkpoint;
 - cpu_baseline: the NumPy oracle (a restatement of the reference's CPU path)
   timed on this host on a bounded sample of the same workload.
 
@@ -413,6 +417,69 @@ def cpu_adaptive_baseline(budget_s=8.0, nx=128, ny=16):
                       f"cfl+advance steps of the NumPy oracle, {wall:.1f} s"}
 
 
+def run_legacy(n, t_end):
+    """The legacy 1D driver with its full time history (every level
+    downloaded as its step ends): Sod on n cells, VanLeer, CFL 0.5."""
+    from torch_amd import run_simulation
+
+    t0 = time.perf_counter()
+    states, times = run_simulation(1.0, n, 1.0, 0.0, 1.0, 0.125, 0.0, 0.1, c_cfl=0.5, t_corte=t_end)
+    wall = time.perf_counter() - t0
+    steps = states.shape[2] - 1
+    return dict(value=round(n * steps / wall, 1), ms_per_step=round(1e3 * wall / steps, 4), steps=steps,
+                history_mb=round(states.nbytes / 1e6, 1), dtype="f64")
+
+
+def cpu_legacy_baseline(n, budget_s=6.0):
+    """The NumPy oracle's legacy driver on the same problem, shortened to a
+    bounded end time, single thread."""
+    os.environ.setdefault("OPENBLAS_NUM_THREADS", "1")
+    from oracle import legacy
+
+    t_end, wall, steps = 0.002, 0.0, 0
+    while wall < budget_s / 4 and t_end < 0.2:
+        t0 = time.perf_counter()
+        states, _ = legacy.run_simulation(1.0, n, 1.0, 0.0, 1.0, 0.125, 0.0, 0.1, c_cfl=0.5, t_corte=t_end)
+        wall = time.perf_counter() - t0
+        steps = states.shape[2] - 1
+        t_end *= 4
+    return {"value": round(n * steps / wall, 1), "unit": "cell-updates/s", "cores": 1, "kind": "port",
+            "sample": f"Sod {n} cells to t={t_end / 4:g}, {steps} steps of the NumPy oracle's driver, {wall:.1f} s"}
+
+
+def run_checkpoint(n):
+    """Save and load of a device session's checkpoint: the 3D Sod tube n^3
+    in FP64 after two steps, written to a temporary directory, read back
+    (with every check of the reader) and compared."""
+    import shutil
+    import tempfile
+
+    from torch_amd import Simulation
+
+    sim = Simulation(sod_primitive(n), [1.0, 1.0, 1.0], precision="fp64", **OPTIONS)
+    sim.step()
+    sim.step()
+    folder = tempfile.mkdtemp(prefix="torch_ckpt_")
+    try:
+        path = os.path.join(folder, "sod.npz")
+        t0 = time.perf_counter()
+        sim.save(path)
+        save_s = time.perf_counter() - t0
+        size = os.path.getsize(path)
+        t0 = time.perf_counter()
+        back = Simulation.load(path)
+        load_s = time.perf_counter() - t0
+        same = bool(np.array_equal(back.state, sim.state)) and back.time == sim.time
+        back.close()
+    finally:
+        shutil.rmtree(folder, ignore_errors=True)
+        sim.close()
+    if not same:
+        raise RuntimeError("checkpoint round trip changed the session")
+    return dict(bytes=size, save_s=round(save_s, 3), load_s=round(load_s, 3),
+                save_gb_s=round(size / save_s / 1e9, 3), load_gb_s=round(size / load_s / 1e9, 3))
+
+
 def copy_calibration(mib=1024, reps=3):
     """Device-to-device copies of a known size, for scaling rocprof's
     FETCH_SIZE/WRITE_SIZE counters (each copy reads and writes `mib` MiB)."""
@@ -435,7 +502,8 @@ def copy_calibration(mib=1024, reps=3):
     print(json.dumps({"copy_mib": mib, "reps": reps, "GBps_rw": round(2 * n * reps / wall / 1e9, 1)}))
 
 
-LEGS = ("fp64", "fp32", "batched", "error", "uneven", "served", "metric", "adaptive", "cpu")
+LEGS = ("fp64", "fp32", "batched", "error", "uneven", "served", "metric", "adaptive", "legacy", "checkpoint",
+        "cpu")
 
 
 def _pair(text):
@@ -482,6 +550,7 @@ def main(argv=None):
     ap.add_argument("--amr-steps", type=int, default=20)
     ap.add_argument("--amr-roots-3d", type=lambda t: tuple(int(v) for v in t.split(",")), default=(64, 16, 16),
                     help="NX,NY,NZ roots of the 3D adaptive workload")
+    ap.add_argument("--legacy-n", type=int, default=4096, help="cells of the legacy 1D history run")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device copies of MIB MiB (counter calibration) and exit")
     args = ap.parse_args(argv)
@@ -500,7 +569,7 @@ def main(argv=None):
         ap.error(f"unknown legs {sorted(unknown)}")
     n = args.n
     say = lambda msg: print(msg, file=sys.stderr, flush=True)  # noqa: E731
-    r64 = r32s = r32 = err = uneven = cpu = None
+    r64 = r64b = r32s = r32 = err = uneven = cpu = None
     if "fp64" in legs or "error" in legs:
         say(f"# 3D Sod {n}^3: FP64 plain, {args.warmup}+{args.steps} steps")
         r64 = run_precision(n, "fp64", args.steps, args.warmup)
@@ -511,6 +580,8 @@ def main(argv=None):
     if "batched" in legs:
         r32 = run_batched_sod(n, args.steps, args.warmup, tile=args.tile)
         say(f"# fp32 batched: {r32['ms_per_step']:.3f} ms/step, kernel {r32['kernel_ms']:.3f} ms, tile {r32['tile']}")
+        r64b = run_batched_sod(n, args.steps, args.warmup, precision="fp64")
+        say(f"# fp64 batched: {r64b['ms_per_step']:.3f} ms/step, kernel {r64b['kernel_ms']:.3f} ms, tile {r64b['tile']}")
     if "error" in legs:
         err = fp32_error(n, args.warmup, args.error_steps, r64["dts"][: args.error_steps])
         say(f"# fp32 L1 rel vs fp64 after {args.error_steps} steps: {err}")
@@ -570,6 +641,20 @@ def main(argv=None):
                 f"{adaptive[precision + '_3d']['leaves']} leaves")
         if "cpu" in legs:
             adaptive["cpu_baseline"] = cpu_adaptive_baseline()
+    legacy = None
+    if "legacy" in legs:
+        ln = args.legacy_n
+        legacy = {"workload": f"legacy 1D driver run_simulation with its full time history: Sod on {ln} cells, "
+                              "VanLeer, CFL 0.5, open ends, t_corte 0.2, every level downloaded",
+                  "unit": "cell-updates/s", **run_legacy(ln, 0.2)}
+        say(f"# legacy 1D: {legacy['ms_per_step']} ms/step, {legacy['steps']} levels")
+        if "cpu" in legs:
+            legacy["cpu_baseline"] = cpu_legacy_baseline(ln)
+    checkpoint = None
+    if "checkpoint" in legs:
+        checkpoint = {"workload": f"checkpoint save + load (every reader check) of the 3D Sod tube {n}^3 FP64 "
+                                  "session, NumPy .npz container", **run_checkpoint(n)}
+        say(f"# checkpoint: save {checkpoint['save_gb_s']} GB/s, load {checkpoint['load_gb_s']} GB/s")
     if "cpu" in legs:
         cpu = cpu_baseline()
     head = r32 or r32s
@@ -588,6 +673,10 @@ def main(argv=None):
             "value": round(r64["value"], 1), "ms_per_step": round(r64["ms_per_step"], 4), "dtype": "f64",
             "kernel_ms": round(r64["kernel_ms"], 4), "roofline": roofline(n, 80, r64["kernel_ms"], "f64", DEFAULT_TILE["fp64"]),
             "fallbacks": r64["fallbacks"]},
+        "fp64_batched": None if not r64b else {
+            "value": round(r64b["value"], 1), "ms_per_step": round(r64b["ms_per_step"], 4), "dtype": "f64",
+            "isolated_kernel_ms": round(r64b["kernel_ms"], 4), "tile": list(r64b["tile"]),
+            "fallbacks": r64b["fallbacks"]},
         "fp32_session_loop": None if not r32s else {
             "value": round(r32s["value"], 1), "ms_per_step": round(r32s["ms_per_step"], 4),
             "kernel_ms": round(r32s["kernel_ms"], 4)},
@@ -598,6 +687,8 @@ def main(argv=None):
         "served": served,
         "metric": metric,
         "adaptive": adaptive,
+        "legacy": legacy,
+        "checkpoint": checkpoint,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))
