@@ -5,7 +5,7 @@
 //
 // k_batch_check (reactive instances, after the first chemistry half-step):
 // the CFL test of the heated state and the dt halving of the retry loop
-// (:398-411). A replay holds a few attempt/check pairs; an accepted attempt
+// (:398-411), on the maxima folded like the post kernel's (below). A replay holds a few attempt/check pairs; an accepted attempt
 // opens the hydro gate and later attempts of the replay skip the instance.
 // An instance still rejected after the last pair commits nothing in this
 // replay and continues the loop in the next one from the same committed
@@ -49,42 +49,6 @@ __device__ __forceinline__ double dt_of(const fvl::BatchCtl& c, const double* mx
 
 __device__ __forceinline__ double py_min(double a, double b) { return b < a ? b : a; }
 
-// Folds slots [0, nslot) of nb per-block partials like the single-session
-// finalize: 256 strided partial results, then a tree (maxima for slots
-// 0-2, sums for the rest). Every thread of the block must call it; thread 0
-// reads the results from red[slot][0].
-//
-// Every slot of a partial row is folded in the same pass (one contiguous
-// row per block instead of one strided column per slot), and the block loop
-// is unrolled so the loads of several rows are in flight together; each
-// slot still accumulates its rows in increasing order, as finalize does.
-__device__ void fold(const double* part, int nb, int nslot, double (*red)[256]) {
-    const int t = threadIdx.x;
-    double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-#pragma unroll 4
-    for (int b = t; b < nb; b += 256) {
-        const double* row = part + (long)b * fvl::NSLOT;
-#pragma unroll
-        for (int slot = 0; slot < 9; ++slot) {
-            if (slot >= nslot) break;
-            const double p = row[slot];
-            v[slot] = slot < 3 ? fmax(v[slot], p) : v[slot] + p;
-        }
-    }
-#pragma unroll
-    for (int slot = 0; slot < 9; ++slot)
-        if (slot < nslot) red[slot][t] = v[slot];
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (t < s)
-            for (int slot = 0; slot < nslot; ++slot) {
-                double a = red[slot][t], b = red[slot][t + s];
-                red[slot][t] = slot < 3 ? fmax(a, b) : a + b;
-            }
-        __syncthreads();
-    }
-}
-
 // One slot of nb per-block partials folded by a 256-thread block in the
 // single-session finalize's order (k_finalize): lane t accumulates the rows
 // t, t+256, ... in increasing order, then a tree over the 256 lanes.
@@ -126,22 +90,32 @@ __device__ __forceinline__ void stop(fvl::BatchCtl& c, int status) {
     c.coef[fvl::K_CHEM] = 0.0;
 }
 
+// Three blocks per instance, one per wave-speed maximum, with the post
+// kernel's ticket: the block that takes the last ticket decides.
 __global__ void __launch_bounds__(256) k_batch_check(fvl::BatchItem* items, fvl::BatchCtl* ctl) {
 #pragma clang fp contract(off)
-    __shared__ double red[3][256];
-    const int i = blockIdx.x;
+    __shared__ double buf[256];
+    __shared__ int last;
+    const int i = blockIdx.x / 3, k = blockIdx.x % 3, t = threadIdx.x;
     fvl::BatchCtl& c = ctl[i];
-    // Stopped, or this step's dt already accepted by an earlier attempt.
+    // Stopped, or this step's dt already accepted by an earlier attempt
+    // (every block of the instance takes the same exit).
     if (c.active != 1 || c.coef[fvl::K_CHEM] == 0.0 || c.coef[fvl::K_HYDRO] != 0.0) return;
     fvl::BatchItem& it = items[i];
-    fold(it.step.partials + (long)it.r1_first * fvl::NSLOT, c.nreact, 3, red);
-    if (threadIdx.x != 0) return;
+    const double v = fold_slot(it.step.partials + (long)it.r1_first * fvl::NSLOT, c.nreact, k, buf);
+    if (t == 0) {
+        c.folded[k] = v;
+        last = __hip_atomic_fetch_add(&c.ticket, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == 2;
+    }
+    __syncthreads();
+    if (!last || t != 0) return;
+    c.ticket = 0;
     int* flags = it.step.flags;
     if (*flags & fvl::F_REACT_BAD) {
         stop(c, TORCH_INVALID);
         return;
     }
-    const double mx[3] = {red[0][0], red[1][0], red[2][0]};
+    const double mx[3] = {c.folded[0], c.folded[1], c.folded[2]};
     const double dt = c.dt, allowed = dt_of(c, mx);
     *flags = 0;  // the hydro pass (or the next attempt) starts from clear flags
     if (dt <= allowed * (1 + 1e-12)) {
@@ -268,7 +242,7 @@ hipError_t batch_post(fvl::BatchItem* items, fvl::BatchCtl* ctl, int n, int prim
 
 hipError_t batch_check(fvl::BatchItem* items, fvl::BatchCtl* ctl, int n, hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    k_batch_check<<<n, 256, 0, st>>>(items, ctl);
+    k_batch_check<<<n * 3, 256, 0, st>>>(items, ctl);
     return hipGetLastError();
 }
 
