@@ -835,7 +835,12 @@ int torch_state_hydro(torch_state* s, double dt, int want_fluxes, torch_step_inf
     info->fallback = fb;
     for (int f = 0; f < s->nf; ++f) info->outward[f] = hv[fvl::S_OUT + f];
     s->cur = out;
-    s->maxima_valid = false;
+    // 1D/2D scalar-gamma inert: run_hydro already folded the CFL maxima of
+    // the new state (the pass compute_maxima would launch), so the next
+    // dt needs no launch and no wait.
+    s->maxima_valid = s->cfg.dim != 3 && !s->cfg.reactive && s->cfg.gamma_mode == 0;
+    if (s->maxima_valid)
+        for (int m = 0; m < 3; ++m) s->maxima[m] = s->host[3 * fvl::NSLOT + m];
     return TORCH_OK;
 }
 
