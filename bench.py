@@ -213,7 +213,7 @@ def run_served(seed, count, t_run, budget_gib, round_steps, precision="fp32", op
 # FETCH_SIZE and WRITE_SIZE counter runs scaled by a 1 GiB device copy
 # (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000); the commands and per-counter
 # values are kept with the profile summaries.
-TRAFFIC = {("f32", 256, (32, 8, 16)): 1557912111, ("f64", 256, (32, 16, 16)): 2307245692}
+TRAFFIC = {("f32", 256, (32, 8, 16)): 1563068992, ("f64", 256, (32, 16, 16)): 2288424448}
 
 # The library's default tiles (BX, BY, z chunk) per precision.
 DEFAULT_TILE = {"fp32": (32, 8, 16), "fp64": (32, 16, 16)}
