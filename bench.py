@@ -209,14 +209,15 @@ def run_served(seed, count, t_run, budget_gib, round_steps, precision="fp32", op
 
 
 # HBM bytes per launch of the 3D step kernel on the 256^3 Sod tube with the
-# default tiles (FP32 32x8, FP64 32x16, z chunk 16), from rocprofv3
+# default tiles (FP32 32x8, FP64 32x16, z chunks of 24, and of 16), from rocprofv3
 # FETCH_SIZE and WRITE_SIZE counter runs scaled by a 1 GiB device copy
 # (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000); the commands and per-counter
 # values are kept with the profile summaries.
-TRAFFIC = {("f32", 256, (32, 8, 16)): 1563068992, ("f64", 256, (32, 16, 16)): 2288424448}
+TRAFFIC = {("f32", 256, (32, 8, 24)): 1503564928, ("f64", 256, (32, 16, 24)): 2208221664,
+           ("f32", 256, (32, 8, 16)): 1563068992, ("f64", 256, (32, 16, 16)): 2288424448}
 
 # The library's default tiles (BX, BY, z chunk) per precision.
-DEFAULT_TILE = {"fp32": (32, 8, 16), "fp64": (32, 16, 16)}
+DEFAULT_TILE = {"fp32": (32, 8, 24), "fp64": (32, 16, 24)}
 
 
 def roofline(n, bytes_per_cell, kernel_ms, dtype=None, tile=None):

@@ -651,7 +651,7 @@ int torch_batch_tune(torch_batch* b, int reps, int* block_x, int* block_y, int* 
     double best = -1;
     if (b->seg[1] > b->seg[0]) {
         static const int shapes[4][2] = {{64, 4}, {64, 8}, {32, 8}, {32, 16}};
-        static const int chunks[4] = {8, 16, 32, 64};
+        static const int chunks[5] = {8, 16, 24, 32, 64};
         int bbx = b->bx, bby = b->by, bz = b->zchunk;
         hipEvent_t e0, e1;
         HIP_TRY(hipEventCreate(&e0));
