@@ -52,7 +52,7 @@ __device__ __forceinline__ double py_min(double a, double b) { return b < a ? b 
 // One slot of nb per-block partials folded by a 256-thread block in the
 // single-session finalize's order (k_finalize): lane t accumulates the rows
 // t, t+256, ... in increasing order, then a tree over the 256 lanes.
-// Every thread must call it; the total is returned to thread 0.
+// Every thread of the block must call it; every thread gets the total.
 __device__ double fold_slot(const double* part, int nb, int slot, double* buf) {
     const int t = threadIdx.x;
     const bool is_max = slot < 3;
