@@ -113,6 +113,24 @@ def test_batch_fp64_matches_sessions_bitwise():
     batch.close()
 
 
+def test_deep_grid_batch_of_one_matches_session_bitwise():
+    """A grid 128 planes deep takes the longer default z chunk in a session
+    and in a batch alike, so the balance sums fold the same blocks."""
+    from torch_amd import Simulation
+    from torch_amd.torchBatch import Batch
+
+    rng = np.random.default_rng(11)
+    for precision in ("fp64", "fp32"):
+        sim = Simulation(_two_state(3, (24, 20, 130), rng, 1.4), [1.0, 1.0, 5.0], t_end=10.0, cfl=0.5,
+                         limiter="Superbee", boundaries=["Open", "Reflective", "Open"], precision=precision)
+        batch = Batch(precision=precision)
+        ident = batch.add(sim)
+        batch.run(6)
+        _check_same(batch, ident, _alone(sim, 6))
+        batch.close()
+        sim.close()
+
+
 def test_batch_fp32_matches_sessions():
     from torch_amd.torchBatch import Batch
     sims = _sessions("fp32", seed=6)

@@ -42,6 +42,7 @@ struct Inst {
     bool fresh = true;  // first dt not derived yet
     // layout of the current build
     int tiles[3] = {0, 0, 0};
+    int zchunk = 16;  // z chunk of its 3D work-list blocks
     int nblk = 0, nllf = 0, ncfl = 0, nreact = 0;  // blocks of the passes
     long pslot = 0;                    // first partial slot
 };
@@ -92,6 +93,7 @@ struct torch_batch {
     int pseg[4] = {0, 0, 0, 0};
     hipGraphExec_t graph = nullptr;
     int bx = 32, by = 16, zchunk = 16;  // tile of the 3D work lists (by: see create)
+    bool tiles_set = false;  // zchunk chosen by set_tiles or tune; else default_zchunk per instance
     bool profile = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double prof_ms = 0;
@@ -129,7 +131,8 @@ void layout(torch_batch* b, Inst* p) {
     if (g.dim == 3) {
         p->tiles[0] = (g.nx + b->bx - 3) / (b->bx - 2);
         p->tiles[1] = (g.ny + b->by - 3) / (b->by - 2);
-        p->tiles[2] = (g.nz + b->zchunk - 1) / b->zchunk;
+        p->zchunk = b->tiles_set ? b->zchunk : default_zchunk(g.nz);
+        p->tiles[2] = (g.nz + p->zchunk - 1) / p->zchunk;
         p->nblk = p->tiles[0] * p->tiles[1] * p->tiles[2];
         p->nllf = o->llf3d_blocks(g);
         p->ncfl = o->cfl_blocks(g);
@@ -360,7 +363,7 @@ int rebuild(torch_batch* b) {
         s.gamma = p->cfg.gamma;
         s.flags = b->flags + (long)i * kFlagStride;
         s.partials = b->partials + p->pslot * fvl::NSLOT;
-        s.zchunk = b->zchunk;
+        s.zchunk = p->zchunk;
         s.limiter = p->cfg.limiter;
         s.want_cfl = p->grid.dim >= 2;
         s.controller = p->cfg.controller;
@@ -653,6 +656,7 @@ int torch_batch_tune(torch_batch* b, int reps, int* block_x, int* block_y, int* 
         static const int shapes[4][2] = {{64, 4}, {64, 8}, {32, 8}, {32, 16}};
         static const int chunks[5] = {8, 16, 24, 32, 64};
         int bbx = b->bx, bby = b->by, bz = b->zchunk;
+        b->tiles_set = true;
         hipEvent_t e0, e1;
         HIP_TRY(hipEventCreate(&e0));
         HIP_TRY(hipEventCreate(&e1));
@@ -733,6 +737,7 @@ int torch_batch_set_tiles(torch_batch* b, int block_x, int block_y, int zchunk) 
     b->bx = block_x;
     b->by = block_y;
     b->zchunk = zchunk;
+    b->tiles_set = true;
     b->dirty = true;
     drop_graph(b);
     return TORCH_OK;

@@ -644,15 +644,12 @@ int torch_state_create(torch_ctx* ctx, const torch_config* cfg, torch_state** ou
     // Tuned default tile: FP64 32x16 threads (30x14 cells per plane, two
     // waves per SIMD at 256 VGPRs); FP32 32x8 (three 256-thread blocks per
     // CU at <= 168 VGPRs). The fastest shapes of the autotuner's sweep on
-    // MI355X from 96^3 to 256^3. Blocks march z chunks of 24 planes where
-    // the grid is deep enough to fill the chip with them (a z-chunk sweep
-    // of the 256^3 Sod step, 10 to 32 planes, is fastest at 24 in both
-    // precisions: 4% below 16), else 16.
+    // MI355X from 96^3 to 256^3; z chunks from default_zchunk.
     if (s->cfg.block_x == 0 || s->cfg.block_y == 0) {
         s->cfg.block_x = 32;
         s->cfg.block_y = c.precision == TORCH_FP32 ? 8 : 16;
     }
-    if (s->cfg.zchunk <= 0) s->cfg.zchunk = c.dim == 3 && c.n[2] >= 128 ? 24 : 16;
+    if (s->cfg.zchunk <= 0) s->cfg.zchunk = default_zchunk(c.dim == 3 ? c.n[2] : 1);
     fvl::Grid& g = s->grid;
     make_grid(c, &g, &s->volume);
     s->ncell = (long)g.nx * g.ny * g.nz;

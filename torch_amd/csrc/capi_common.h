@@ -25,6 +25,13 @@ struct Ops;
 int upload_as(const Ops* ops, hipStream_t st, void* dst, const double* src, long n);
 int download_as(const Ops* ops, hipStream_t st, double* dst, const void* src, long n);
 
+// Default z chunk of the 3D step for a grid nz planes deep, shared by
+// sessions and batches so that a batch of one folds its partials in the
+// same blocks as the session: 24 planes where the grid is deep enough to
+// fill the chip with such chunks (fastest in a 10-32 plane sweep of the
+// 256^3 Sod step in both precisions), else 16.
+inline int default_zchunk(int nz) { return nz >= 128 ? 24 : 16; }
+
 #define HIP_TRY(expr)                                                                              \
     do {                                                                                           \
         hipError_t e_ = (expr);                                                                    \
