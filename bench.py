@@ -24,7 +24,7 @@ workload's length). The JSON line reports:
 - served: seeded 3D ignition requests through the continuous-batching
   server (closed loop for capacity, then Poisson arrivals at 80% of it):
   p50/p99 time-to-solution and aggregate cell-updates/s;
-- metric: the metric/face scheme (curvilinear grid, face boundaries,
+- metric_face: the metric/face scheme (curvilinear grid, face boundaries,
   SSPRK2 MUSCL-Rusanov) on a cylindrical-sector blast, FP64 and FP32, with
   its own CPU baseline;
 - adaptive: the dyadic AMR leaves (2D and 3D Sod, left half refined twice),
@@ -686,7 +686,7 @@ def main(argv=None):
         "fp32_l1_rel": [float(f"{e:.3e}") for e in err] if err else None,
         "batched": uneven,
         "served": served,
-        "metric": metric,
+        "metric_face": metric,
         "adaptive": adaptive,
         "legacy": legacy,
         "checkpoint": checkpoint,

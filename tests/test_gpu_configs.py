@@ -10,7 +10,8 @@
   steps, FP32 against FP64 on the device, and the balance of mass and energy.
 - 3D pressure sphere, Superbee, CFL 0.4, open: FP64 and FP32 against the
   oracle at 48^3 (10 steps); at the full 192^3, 100 steps, FP32 against
-  FP64, the balances, and the sphere's symmetry (x, y and z exchanged;
+  FP64 (1e-5 relative L1 on every field, momenta included), the balances,
+  and the sphere's symmetry (x, y and z exchanged;
   1e-10, since the update sums the directions in a fixed order).
 
 The per-field errors are printed (the numbers stated for these workloads).
@@ -96,7 +97,7 @@ def test_quadrant2d_1024_fp32_against_fp64():
     err = l1_rel(d32.state, d64.state)
     print("quadrant 1024^2, 100 steps, FP32 vs FP64 L1 rel per field:", err,
           "balance FP64:", balance(d64), "FP32:", balance(d32))
-    assert np.all(err < 1e-4)
+    assert np.all(err < 1e-5)
     assert np.all(balance(d64)[[0, 3]] < 1e-12) and np.all(balance(d32)[[0, 3]] < 1e-5)
     d64.close()
     d32.close()
@@ -112,7 +113,7 @@ def test_sphere3d_against_the_oracle():
     dev32, _ = pair(prim, [1.0] * 3, 10, precision="fp32", oracle=False, **opts)
     err = l1_rel(dev32.state, ref.state)
     print("sphere 48^3 FP32 vs oracle L1 rel per field:", err)
-    assert np.all(err[[0, 4]] < 1e-5)
+    assert np.all(err < 1e-5)
     dev.close()
     dev32.close()
 
@@ -128,7 +129,7 @@ def test_sphere3d_192_fp32_against_fp64_and_symmetry():
     err = l1_rel(s32, s64)
     print("sphere 192^3, 100 steps, FP32 vs FP64 L1 rel per field:", err,
           "balance FP64:", balance(d64), "FP32:", balance(d32))
-    assert np.all(err[[0, 4]] < 1e-4)
+    assert np.all(err < 1e-5)
     assert np.all(balance(d64)[[0, 4]] < 1e-12)
     # State axes are (z, y, x): exchanging two axes exchanges the matching momenta.
     rho, e = s64[0], s64[4]

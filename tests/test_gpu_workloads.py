@@ -1,0 +1,244 @@
+"""The benchmark workloads at their own sizes, as parity cases.
+
+- 3D Sod tube 256^3 (torchSolver3D.py:271-274, ``3d_sod_x``), MINBEE,
+  open x ends, reflective y/z walls, CFL 0.8 (outside the reference's
+  validated range, run unchecked), 200 steps, FP64 plain method and FP32
+  form. The state is uniform in y and z and the walls mirror it, so every
+  (y, z) column carries the arithmetic of a 256 x 4 x 4 replica with the
+  same cell sizes (lengths 1 x 4/256 x 4/256), which the oracle runs in
+  seconds: every column of the device state must equal every other, the
+  FP64 column must equal the replica's to 1e-12 per field with the same dt
+  sequence and fallback count, and the FP32 column must stay within 1e-5
+  relative L1 of it on every field with a nonzero norm (Linf and the
+  conservation drift printed per field).
+- The batched uneven mix: 256 seeded problems (rng 508; 1D N in 256..8192,
+  2D 64^2..512^2, 3D 32^3..160^3, random limiter and gamma, open), 100 steps
+  in FP32 and FP64 batches: every instance's FP32 state within 1e-5 relative
+  L1 of its FP64 state per field, and the smallest 1D, 2D and 3D instances
+  of the FP64 batch equal to the oracle's session to 1e-12.
+- Served 3D ignition requests drawn at 32^3..160^3 (Arrhenius and ZND):
+  each result the same (bitwise) alone and in the mix in both precisions,
+  and an Arrhenius and a ZND request from the low end of the range served
+  in FP64 equal to the oracle's reactive session (torchSession.py:374-522,
+  the dt-halving retry included) to 1e-12.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import session as osession
+from parity import assert_fields
+
+pytestmark = pytest.mark.gpu
+
+SOD = dict(t_end=10.0, cfl=0.8, limiter="MINBEE", boundaries=["Open", "Reflective", "Reflective"], checked=False)
+SOD_N, SOD_STEPS = 256, 200
+
+
+def say(*args):
+    print(*args, flush=True)
+
+
+@pytest.fixture(scope="module")
+def sod_replica():
+    """The oracle's 256 x 4 x 4 replica: final state, dt sequence, fallbacks, balances."""
+    from torch_amd.workloads import sod_3d
+
+    prim = np.ascontiguousarray(sod_3d(SOD_N)[:, :4, :4, :])
+    ref = osession.Session(prim, [1.0, 4 / SOD_N, 4 / SOD_N], **SOD)
+    dts = []
+    for _ in range(SOD_STEPS):
+        t0 = ref.time
+        ref.step()
+        dts.append(ref.time - t0)
+    col = ref.state[:, 0, 0, :].copy()
+    assert np.array_equal(ref.state, np.broadcast_to(col[:, None, None, :], ref.state.shape))
+    return dict(column=col, dts=np.array(dts), time=ref.time, fallbacks=ref.fallbacks)
+
+
+def run_sod(precision):
+    from torch_amd import Simulation
+    from torch_amd.workloads import sod_3d
+
+    sim = Simulation(sod_3d(SOD_N), [1.0, 1.0, 1.0], precision=precision, **SOD)
+    dts = []
+    for _ in range(SOD_STEPS):
+        t0 = sim.time
+        sim.step()
+        dts.append(sim.time - t0)
+    return sim, np.array(dts)
+
+
+def columns_equal(state):
+    """Every (y, z) column equals the first (values; +-0 compare equal)."""
+    return np.array_equal(state, np.broadcast_to(state[:, :1, :1, :], state.shape))
+
+
+def drift(sim):
+    """Conservation residual per field (final - initial + net outflow),
+    relative to the largest of the three terms."""
+    scale = np.maximum.reduce([np.abs(sim.initial_integral), np.abs(sim.integral), np.abs(sim.boundary_integral)])
+    return np.abs(sim.balance_residual) / np.maximum(scale, 1e-300)
+
+
+def test_sod256_fp64_200_steps_against_the_replica(sod_replica):
+    sim, dts = run_sod("fp64")
+    try:
+        say(f"Sod 256^3 FP64: {sim.steps} steps, t = {sim.time!r}, fallbacks {sim.fallbacks}")
+        assert sim.fallbacks == sod_replica["fallbacks"]
+        np.testing.assert_allclose(dts, sod_replica["dts"], rtol=1e-13, atol=0)
+        assert sim.time == pytest.approx(sod_replica["time"], rel=1e-13)
+        state = sim.state
+        assert columns_equal(state)
+        err = assert_fields(state[:, 0, 0, :], sod_replica["column"], what="Sod 256^3 FP64 column")
+        say("Sod 256^3 FP64 column vs oracle replica, max rel per field:", err, "drift per field:", drift(sim))
+        assert np.all(drift(sim)[[0, 1, 4]] < 1e-12)
+    finally:
+        sim.close()
+
+
+def test_sod256_fp32_200_steps_against_the_replica(sod_replica):
+    sim, dts = run_sod("fp32")
+    try:
+        state = sim.state
+        assert columns_equal(state)
+        col, ref = state[:, 0, 0, :], sod_replica["column"]
+        l1, linf = [], []
+        for f in range(5):
+            norm = np.sum(np.abs(ref[f]))
+            if norm == 0:
+                assert np.all(col[f] == 0), f"field {f} should stay zero"
+                l1.append(0.0)
+                linf.append(0.0)
+                continue
+            l1.append(np.sum(np.abs(col[f] - ref[f])) / norm)
+            linf.append(np.max(np.abs(col[f] - ref[f])) / np.max(np.abs(ref[f])))
+        d = drift(sim)
+        say(f"Sod 256^3 FP32: t = {sim.time!r} (oracle {sod_replica['time']!r}), fallbacks {sim.fallbacks}")
+        say("  L1 rel per field:", np.array(l1))
+        say("  Linf rel per field:", np.array(linf))
+        say("  conservation drift per field:", d)
+        assert np.all(np.array(l1) <= 1e-5)
+        assert sim.fallbacks == sod_replica["fallbacks"]
+        assert np.all(d[[0, 1, 4]] < 1e-5)
+    finally:
+        sim.close()
+
+
+def test_uneven_mix_at_its_sizes():
+    from torch_amd.torchBatch import Batch
+    from torch_amd.workloads import uneven_problems
+
+    steps = 100
+    batches, ids, smallest = {}, {}, {}
+    try:
+        for precision in ("fp64", "fp32"):
+            batch = Batch(precision=precision)
+            ids[precision] = []
+            for i, prim, opts in uneven_problems(508, 256):
+                ident = batch.add_problem(prim, opts["lengths"], t_end=100.0, cfl=opts["cfl"], gamma=opts["gamma"],
+                                          limiter=opts["limiter"], boundaries=opts["boundaries"],
+                                          max_steps=steps)
+                ids[precision].append(ident)
+                dim, cells = prim.ndim - 1, prim[0].size
+                if precision == "fp64" and (dim not in smallest or cells < smallest[dim][1]):
+                    smallest[dim] = (i, cells, prim, opts)
+            batch.run(steps)
+            batches[precision] = batch
+            say(f"uneven mix {precision}: {steps} steps done")
+        worst = np.zeros(5)
+        for i, (a, b) in enumerate(zip(ids["fp32"], ids["fp64"])):
+            i32, i64 = batches["fp32"].info(a), batches["fp64"].info(b)
+            assert i32["status"] == 0 and i64["status"] == 0, (i, i32, i64)
+            assert i32["steps"] == steps and i64["steps"] == steps, (i, i32["steps"], i64["steps"])
+            x, y = batches["fp32"].state(a), batches["fp64"].state(b)
+            for f, (u, v) in enumerate(zip(x, y)):
+                norm = np.sum(np.abs(v))
+                if norm > 0:
+                    e = np.sum(np.abs(u - v)) / norm
+                    worst[f] = max(worst[f], e)
+                    assert e <= 1e-5, f"instance {i} field {f}: FP32 vs FP64 L1 {e:.3e}"
+        say("uneven mix FP32 vs FP64, worst L1 rel per field over 256 instances:", worst)
+        for dim, (i, cells, prim, opts) in sorted(smallest.items()):
+            ref = osession.Session(prim, opts["lengths"], t_end=100.0, cfl=opts["cfl"], gamma=opts["gamma"],
+                                   limiter=opts["limiter"], boundaries=opts["boundaries"])
+            for _ in range(steps):
+                ref.step()
+            info = batches["fp64"].info(ids["fp64"][i])
+            got = batches["fp64"].state(ids["fp64"][i])
+            err = assert_fields(got, ref.state, what=f"instance {i} ({opts['desc']})")
+            assert info["fallbacks"] == ref.fallbacks
+            assert info["time"] == pytest.approx(ref.time, rel=1e-13)
+            say(f"instance {i} ({opts['desc']}, {cells} cells) vs oracle: max rel per field {err}")
+    finally:
+        for b in batches.values():
+            b.close()
+
+
+def _served_specs():
+    from torch_amd.workloads import served_requests
+
+    specs = served_requests(seed=355, count=6, t_run=0.02, znd_fraction=0.34, sizes=(32, 160))
+    assert any(s["model"] == "znd" for s in specs)
+    assert min(s["n"] for s in specs) >= 32 and max(s["n"] for s in specs) > 100
+    return specs
+
+
+def _serve(specs, arrivals, precision):
+    from torch_amd.torchServe import Server, ignition_request
+
+    server = Server(precision=precision, hbm_budget=0, round_steps=4, keep_results=True)
+    try:
+        reqs = server.serve([ignition_request(s, a) for s, a in zip(specs, arrivals)])
+    finally:
+        server.close()
+    return {r.ident: r for r in reqs}
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_served_full_sizes_same_alone_and_in_mix(precision):
+    specs = _served_specs()
+    mixed = _serve(specs, [0.0, 0.0, 0.002, 0.004, 0.006, 0.008], precision)
+    for spec in specs:
+        alone = _serve([spec], [0.0], precision)[spec["index"]]
+        got = mixed[spec["index"]]
+        assert got.status == 0 and alone.status == 0
+        assert got.steps == alone.steps > 0
+        say(f"request {spec['index']} n={spec['n']} {spec['model']}: {got.steps} steps, {got.retries} retries")
+        for a, b in zip(got.result, alone.result):
+            if b is not None:
+                assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("model", ["arrhenius", "znd"])
+def test_served_fp64_request_matches_the_oracle(model):
+    """Requests from the low end of the size range (rng 356, n in 32..40,
+    t_run 0.02); the Arrhenius one needs dt halvings on its first steps."""
+    from torch_amd.torchServe import ignition_request
+    from torch_amd.workloads import served_requests
+
+    specs = served_requests(seed=356, count=8, t_run=0.02, znd_fraction=0.1, sizes=(32, 40))
+    spec = next(s for s in specs if s["model"] == model)
+    prim, opts = ignition_request(spec, 0.0).build()
+    ref = osession.Session(prim, [1.0] * 3, t_end=opts["t_end"], cfl=opts["cfl"], limiter=opts["limiter"],
+                           reactive=True, kinetic_model=opts["kinetic_model"], lambda_init=opts["lambda_init"],
+                           constants=opts["constants"])
+    halved = 0
+    while not ref.completed:
+        limit = min(ref.dt_limit(ref.state, ref.gamma), opts["t_end"] - ref.time)
+        t0 = ref.time
+        ref.step()
+        halved += int(ref.time - t0 < limit)
+    served = _serve([spec], [0.0], "fp64")[spec["index"]]
+    say(f"served request n={spec['n']} {model}: {served.steps} steps ({ref.steps} oracle), "
+        f"{served.retries} halvings ({halved} halved oracle steps)")
+    assert served.status == 0
+    assert served.steps == ref.steps
+    assert (served.retries > 0) == (halved > 0)
+    if model == "arrhenius":
+        assert halved > 0
+    state, lam, beta = served.result
+    assert_fields(state, ref.state, what="served state")
+    assert np.max(np.abs(lam - ref.lam)) <= 1e-12
+    if ref.beta is not None:
+        assert np.max(np.abs(beta - ref.beta)) <= 1e-12

@@ -30,8 +30,10 @@ enum Boundary { BC_OPEN = 0, BC_PERIODIC = 1, BC_REFLECTIVE = 2, BC_INFLOW = 3 }
 __device__ __forceinline__ real rmax(real a, real b) { return fmax(a, b); }
 __device__ __forceinline__ real rmin(real a, real b) { return fmin(a, b); }
 // NumPy's maximum/minimum propagate NaN; so do these (a NaN operand wins).
-__device__ __forceinline__ real nmax(real a, real b) { return (a != a || b != b) ? (a + b) : (a > b ? a : b); }
-__device__ __forceinline__ real nmin(real a, real b) { return (a != a || b != b) ? (a + b) : (a < b ? a : b); }
+template <typename T>
+__device__ __forceinline__ T nmax(T a, T b) { return (a != a || b != b) ? (a + b) : (a > b ? a : b); }
+template <typename T>
+__device__ __forceinline__ T nmin(T a, T b) { return (a != a || b != b) ? (a + b) : (a < b ? a : b); }
 
 // |x| < 1e-14 -> +1e-14 (torchFlux.py:190, torchSolver3D.py:90).
 __device__ __forceinline__ real guard14(real x) { return fabs(x) < (real)1e-14 ? (real)1e-14 : x; }
@@ -54,42 +56,54 @@ __device__ __forceinline__ bool ghost_flip(int i, int n, int kind) {
     return kind == BC_REFLECTIVE && (i < 0 || i >= n);
 }
 
-// Limiter phi(r) of the 2D/3D MUSCL schemes (torchSolver3D.py:91-110).
-template <int LIM>
-__device__ __forceinline__ real phi3(real r) {
+// Limiter phi(r) of the 2D/3D MUSCL schemes (torchSolver3D.py:91-110), in
+// the arithmetic type T of the caller.
+template <int LIM, typename T>
+__device__ __forceinline__ T phi3(T r) {
     if (LIM == VANLEER) {
-        real pos = nmax(r, (real)0);
-        return ((real)2 * pos) / ((real)1 + pos);
+        T pos = nmax(r, (T)0);
+        return ((T)2 * pos) / ((T)1 + pos);
     } else if (LIM == ALBADA) {
-        real pos = nmax(r, (real)0);
-        return (pos * pos + pos) / (pos * pos + (real)1);
+        T pos = nmax(r, (T)0);
+        return (pos * pos + pos) / (pos * pos + (T)1);
     } else if (LIM == MINBEE) {
-        return nmax((real)0, nmin(r, (real)1));
+        return nmax((T)0, nmin(r, (T)1));
     } else if (LIM == SUPERBEE) {
-        return nmax((real)0, nmax(nmin((real)2 * r, (real)1), nmin(r, (real)2)));
+        return nmax((T)0, nmax(nmin((T)2 * r, (T)1), nmin(r, (T)2)));
     } else {
-        return nmax((real)0, nmin(nmin(nmin((real)2, (real)2 * r), ((real)2 + r) / (real)3),
-                                  ((real)1 + (real)2 * r) / (real)3));
+        return nmax((T)0, nmin(nmin(nmin((T)2, (T)2 * r), ((T)2 + r) / (T)3), ((T)1 + (T)2 * r) / (T)3));
     }
 }
 
 // 2D limiters differ: VanLeer/Albada use where(r > 0, ..., 0) and Unos has no
 // (1+2r)/3 bound (torchSolver2D.py:144-168).
-template <int LIM>
-__device__ __forceinline__ real phi2(real r) {
+template <int LIM, typename T>
+__device__ __forceinline__ T phi2(T r) {
     if (LIM == VANLEER) {
-        real pos = nmax(r, (real)0);
-        return r > (real)0 ? ((real)2 * pos) / ((real)1 + pos) : (real)0;
+        T pos = nmax(r, (T)0);
+        return r > (T)0 ? ((T)2 * pos) / ((T)1 + pos) : (T)0;
     } else if (LIM == ALBADA) {
-        real pos = nmax(r, (real)0);
-        return r > (real)0 ? (pos * pos + pos) / (pos * pos + (real)1) : (real)0;
+        T pos = nmax(r, (T)0);
+        return r > (T)0 ? (pos * pos + pos) / (pos * pos + (T)1) : (T)0;
     } else if (LIM == MINBEE) {
-        return nmax((real)0, nmin(r, (real)1));
+        return nmax((T)0, nmin(r, (T)1));
     } else if (LIM == SUPERBEE) {
-        return nmax((real)0, nmax(nmin((real)2 * r, (real)1), nmin(r, (real)2)));
+        return nmax((T)0, nmax(nmin((T)2 * r, (T)1), nmin(r, (T)2)));
     } else {
-        return nmax((real)0, nmin(nmin((real)2, (real)2 * r), ((real)2 + r) / (real)3));
+        return nmax((T)0, nmin(nmin((T)2, (T)2 * r), ((T)2 + r) / (T)3));
     }
+}
+
+// Largest |minus| for which the reference's float64 phi(minus / 1e-14) is
+// finite, so that phi * plus is a zero when plus == 0. Albada squares r
+// (pos * pos overflows from |r| ~ 1.34e154, i.e. |minus| ~ 1.34e140, and
+// inf / inf is NaN); VanLeer doubles it (overflow from |minus| ~ 8.9e293);
+// MINBEE, Superbee and Unos clamp it and stay finite for every non-NaN r.
+// Below these bounds (with a margin) the zero is exact; at and above them the
+// literal expression runs and yields the reference's NaN.
+template <int LIM>
+__device__ __forceinline__ constexpr double zero_plus_bound() {
+    return LIM == ALBADA ? 1e140 : 1e280;
 }
 
 // Reciprocal and division. The FP32 form uses the hardware reciprocal and
@@ -120,16 +134,18 @@ __device__ __forceinline__ real rsqrt_(real x) { return sqrt(x); }
 template <int LIM, int DIM>
 __device__ __forceinline__ real limited(real minus, real plus) {
     // plus == +-0 (uniform neighbours): phi(minus / 1e-14) is finite and >= 0
-    // for every limiter when |minus| is far from overflow, so the product is
-    // a zero of plus's sign, up to the sign of a zero slope (which adding it
-    // to the cell value erases).
-    if (plus == (real)0 && fabs(minus) < (real)1e280) return plus;
+    // below the limiter's bound, so the product is a zero of plus's sign, up
+    // to the sign of a zero slope (which adding it to the cell value erases).
+    if (plus == (real)0 && fabs((double)minus) < zero_plus_bound<LIM>()) return plus;
     // Only when some lane of the wave needs it (the test is wave-uniform, so
-    // the rare literal path is a real branch, not speculated code).
+    // the rare literal path is a real branch, not speculated code). The
+    // quotient and phi are evaluated in double in both builds: the FP32 form
+    // then overflows to the reference's NaN (and takes the first-order
+    // fallback) for exactly the operands where the float64 reference does.
     if (__any(fabs(plus) < (real)1e-14)) {
         if (fabs(plus) < (real)1e-14) {
-            real r = minus / (real)1e-14;
-            return (DIM == 3 ? phi3<LIM>(r) : phi2<LIM>(r)) * plus;
+            const double r = (double)minus / 1e-14;
+            return (real)((DIM == 3 ? phi3<LIM>(r) : phi2<LIM>(r)) * (double)plus);
         }
     }
     if (!(minus * plus > (real)0)) {

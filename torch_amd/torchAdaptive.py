@@ -224,6 +224,7 @@ class AdaptiveSimulation(Simulation):
         self._leaves.upload(state, lam, beta)
         self.mesh = self._mesh_on_device = mesh
         self._host = (state, lam, beta)
+        self._remember()
 
     def _sync(self):
         """Reinstall after the mesh or the solver options were replaced on
@@ -231,6 +232,8 @@ class AdaptiveSimulation(Simulation):
         if self.mesh is not self._mesh_on_device or self._solver_key() != self._leaves_key:
             state, lam, beta = self._fetch()
             self._install(self.mesh, state, lam, beta)
+        else:
+            self._push_edits()
 
     def _upload(self, state, lam, beta):
         self._install(self.mesh, state, lam, beta)
@@ -238,7 +241,11 @@ class AdaptiveSimulation(Simulation):
     def _fetch(self):
         if self._host is None:
             self._host = self._leaves.download()
+            self._remember()
         return self._host
+
+    def _upload_edits(self, state, lam, beta):
+        self._leaves.upload(state, lam, beta)
 
     def _fraction(self, values, name):
         q = np.asarray(values, dtype=float)

@@ -29,6 +29,28 @@ from .presets import preset as _preset
 from .torchExecution import DeviceExecution, validate_workers
 from .torchSolver3D import LIMITERS, config as _config
 
+try:
+    import xxhash
+
+    def _hasher():
+        return xxhash.xxh3_128()
+except ImportError:  # pragma: no cover - the image ships xxhash
+    import hashlib
+
+    def _hasher():
+        return hashlib.blake2b(digest_size=16)
+
+
+def _digest(arrays):
+    """Content digest of the host copies (state, lam, beta; None skipped)."""
+    h = _hasher()
+    for a in arrays:
+        if a is not None:
+            h.update(memoryview(np.ascontiguousarray(a)).cast("B"))
+        h.update(b"|")
+    return h.digest()
+
+
 _OPTIONS = {
     "controller": _abi.CONTROLLERS,
     "variant": _abi.VARIANTS,
@@ -163,13 +185,30 @@ class Simulation:
             self._device.set_metric(c.get("geometry", "cartesian"), _geometry.factor_table(c), c["face_boundaries"])
         self._device.upload(state, lam, beta)
         self._host = (state, lam, beta)
+        self._remember()
 
     def _fetch(self):
         if self._host is None:
             c = self.config
             znd = c["reactive"] and c["kinetic_model"] == "znd"
             self._host = self._device.download(c["reactive"], znd)
+            self._remember()
         return self._host
+
+    # The reference's state, lam and beta are plain arrays that callers edit
+    # in place (testAdaptive.py:127-128, 288). Here they are host copies of
+    # the device state, so every device operation first uploads copies that
+    # changed since they were last uploaded or downloaded.
+    def _remember(self):
+        self._host_digest = None if self._host is None else _digest(self._host)
+
+    def _push_edits(self):
+        if self._host is not None and _digest(self._host) != getattr(self, "_host_digest", None):
+            self._upload_edits(*self._host)
+            self._remember()
+
+    def _upload_edits(self, state, lam, beta):
+        self._device.upload(state, lam, beta)
 
     # -- reference properties ------------------------------------------------
     @property
@@ -270,10 +309,12 @@ class Simulation:
     def dt_limit(self):
         """Simulation._dt on the committed state (torchSession.py:333-360), or
         cfl_dt for metric sessions (torchFiniteVolume.py:66-98)."""
+        self._push_edits()
         return self._device.metric_dt() if self.uses_metric_kernel else self._device.dt_limit()
 
     def device_totals(self):
         """Conserved totals reduced on the device (double)."""
+        self._push_edits()
         return self._device.totals()
 
     # -- stepping ------------------------------------------------------------
@@ -284,6 +325,7 @@ class Simulation:
         c = self.config
         if self.steps >= c["max_steps"]:
             raise RuntimeError("Maximum steps reached before requested final time")
+        self._push_edits()
         if self.uses_metric_kernel:
             info = self._device.metric_step(self.time, c["t_end"])
         else:
