@@ -19,8 +19,10 @@
 - Served 3D ignition requests drawn at 32^3..160^3 (Arrhenius and ZND):
   each result the same (bitwise) alone and in the mix in both precisions,
   and an Arrhenius and a ZND request from the low end of the range served
-  in FP64 equal to the oracle's reactive session (torchSession.py:374-522,
-  the dt-halving retry included) to 1e-12.
+  in FP64 against the oracle's reactive session (torchSession.py:374-522,
+  the dt-halving retry included) at every step: 1e-12, widened only where
+  the oracle itself moves by more under half-ulp changes of exp (ignition
+  amplifies rounding).
 """
 
 import numpy as np
@@ -210,35 +212,90 @@ def test_served_full_sizes_same_alone_and_in_mix(precision):
                 assert np.array_equal(a, b)
 
 
+class PerturbedExp:
+    """numpy for the oracle's kinetics module with every exp rounded off by up
+    to half an ulp (a seeded factor 1 + {-1, 0, 1} 2^-53): a plausible
+    difference between two correct exp implementations."""
+
+    def __init__(self, seed):
+        self._rng = np.random.default_rng(seed)
+
+    def __getattr__(self, name):
+        return getattr(np, name)
+
+    def exp(self, x):
+        y = np.exp(x)
+        return y * (1.0 + self._rng.integers(-1, 2, np.shape(y)) * 2.0 ** -53)
+
+
+def oracle_steps(prim, opts):
+    """The oracle's reactive session step by step: per-step (state, lam,
+    beta) and the number of steps whose dt was halved."""
+    ref = osession.Session(prim, [1.0] * 3, t_end=opts["t_end"], cfl=opts["cfl"], limiter=opts["limiter"],
+                           reactive=True, kinetic_model=opts["kinetic_model"], lambda_init=opts["lambda_init"],
+                           constants=opts["constants"])
+    out, halved = [], 0
+    while not ref.completed:
+        limit = min(ref.dt_limit(ref.state, ref.gamma), opts["t_end"] - ref.time)
+        t0 = ref.time
+        ref.step()
+        halved += int(ref.time - t0 < 0.75 * limit)
+        out.append((ref.state.copy(), ref.lam.copy(), None if ref.beta is None else ref.beta.copy()))
+    return out, halved
+
+
 @pytest.mark.parametrize("model", ["arrhenius", "znd"])
-def test_served_fp64_request_matches_the_oracle(model):
+def test_served_fp64_request_matches_the_oracle(model, monkeypatch):
     """Requests from the low end of the size range (rng 356, n in 32..40,
-    t_run 0.02); the Arrhenius one needs dt halvings on its first steps."""
+    t_run 0.02); the Arrhenius one needs dt halvings on its first steps.
+
+    Bar: 1e-12 per field at every step, except where the problem itself
+    amplifies rounding beyond it. Ignition is a thermal runaway (the rate
+    goes as exp(-Ea/RT) with Ea/RT ~ 25), so a half-ulp change of exp in
+    the oracle moves its own later states by up to ~1e-11. At each step the
+    device may differ from the oracle by max(1e-12, 4 x that spread): four,
+    because two correct exp implementations differ by up to ~2 ulp. The
+    served result equals the device session stepped alone, bitwise.
+    """
+    from oracle import solver1d
+    from parity import field_errors
+    from torch_amd import Simulation
     from torch_amd.torchServe import ignition_request
     from torch_amd.workloads import served_requests
 
     specs = served_requests(seed=356, count=8, t_run=0.02, znd_fraction=0.1, sizes=(32, 40))
     spec = next(s for s in specs if s["model"] == model)
     prim, opts = ignition_request(spec, 0.0).build()
-    ref = osession.Session(prim, [1.0] * 3, t_end=opts["t_end"], cfl=opts["cfl"], limiter=opts["limiter"],
-                           reactive=True, kinetic_model=opts["kinetic_model"], lambda_init=opts["lambda_init"],
-                           constants=opts["constants"])
-    halved = 0
-    while not ref.completed:
-        limit = min(ref.dt_limit(ref.state, ref.gamma), opts["t_end"] - ref.time)
-        t0 = ref.time
-        ref.step()
-        halved += int(ref.time - t0 < limit)
-    served = _serve([spec], [0.0], "fp64")[spec["index"]]
-    say(f"served request n={spec['n']} {model}: {served.steps} steps ({ref.steps} oracle), "
-        f"{served.retries} halvings ({halved} halved oracle steps)")
-    assert served.status == 0
-    assert served.steps == ref.steps
-    assert (served.retries > 0) == (halved > 0)
-    if model == "arrhenius":
-        assert halved > 0
-    state, lam, beta = served.result
-    assert_fields(state, ref.state, what="served state")
-    assert np.max(np.abs(lam - ref.lam)) <= 1e-12
-    if ref.beta is not None:
-        assert np.max(np.abs(beta - ref.beta)) <= 1e-12
+    ref, halved = oracle_steps(prim, opts)
+    monkeypatch.setattr(solver1d, "np", PerturbedExp(1))
+    spread, _ = oracle_steps(prim, opts)
+    monkeypatch.undo()
+    assert len(spread) == len(ref)
+    sim = Simulation(prim, [1.0] * 3, t_end=opts["t_end"], reactive=True, kinetic_model=model,
+                     lambda_init=opts["lambda_init"], constants=opts["constants"], cfl=opts["cfl"],
+                     limiter=opts["limiter"], precision="fp64")
+    try:
+        strict, retries = 0, 0
+        for k, ((state, lam, beta), (p_state, p_lam, _)) in enumerate(zip(ref, spread)):
+            info = sim.step()
+            retries += info.retries
+            err = max(field_errors(sim.state, state).max(), np.max(np.abs(sim.lam - lam)))
+            if beta is not None:
+                err = max(err, np.max(np.abs(sim.beta - beta)))
+            cond = max(field_errors(p_state, state).max(), np.max(np.abs(p_lam - lam)))
+            bar = max(1e-12, 4.0 * cond)
+            strict += int(bar == 1e-12)
+            say(f"  step {k}: device vs oracle {err:.2e}, oracle under half-ulp exp changes {cond:.2e}")
+            assert err <= bar, f"step {k}: {err:.3e} > {bar:.3e}"
+        assert sim.completed and sim.steps == len(ref)
+        assert (retries > 0) == (halved > 0)
+        if model == "arrhenius":
+            assert halved > 0
+        served = _serve([spec], [0.0], "fp64")[spec["index"]]
+        say(f"served request n={spec['n']} {model}: {served.steps} steps, {served.retries} halvings "
+            f"({halved} halved oracle steps), 1e-12 held at {strict} of {len(ref)} steps")
+        assert served.status == 0 and served.steps == sim.steps and served.retries == retries
+        assert np.array_equal(served.result[0], sim.state)
+        assert np.array_equal(served.result[1], sim.lam)
+    finally:
+        sim.close()
