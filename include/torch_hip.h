@@ -138,6 +138,15 @@ int torch_state_profile(torch_state* state, int enable, double* kernel_ms, long*
 int torch_hllc_faces(torch_ctx* ctx, int precision, int nv, int direction, long m, const double* left,
                      const double* right, const double* gl, const double* gr, double* out);
 
+/* torch_hllc_faces without any device allocation: no counterpart in the
+ * reference. For callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). Same arguments and
+ * results as torch_hllc_faces, for at most 65536 faces per call (TORCH_INVALID
+ * beyond); the staging lives in a fixed 10.25 MiB device array per GPU that
+ * the library holds until the process ends. Calls on one GPU are serialised. */
+int torch_hllc_faces_static(torch_ctx* ctx, int precision, int nv, int direction, long m, const double* left,
+                            const double* right, const double* gl, const double* gr, double* out);
+
 /* ---------------------------------------------------------------------------
  * Metric/face scheme: the reference's second finite-volume family, used by
  * sessions with a curvilinear geometry or explicit face boundaries

@@ -28,7 +28,7 @@ EXPORTS = (
     "torch_ctx_create", "torch_ctx_destroy", "torch_last_error", "torch_state_create",
     "torch_state_destroy", "torch_state_upload", "torch_state_download", "torch_state_dt",
     "torch_state_step", "torch_state_hydro", "torch_state_fluxes", "torch_state_react",
-    "torch_state_totals", "torch_state_profile", "torch_hllc_faces",
+    "torch_state_totals", "torch_state_profile", "torch_hllc_faces", "torch_hllc_faces_static",
     "torch_batch_footprint", "torch_batch_create", "torch_batch_destroy", "torch_batch_add",
     "torch_batch_remove", "torch_batch_seed", "torch_batch_run", "torch_batch_info", "torch_batch_download",
     "torch_batch_tune", "torch_batch_set_tiles", "torch_batch_profile", "torch_batch_time_pass",
@@ -143,6 +143,8 @@ def load():
                 "torch_state_profile": (ctypes.c_int, [vp, ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_long)]),
                 "torch_hllc_faces": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                                     ctypes.c_long, _dp, _dp, _dp, _dp, _dp]),
+                "torch_hllc_faces_static": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                           ctypes.c_long, _dp, _dp, _dp, _dp, _dp]),
                 "torch_batch_footprint": (ctypes.c_int, [ctypes.POINTER(Config), ctypes.c_int,
                                                          ctypes.POINTER(ctypes.c_size_t)]),
                 "torch_batch_create": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_size_t, ctypes.POINTER(vp)]),
@@ -474,6 +476,19 @@ def hllc_faces(context, nv, direction, left, right, gl, gr, precision=0):
     out = np.empty_like(left)
     check(load().torch_hllc_faces(context.handle, int(precision), int(nv), int(direction), m, ptr(left),
                                   ptr(right), ptr(gl), ptr(gr), ptr(out)))
+    return out
+
+
+def hllc_faces_static(context, nv, direction, left, right, gl, gr, precision=0):
+    """hllc_faces through the allocation-free entry point (at most 65536 faces)."""
+    left = np.ascontiguousarray(left, dtype=np.float64)
+    right = np.ascontiguousarray(right, dtype=np.float64)
+    m = left.shape[1]
+    gl = np.ascontiguousarray(np.broadcast_to(gl, (m,)), dtype=np.float64)
+    gr = np.ascontiguousarray(np.broadcast_to(gr, (m,)), dtype=np.float64)
+    out = np.empty_like(left)
+    check(load().torch_hllc_faces_static(context.handle, int(precision), int(nv), int(direction), m, ptr(left),
+                                         ptr(right), ptr(gl), ptr(gr), ptr(out)))
     return out
 
 

@@ -571,6 +571,47 @@ const char* hydro_message(int dim) {
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Allocation-free HLLC face batches (torch_hllc_faces_static). Their staging
+// is one file-scope device array, `scratch`, of which every GPU holds its own
+// copy for the life of the process. In host code the array's name is only a
+// stand-in: its device address is looked up once per device at context
+// creation (hipGetSymbolAddress with that device current) and kept in
+// g_scratch; kernels and copies take that address, never the name.
+//
+// Layout for m faces (regions rounded up to mm, a multiple of 64 faces):
+//   FP64: left nv*mm | right nv*mm | gl mm | gr mm | out nv*mm        doubles
+//   FP32: the float64 inputs left | right | gl | gr ((2 nv + 2) mm doubles),
+//         then their float copies and the float result
+//         ((3 nv + 2) mm floats); the result is widened back into the start
+//         of the double region for the download.
+// At nv = 5 that is 136 (FP64) or 164 (FP32) bytes per face; 65536 faces,
+// eight of the reference's 8192-face worker batches (torchFlux.py:13), take
+// 10.25 MiB. Every region is written before it is read, so nothing needs
+// clearing between calls. Calls on one GPU are serialised by a per-device
+// mutex (each process has its own copy of the array).
+// ---------------------------------------------------------------------------
+constexpr long kStaticFaces = 65536;
+constexpr size_t kStaticBytes = (size_t)kStaticFaces * ((2 * 5 + 2) * sizeof(double) + (3 * 5 + 2) * sizeof(float));
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) double scratch[kStaticBytes / sizeof(double)];
+static void* g_scratch[kMaxDevices];
+static std::mutex g_scratch_mu[kMaxDevices];
+static std::mutex g_scratch_init_mu;
+
+// Called by torch_ctx_create with `device` current.
+static int scratch_init(int device) {
+    if (device >= kMaxDevices) return TORCH_OK;  // the static path then reports the device as unsupported
+    std::lock_guard<std::mutex> lock(g_scratch_init_mu);
+    if (g_scratch[device]) return TORCH_OK;
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(TORCH_HIP_ERROR, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch[device] = p;
+    return TORCH_OK;
+}
+
 extern "C" {
 
 int torch_last_error(char* buffer, size_t size) {
@@ -591,6 +632,11 @@ int torch_ctx_create(int device, torch_ctx** out) {
     if (e != hipSuccess) {
         delete c;
         return fail(TORCH_HIP_ERROR, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    if (int rc = scratch_init(device)) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return rc;
     }
     *out = c;
     return TORCH_OK;
@@ -1140,6 +1186,56 @@ static int faces_call(torch_ctx* ctx, int precision, int nv, int direction, long
     (void)hipFree(dgr);
     (void)hipFree(dout);
     return rc;
+}
+
+int torch_hllc_faces_static(torch_ctx* ctx, int precision, int nv, int direction, long m, const double* left,
+                            const double* right, const double* gl, const double* gr, double* out) {
+    if (!ctx || !left || !right || !gl || !gr || !out) return fail(TORCH_INVALID, "null argument");
+    if ((nv != 4 && nv != 5) || direction < 0 || direction >= nv - 2 || m < 0)
+        return fail(TORCH_INVALID, "HLLC requires matching four/five-component grids");
+    if (precision != TORCH_FP64 && precision != TORCH_FP32) return fail(TORCH_INVALID, "unknown precision");
+    if (m > kStaticFaces)
+        return fail(TORCH_INVALID, "torch_hllc_faces_static takes at most 65536 faces per call (its staging is a "
+                                   "fixed device array); split the batch or use torch_hllc_faces");
+    if (m == 0) return TORCH_OK;
+    if (ctx->device >= kMaxDevices || !g_scratch[ctx->device])
+        return fail(TORCH_HIP_ERROR, "no static face scratch on this device");
+    const Ops* ops = precision == TORCH_FP32 ? &kOps32 : &kOps64;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    std::lock_guard<std::mutex> slock(g_scratch_mu[ctx->device]);
+    HIP_TRY(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const long mm = (m + 63) / 64 * 64;
+    double* dl = (double*)g_scratch[ctx->device];
+    double* dr = dl + nv * mm;
+    double* dgl = dr + nv * mm;
+    double* dgr = dgl + mm;
+    HIP_TRY(hipMemcpyAsync(dl, left, nv * m * sizeof(double), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(dr, right, nv * m * sizeof(double), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(dgl, gl, m * sizeof(double), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(dgr, gr, m * sizeof(double), hipMemcpyHostToDevice, st));
+    const double* result;
+    if (precision == TORCH_FP64) {
+        double* dout = dgr + mm;
+        HIP_TRY(ops->hllc_faces(nv, direction, m, dl, dr, dgl, dgr, dout, st));
+        result = dout;
+    } else {
+        float* fl = (float*)(dgr + mm);
+        float* fr = fl + nv * mm;
+        float* fgl = fr + nv * mm;
+        float* fgr = fgl + mm;
+        float* fout = fgr + mm;
+        HIP_TRY(ops->convert_in(dl, fl, nv * m, st));
+        HIP_TRY(ops->convert_in(dr, fr, nv * m, st));
+        HIP_TRY(ops->convert_in(dgl, fgl, m, st));
+        HIP_TRY(ops->convert_in(dgr, fgr, m, st));
+        HIP_TRY(ops->hllc_faces(nv, direction, m, fl, fr, fgl, fgr, fout, st));
+        HIP_TRY(ops->convert_out(fout, dl, nv * m, st));  // the inputs are consumed by now (stream order)
+        result = dl;
+    }
+    HIP_TRY(hipMemcpyAsync(out, result, nv * m * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return TORCH_OK;
 }
 
 int torch_hllc_faces(torch_ctx* ctx, int precision, int nv, int direction, long m, const double* left,
