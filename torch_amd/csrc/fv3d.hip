@@ -94,9 +94,9 @@ __device__ void block_partials(double (&acc)[5], double (&mx)[3], double src, do
 // is inlined.
 __device__ __forceinline__ void speeds3(const real* o, real g, real (&v)[3]) {
 #pragma clang fp contract(off)
-    const real ir = rcp(o[0]);
+    const real ir = crcp(o[0]);
     const real p = (g - (real)1) * (o[4] - ((real)0.5 * (o[1] * o[1] + o[2] * o[2] + o[3] * o[3])) * ir);
-    const real a = rsqrt_((g * p) * ir);
+    const real a = csqrt((g * p) * ir);
 #pragma unroll
     for (int m = 0; m < 3; ++m) v[m] = fabs(o[m + 1] * ir) + a;
 }

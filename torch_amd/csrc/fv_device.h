@@ -106,19 +106,51 @@ __device__ __forceinline__ constexpr double zero_plus_bound() {
     return LIM == ALBADA ? 1e140 : 1e280;
 }
 
-// Reciprocal and division. The FP32 form uses the hardware reciprocal and
-// square root (one instruction each, about 1 ulp). The FP64 plain method
-// keeps IEEE division where the reference divides once, and reuses one
-// reciprocal where the reference divides several values by the same
-// density (a <= 2 ulp change per product, far inside the 1e-12 bar).
+// Reciprocal, division and square root. The FP32 form uses the hardware
+// reciprocal and square root (one instruction each, about 1 ulp). The FP64
+// plain method refines the hardware reciprocal and reciprocal square root
+// with Newton-Raphson steps in fused multiply-adds instead of the IEEE
+// division and square-root sequences (about twice the instructions, with
+// scaling and special-case fix-ups): 1/x by r' = r + r (1 - x r), twice; a/b
+// as q = a r plus one residual correction q' = q + r (a - b q); sqrt(x) by
+// the coupled iteration on g ~ sqrt(x), h ~ 1/(2 sqrt(x)) from y = rsq(x)
+// (g = x y, h = y/2; r = 1/2 - g h; g' = g + g r, h' = h + h r) and a final
+// residual step g'' = g' + h' (x - g'^2). Each lands within a few ulp of the
+// correctly rounded result, far inside the 1e-12 bar. They are used where
+// the operands are positive and normal by construction (floored densities
+// and pressures, guarded denominators) or where any non-finite result
+// already decides the step (a non-positive predicted density makes the
+// predictor non-finite either way). crcp / csqrt stay correctly rounded in
+// the FP64 build: the CFL speeds, whose dt must round like the reference's.
 #ifdef FV_FAST
 __device__ __forceinline__ real rcp(real x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ real dv(real a, real b) { return a * __builtin_amdgcn_rcpf(b); }
 __device__ __forceinline__ real rsqrt_(real x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ real crcp(real x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ real csqrt(real x) { return __builtin_amdgcn_sqrtf(x); }
 #else
-__device__ __forceinline__ real rcp(real x) { return (real)1 / x; }
-__device__ __forceinline__ real dv(real a, real b) { return a / b; }
-__device__ __forceinline__ real rsqrt_(real x) { return sqrt(x); }
+__device__ __forceinline__ real rcp(real x) {
+    real r = __builtin_amdgcn_rcp(x);
+    real e = fma(-x, r, (real)1);
+    r = fma(e, r, r);
+    e = fma(-x, r, (real)1);
+    return fma(e, r, r);
+}
+__device__ __forceinline__ real dv(real a, real b) {
+    const real r = rcp(b);
+    const real q = a * r;
+    return fma(fma(-b, q, a), r, q);
+}
+__device__ __forceinline__ real rsqrt_(real x) {
+    const real y = __builtin_amdgcn_rsq(x);
+    real g = x * y, h = (real)0.5 * y;
+    const real r = fma(-g, h, (real)0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    return fma(fma(-g, g, x), h, g);
+}
+__device__ __forceinline__ real crcp(real x) { return (real)1 / x; }
+__device__ __forceinline__ real csqrt(real x) { return sqrt(x); }
 #endif
 
 // Limited slope phi(minus/plus) * plus (torchSolver3D.py:86-111,
