@@ -153,7 +153,7 @@ using fvl::coef_of;
 // block's slot in s.partials.
 template <int LIM, int BX, int BY, bool SPEC>
 __device__ __forceinline__ void step3d_tile(const Step& s, const G3& g, int tbx, int tby, int tbz, long pblk,
-                                            real* sbuf, real* sybuf, real* sfbuf) {
+                                            real* sbuf, real* sybuf, real* sfbuf, double* sacc) {
     using T = Tile3<BX, BY, SPEC>;
     constexpr int TX = BX - 2, TY = BY - 2;
     constexpr int NQ = T::NQ;
@@ -206,7 +206,12 @@ __device__ __forceinline__ void step3d_tile(const Step& s, const G3& g, int tbx,
     // plane's first barrier, from the y fluxes left in sF (one barrier fewer
     // per plane).
     bool pend = false;
-    double acc[5] = {0, 0, 0, 0, 0}, mx[3];
+    // Outward boundary sums, per thread in plane order: kept in the thread's
+    // own LDS slot (only boundary faces touch it) instead of ten registers.
+    constexpr int NT = BX * BY;
+#pragma unroll
+    for (int c = 0; c < 5; ++c) sacc[c * NT + lane + BX * row] = 0.0;
+    double mx[3];
     real mxr[3] = {0, 0, 0};
     int bad = 0;
 
@@ -223,6 +228,9 @@ __device__ __forceinline__ void step3d_tile(const Step& s, const G3& g, int tbx,
     }
 
     for (int k = z0 - 1; k <= z1; ++k) {
+        // Every load of the plane is issued before the first wait, so the
+        // column's next plane and the ring cell arrive in one memory
+        // latency, not two in a row.
         load5(U, N, g, x, y, k + 1, An);
         if (lamp) gn = gamma_from(load1(lamp, g, x, y, k + 1), direct);
         for (int j = 0; j < NQ; ++j) if (j < nq) qn[j] = load1((const real*)s.q[j], g, x, y, k + 1);
@@ -379,7 +387,7 @@ __device__ __forceinline__ void step3d_tile(const Step& s, const G3& g, int tbx,
             if (x == 0 || x == nx) {
                 double sgn = x == 0 ? -w0 : w0;
 #pragma unroll
-                for (int c = 0; c < 5; ++c) acc[c] += sgn * (double)Fx[c];
+                for (int c = 0; c < 5; ++c) sacc[c * NT + tid] += sgn * (double)Fx[c];
             }
             if (s.F[0]) {
                 long f = ((long)k * ny + y) * (nx + 1) + x, M = (long)nz * ny * (nx + 1);
@@ -391,7 +399,7 @@ __device__ __forceinline__ void step3d_tile(const Step& s, const G3& g, int tbx,
             if (y == 0 || y == ny) {
                 double sgn = y == 0 ? -w1 : w1;
 #pragma unroll
-                for (int c = 0; c < 5; ++c) acc[c] += sgn * (double)Fy[c];
+                for (int c = 0; c < 5; ++c) sacc[c * NT + tid] += sgn * (double)Fy[c];
             }
             if (s.F[1]) {
                 long f = ((long)k * (ny + 1) + y) * nx + x, M = (long)nz * (ny + 1) * nx;
@@ -403,7 +411,7 @@ __device__ __forceinline__ void step3d_tile(const Step& s, const G3& g, int tbx,
             if (k == 0 || k == nz) {
                 double sgn = k == 0 ? -w2 : w2;
 #pragma unroll
-                for (int c = 0; c < 5; ++c) acc[c] += sgn * (double)Fz[c];
+                for (int c = 0; c < 5; ++c) sacc[c * NT + tid] += sgn * (double)Fz[c];
             }
             if (s.F[2]) {
                 long f = ((long)k * ny + y) * nx + x, M = (long)(nz + 1) * ny * nx;
@@ -428,6 +436,9 @@ __device__ __forceinline__ void step3d_tile(const Step& s, const G3& g, int tbx,
     }
 #pragma unroll
     for (int m = 0; m < 3; ++m) mx[m] = (double)mxr[m];
+    double acc[5];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) acc[c] = sacc[c * NT + tid];
     block_partials<BX * BY>(acc, mx, 0.0, s.partials, s.flags, bad, pblk);
 }
 
@@ -437,8 +448,9 @@ __global__ void __launch_bounds__(BX* BY, occupancy_of(BX* BY, SPEC)) k_step3d(S
     __shared__ real sbuf[T::BUF];
     __shared__ real sy[T::YS];
     __shared__ real sf[T::FLUX];
+    __shared__ double sacc[5 * BX * BY];
     const long b = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
-    step3d_tile<LIM, BX, BY, SPEC>(s, g, blockIdx.x, blockIdx.y, blockIdx.z, b, sbuf, sy, sf);
+    step3d_tile<LIM, BX, BY, SPEC>(s, g, blockIdx.x, blockIdx.y, blockIdx.z, b, sbuf, sy, sf, sacc);
 }
 
 // Work-list form: blocks of many instances of different sizes and
@@ -452,6 +464,7 @@ __global__ void __launch_bounds__(BX* BY, occupancy_of(BX* BY, SPEC)) k_batch3d(
     __shared__ real sbuf[T::BUF];
     __shared__ real sy[T::YS];
     __shared__ real sf[T::FLUX];
+    __shared__ double sacc[5 * BX * BY];
     const int b = blockIdx.x;
     const int lo = fvl::owner(first, n, b);
     const fvl::BatchItem& it = fvl::item_at(items, lo);
@@ -465,7 +478,7 @@ __global__ void __launch_bounds__(BX* BY, occupancy_of(BX* BY, SPEC)) k_batch3d(
         block_partials<BX * BY>(acc, mx, 0.0, s.partials, s.flags, 0, local);
         return;
     }
-    step3d_tile<LIM, BX, BY, SPEC>(s, g, tbx, tby, tbz, local, sbuf, sy, sf);
+    step3d_tile<LIM, BX, BY, SPEC>(s, g, tbx, tby, tbz, local, sbuf, sy, sf, sacc);
 }
 
 // First-order LLF pass over the whole grid, run only when the second-order
