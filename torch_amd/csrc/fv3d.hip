@@ -228,9 +228,6 @@ __device__ __forceinline__ void step3d_tile(const Step& s, const G3& g, int tbx,
     }
 
     for (int k = z0 - 1; k <= z1; ++k) {
-        // Every load of the plane is issued before the first wait, so the
-        // column's next plane and the ring cell arrive in one memory
-        // latency, not two in a row.
         load5(U, N, g, x, y, k + 1, An);
         if (lamp) gn = gamma_from(load1(lamp, g, x, y, k + 1), direct);
         for (int j = 0; j < NQ; ++j) if (j < nq) qn[j] = load1((const real*)s.q[j], g, x, y, k + 1);
