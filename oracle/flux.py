@@ -100,11 +100,16 @@ def normal_order(nv, direction):
     return [0, normal] + [m for m in range(1, nv - 1) if m != normal] + [nv - 1]
 
 
-def hllc_grid(left, right, gamma_l, gamma_r=None, direction=0):
+def hllc_grid(left, right, gamma_l, gamma_r=None, direction=0, batch_size=None, execution=None):
     """HLLC over Cartesian-ordered face grids (torchFlux.py:6-77).
 
-    The result does not depend on batching (the reference evaluates 8192-face
-    batches; every face is independent), so the whole grid is one batch here.
+    Every face is independent, so the result does not depend on batching
+    (the reference's own test, testPerformance.py:23-52). By default the
+    whole grid is one batch. With ``batch_size`` the faces go through the
+    reference's worker path (torchFlux.py:44-76): flattened in C order, cut
+    into batches of that many faces, each gathered, rotated to the normal
+    basis and solved by one task of ``execution.map`` (oracle.execution),
+    and scattered back in order.
     """
     nv = left.shape[0]
     if left.shape != right.shape or nv not in (4, 5):
@@ -115,8 +120,23 @@ def hllc_grid(left, right, gamma_l, gamma_r=None, direction=0):
     gl = np.broadcast_to(gamma_l, left.shape[1:]).reshape(-1)
     gr = gl if gamma_r is None else np.broadcast_to(
         gamma_r, left.shape[1:]).reshape(-1)
-    flux = hllc(left[order].reshape(nv, -1), right[order].reshape(nv, -1), gl,
-                gr)
-    out = np.empty_like(flux)
-    out[order] = flux
+    if batch_size is None:
+        flux = hllc(left[order].reshape(nv, -1), right[order].reshape(nv, -1), gl,
+                    gr)
+        out = np.empty_like(flux)
+        out[order] = flux
+        return out.reshape(left.shape)
+    rows_l = [left[m].reshape(-1) for m in order]
+    rows_r = [right[m].reshape(-1) for m in order]
+    out = np.empty((nv, gl.size))
+
+    def solve(bounds):
+        a, b = bounds
+        ul = np.array([row[a:b] for row in rows_l])
+        ur = np.array([row[a:b] for row in rows_r])
+        return a, b, hllc(ul, ur, gl[a:b], gr[a:b])
+
+    spans = ((a, min(a + batch_size, gl.size)) for a in range(0, gl.size, batch_size))
+    for a, b, value in (map(solve, spans) if execution is None else execution.map(solve, spans)):
+        out[order, a:b] = value
     return out.reshape(left.shape)

@@ -15,6 +15,7 @@ import numpy as np
 
 from . import metric
 from . import solver1d as one
+from .execution import Execution
 from . import solver2d as two
 from . import solver3d as three
 
@@ -25,7 +26,7 @@ class Session:
                  lambda_init=None, beta_init=None, inlet_primitive=None,
                  controller="Sound Speed Controlled", variant="Variante 1 - Original",
                  wavespeed="Pressure Based", max_steps=100000, checked=True, constants=None,
-                 geometry="cartesian", origins=None, face_boundaries=None):
+                 geometry="cartesian", origins=None, face_boundaries=None, workers=None):
         primitive = np.array(primitive, dtype=float, copy=True)
         dim = primitive.ndim - 1
         if dim not in (1, 2, 3) or primitive.shape[0] != dim + 2:
@@ -71,6 +72,10 @@ class Session:
         self.boundary_integral = np.zeros(dim + 2)
         self.source_integral = np.zeros(dim + 2)
         self.fallbacks = 0
+        # workers=None: the whole face grid as one batch; an int: the
+        # reference's 8192-face batches through its worker pool
+        # (torchSession.py:160-183, torchFlux.py:13, 66-76).
+        self.execution = None if workers is None else Execution(workers)
 
     @staticmethod
     def _fraction(values, shape):
@@ -135,7 +140,9 @@ class Session:
             state, fx, fy, fb = two.hydro(old, h[0], h[1], dt, gamma, c["limiter"], bc[0], bc[1],
                                           c["inlet_primitive"])
             return state, [fx, fy], fb
-        state, fluxes, fb = three.hydro(old, gamma, tuple(h), dt, c["limiter"], tuple(bc[::-1]))
+        batch = None if self.execution is None else 8192
+        state, fluxes, fb = three.hydro(old, gamma, tuple(h), dt, c["limiter"], tuple(bc[::-1]),
+                                        batch_size=batch, execution=self.execution)
         return state, fluxes, fb
 
     def advect(self, q, old, state, fluxes, dt):

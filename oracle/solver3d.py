@@ -173,7 +173,7 @@ def predictor(padded, gp, spacing, dt, limiter):
     return slopes, predicted, ok
 
 
-def hydro(state, gamma, spacing, dt, limiter, boundaries, first_order=False):
+def hydro(state, gamma, spacing, dt, limiter, boundaries, first_order=False, batch_size=None, execution=None):
     """One unsplit step with the global first-order fallback (torchSolver3D.py:122-203).
 
     Returns (U', [Fx, Fy, Fz], used_first_order). Raises RuntimeError when
@@ -184,14 +184,15 @@ def hydro(state, gamma, spacing, dt, limiter, boundaries, first_order=False):
     if not first_order:
         slopes, predicted, ok = predictor(padded, gp, spacing, dt, limiter)
         if not ok:
-            return hydro(state, gamma, spacing, dt, limiter, boundaries, True)
+            return hydro(state, gamma, spacing, dt, limiter, boundaries, True, batch_size, execution)
         ga = gp[1:-1, 1:-1, 1:-1]
         fluxes = []
         for m, axis in enumerate((3, 2, 1), 1):
             pl, pr = faces(predicted, axis, True)
             sl, sr = faces(slopes[m - 1], axis, True)
             gl, gr = faces(ga[None], axis, True)
-            fluxes.append(hllc_grid(pl + 0.5 * sl, pr - 0.5 * sr, gl[0], gr[0], direction=m - 1))
+            fluxes.append(hllc_grid(pl + 0.5 * sl, pr - 0.5 * sr, gl[0], gr[0], direction=m - 1,
+                                    batch_size=batch_size, execution=execution))
     else:
         fluxes = first_order_fluxes(padded, gp)
     result = state.copy()
@@ -199,7 +200,7 @@ def hydro(state, gamma, spacing, dt, limiter, boundaries, first_order=False):
         result -= dt / h * np.diff(flux, axis=axis)
     if not one.physical_state(result):
         if not first_order:
-            return hydro(state, gamma, spacing, dt, limiter, boundaries, True)
+            return hydro(state, gamma, spacing, dt, limiter, boundaries, True, batch_size, execution)
         raise RuntimeError("3D hydrodynamic update produced a nonphysical state")
     return result, fluxes, first_order
 

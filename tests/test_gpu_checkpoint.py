@@ -132,3 +132,73 @@ def test_checkpoint_resumes_inside_a_batch(tmp_path, precision):
         batch.commit(ident, resumed)
         batch.close()
         assert_same_run(resumed, continuous)
+
+
+# Metadata keys of the reference's writer (torchIO.py:66-119); the device
+# writer adds one record, "device", which the reference's reader ignores.
+REFERENCE_KEYS = {"format", "schema_version", "solver_version", "geometry", "units", "layout", "config", "run_id",
+                  "time", "steps", "initial_integral", "boundary_integral", "source_integral", "current_integral"}
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_device_checkpoint_resumed_by_the_oracle(tmp_path, dim):
+    """A checkpoint the device wrote, read with plain NumPy (no device code),
+    resumed by the oracle's session: it ends within 1e-12 of the device's
+    uninterrupted run (testSession.py:80-121 across the two implementations),
+    and its layout is the reference writer's (torchIO.py:66-119, 192-214)."""
+    import json
+
+    from oracle import session as osession
+    from parity import assert_fields
+
+    for model in MODELS:
+        a = example(dim, model)
+        continuous = copy.deepcopy(a)
+        continuous.run()
+        path = tmp_path / f"device_{model}.npz"
+        a.run(step_limit=2, checkpoint=path)
+        a.close()
+        with np.load(path, allow_pickle=False) as f:
+            data = {k: f[k] for k in f.files}
+        meta = json.loads(data["metadata"].tobytes().decode("utf-8"))
+        assert REFERENCE_KEYS <= set(meta) and set(meta) - REFERENCE_KEYS <= {"device"}
+        assert meta["format"] == "TORCH checkpoint" and meta["layout"] == "x-fastest"
+        c = meta["config"]
+        axes = "xyz"[:dim]
+        names = ["rho"] + ["momentum_" + ax for ax in axes] + ["energy"]
+        want = {"metadata"} | {"state/" + n for n in names}
+        if c["reactive"]:
+            want |= {"state/lambda"} | ({"state/beta"} if c["kinetic_model"] == "znd" else set())
+        assert set(data) == want
+        shape = tuple(c["counts_xyz"][::-1])
+        state = np.stack([data["state/" + n].reshape(shape) for n in names])
+        assert all(data[k].dtype == np.dtype("<f8") for k in data if k != "metadata")
+        ref = osession.Session(
+            np.ones((dim + 2,) + shape), c["lengths_xyz"], t_end=c["t_end"], cfl=c["cfl"], gamma=c["gamma"],
+            boundaries=c["boundaries_xyz"], limiter=c["limiter"], reactive=c["reactive"],
+            kinetic_model=c["kinetic_model"], max_steps=c["max_steps"])
+        ref.state = state
+        if c["reactive"]:
+            ref.lam = data["state/lambda"].reshape(shape)
+            ref.beta = data["state/beta"].reshape(shape) if c["kinetic_model"] == "znd" else None
+        ref.time, ref.steps = meta["time"], meta["steps"]
+        ref.initial_integral = np.array(meta["initial_integral"])
+        ref.boundary_integral = np.array(meta["boundary_integral"])
+        ref.source_integral = np.array(meta["source_integral"])
+        assert ref.steps == 2
+        np.testing.assert_allclose(ref.integral, meta["current_integral"], rtol=1e-15, atol=1e-300)
+        while not ref.completed:
+            ref.step()
+        assert ref.steps == continuous.steps and ref.time == pytest.approx(continuous.time, rel=1e-14)
+        assert_fields(continuous.state, ref.state, what=f"{dim}D {model} resumed by the oracle")
+        for name in ("lam", "beta"):
+            x, y = getattr(continuous, name), getattr(ref, name)
+            assert (x is None) == (y is None)
+            if x is not None:
+                assert np.max(np.abs(x - y)) <= 1e-12
+        # Periodic boundaries: the outflow sums are rounding residues of
+        # cancelling face sums, so they are held to the totals' scale.
+        scale = np.max(np.abs(continuous.initial_integral))
+        np.testing.assert_allclose(continuous.boundary_integral, ref.boundary_integral, rtol=1e-11, atol=1e-12 * scale)
+        np.testing.assert_allclose(continuous.source_integral, ref.source_integral, rtol=1e-11, atol=1e-12 * scale)
+        continuous.close()
