@@ -210,6 +210,37 @@ def run_served(seed, count, t_run, budget_gib, round_steps, precision="fp32", op
     return out
 
 
+def run_reactive(seed, count, t_run, steps, warmup, precision="fp32"):
+    """The served ignition requests admitted together into one batch of fixed
+    membership (no arrivals or retirements), stepped by graph replays: the
+    reactive kernels' throughput without admission work."""
+    from torch_amd.torchServe import Server, ignition_request
+    from torch_amd.workloads import served_requests
+
+    specs = served_requests(seed=seed, count=count, t_run=t_run)
+    server = Server(precision=precision, hbm_budget=0, round_steps=steps)
+    ids = []
+    for spec in specs:
+        req = ignition_request(spec, 0.0)
+        ident = server._admit(req, server.payload(*req.build()))
+        if ident is None:
+            raise RuntimeError("reactive batch instance not admitted")
+        ids.append(ident)
+    batch = server.batch
+    batch.run(warmup)
+    before = [batch.info(i) for i in ids]
+    t0 = time.perf_counter()
+    batch.run(steps)
+    wall = time.perf_counter() - t0
+    after = [batch.info(i) for i in ids]
+    server.close()
+    work = sum(b["cells"] * (b["steps"] - a["steps"]) for a, b in zip(before, after))
+    retries = sum(b["retries"] - a["retries"] for a, b in zip(before, after))
+    done = sum(b["steps"] - a["steps"] for a, b in zip(before, after))
+    return dict(value=round(work / wall, 1), ms_per_replay=round(1e3 * wall / steps, 4), replays=steps,
+                steps=done, retries=retries, cells=sum(b["cells"] for b in after), dtype=precision.replace("fp", "f"))
+
+
 # HBM bytes per launch of the 3D step kernel on the 256^3 Sod tube with the
 # default tiles (FP32 32x8, FP64 32x16, z chunks of 24, and of 16), from rocprofv3
 # FETCH_SIZE and WRITE_SIZE counter runs scaled by a 1 GiB device copy
@@ -576,8 +607,8 @@ def copy_calibration(mib=1024, reps=3):
     print(json.dumps({"copy_mib": mib, "reps": reps, "GBps_rw": round(2 * n * reps / wall / 1e9, 1)}))
 
 
-LEGS = ("fp64", "fp32", "batched", "error", "uneven", "served", "metric", "adaptive", "legacy", "checkpoint",
-        "cpu")
+LEGS = ("fp64", "fp32", "batched", "error", "uneven", "served", "reactive", "metric", "adaptive", "legacy",
+        "checkpoint", "cpu")
 
 
 def _pair(text):
@@ -620,6 +651,7 @@ def main(argv=None):
     ap.add_argument("--serve-budget-gib", type=float, default=4.0, help="HBM budget of the served batch")
     ap.add_argument("--serve-round", type=int, default=4, help="steps between admission checks")
     ap.add_argument("--serve-closed-only", action="store_true", help="skip the open-loop (Poisson) run")
+    ap.add_argument("--reactive-steps", type=int, default=20, help="graph replays of the fixed reactive batch")
     ap.add_argument("--metric-n", type=int, default=96, help="cells per axis of the metric/face workload")
     ap.add_argument("--metric-steps", type=int, default=20)
     ap.add_argument("--amr-roots", type=_pair, default=(512, 64), help="NX,NY roots of the adaptive workload")
@@ -685,6 +717,15 @@ def main(argv=None):
             f"Arrhenius (ZND for ~10%), Ea/RT0 in [20,30], Q/RT0 in [10,50], Gaussian hot spot T/T0 in "
             f"[1.5,3], run length {args.serve_t_run} (induction ~0.1 of it), VanLeer, CFL 0.4, open "
             f"boundaries; FP32 batch, HBM budget {args.serve_budget_gib} GiB, {args.serve_round} steps/round")
+    if "reactive" in legs:
+        served = served or {}
+        served["reactive_batch"] = run_reactive(args.serve_seed, args.serve_count, args.serve_t_run,
+                                                args.reactive_steps, 3)
+        served["reactive_batch"]["workload"] = (
+            f"the {args.serve_count} served ignition requests admitted together into one FP32 batch (fixed "
+            f"membership), {args.reactive_steps} graph replays after 3")
+        say(f"# reactive batch: {served['reactive_batch']['ms_per_replay']} ms/replay, "
+            f"{served['reactive_batch']['value']:.4g} cell-updates/s, {served['reactive_batch']['retries']} halvings")
     metric = None
     if "metric" in legs:
         mn = args.metric_n

@@ -51,8 +51,11 @@ __device__ __forceinline__ real load1(const real* __restrict__ q, const G3& g, i
 }
 
 // Fold one block's partial results: maxima and sums over the block's
-// threads, written to partials[block * NSLOT + slot] by thread 0.
-template <int NT>
+// threads, written to partials[block * NSLOT + slot] by thread 0. Passes
+// without outward sums (SUMS false: chemistry, CFL passes) or without a
+// source sum (SRC false) skip those wave reductions and store +0, which is
+// what reducing their all-zero inputs gives.
+template <int NT, bool SUMS = true, bool SRC = true>
 __device__ void block_partials(double (&acc)[5], double (&mx)[3], double src, double* partials, int* flags,
                                int bad, long block = -1) {
     __shared__ double red[NT / 64 > 0 ? NT / 64 : 1][9];
@@ -65,8 +68,8 @@ __device__ void block_partials(double (&acc)[5], double (&mx)[3], double src, do
 #pragma unroll
     for (int i = 0; i < 3; ++i) v[i] = wave_max(mx[i]);
 #pragma unroll
-    for (int i = 0; i < 5; ++i) v[3 + i] = wave_sum(acc[i]);
-    v[8] = wave_sum(src);
+    for (int i = 0; i < 5; ++i) v[3 + i] = SUMS ? wave_sum(acc[i]) : 0.0;
+    v[8] = SRC ? wave_sum(src) : 0.0;
     const int w = tid / 64, nw = (NT + 63) / 64;
     if ((tid & 63) == 0) {
 #pragma unroll
@@ -79,7 +82,8 @@ __device__ void block_partials(double (&acc)[5], double (&mx)[3], double src, do
         double* p = partials + b * fvl::NSLOT;
         for (int i = 0; i < 9; ++i) {
             double r = red[0][i];
-            for (int k = 1; k < nw; ++k) r = i < 3 ? fmax(r, red[k][i]) : r + red[k][i];
+            if ((i < 3) | (SUMS & (i < 8)) | (SRC & (i == 8)))
+                for (int k = 1; k < nw; ++k) r = i < 3 ? fmax(r, red[k][i]) : r + red[k][i];
             p[i < 3 ? i : (i < 8 ? fvl::S_OUT + (i - 3) : fvl::S_SRC)] = r;
         }
         if (sbad) atomicOr(flags, sbad);
@@ -472,7 +476,7 @@ __global__ void __launch_bounds__(BX* BY, occupancy_of(BX* BY, SPEC)) k_batch3d(
     const int tbz = local / (it.tiles[0] * it.tiles[1]);
     if (s.coef && s.coef[9] == 0.0) {
         double acc[5] = {0, 0, 0, 0, 0}, mx[3] = {0, 0, 0};
-        block_partials<BX * BY>(acc, mx, 0.0, s.partials, s.flags, 0, local);
+        block_partials<BX * BY, false, false>(acc, mx, 0.0, s.partials, s.flags, 0, local);
         return;
     }
     step3d_tile<LIM, BX, BY, SPEC>(s, g, tbx, tby, tbz, local, sbuf, sy, sf, sacc);
@@ -588,87 +592,153 @@ struct Kin {
     double q, ea, a;
 };
 
+// Division and exp of the kinetics. The FP64 plain method keeps IEEE
+// division and the library exp (the reference's arithmetic to a few ulp).
+// The FP32 form keeps double arithmetic for the temperature and the stiff
+// Arrhenius exponent -Ea/(R T), but divides by a Newton-refined hardware
+// reciprocal (r' = r + r (1 - b r), twice, then one residual correction of
+// the quotient) and evaluates exp(x) as 2^n 2^f with x log2(e) = n + f split
+// in double: 2^f (0 <= f < 1) by the hardware exp2 and 2^n exactly by
+// ldexp. Its relative error is about 1e-7, the level of the FP32 storage,
+// and does not grow with the exponent's magnitude.
+#ifdef FV_FAST
+__device__ __forceinline__ double kdiv(double a, double b) {
+    double r = __builtin_amdgcn_rcp(b);
+    double e = fma(-b, r, 1.0);
+    r = fma(e, r, r);
+    e = fma(-b, r, 1.0);
+    r = fma(e, r, r);
+    const double q = a * r;
+    return fma(fma(-b, q, a), r, q);
+}
+__device__ __forceinline__ double kexp(double x) {
+    if (x != x) return x;
+    const double y = fmin(fmax(x * 1.4426950408889634, -1100.0), 1100.0);
+    const double n = floor(y);
+    return ldexp((double)__builtin_amdgcn_exp2f((float)(y - n)), (int)n);
+}
+#else
+__device__ __forceinline__ double kdiv(double a, double b) { return a / b; }
+__device__ __forceinline__ double kexp(double x) { return exp(x); }
+#endif
+
 __device__ __forceinline__ double rate_at(const Kin& k, double lam, double n0, double n1, double n2) {
-    double g = (4.5 - lam) / (3.5 - lam);
+    double g = kdiv(4.5 - lam, 3.5 - lam);
     double rho = fmax(n0, 1e-12);
-    double u = n1 / rho;
+    double u = kdiv(n1, rho);
     double p = fmax((n2 - (0.5 * rho) * (u * u)) * (g - 1.0), 1e-12);
     double rho2 = fmax(rho, 1e-12);
     double molar = lam * (2.0 / 3.0) * 2.016 + lam * (1.0 / 3.0) * 32.0 + (1.0 - lam) * 18.016;
-    double t = fmax((p * molar) / (rho2 * 8314.46), 1e-12);
-    return k.a * exp(-k.ea / (8.314 * t));
+    double t = fmax(kdiv(p * molar, rho2 * 8314.46), 1e-12);
+    return k.a * kexp(kdiv(-k.ea, 8.314 * t));
 }
 
 __device__ __forceinline__ void advance(const Kin& k, double lam0, double beta0, double n0, double n2, double rate,
                                         double dur, double& lam, double& beta, double& e) {
     if (k.model == 1) {
-        double active = fmax(0.0, dur - beta0 / fmax(rate, 1e-300));
-        lam = lam0 * exp(-5.0 * rate * active);
+        double active = fmax(0.0, dur - kdiv(beta0, fmax(rate, 1e-300)));
+        lam = lam0 * kexp(-5.0 * rate * active);
         beta = fmin(fmax(beta0 - rate * dur, 0.0), 1.0);
     } else {
-        lam = lam0 * exp(-rate * dur);
+        lam = lam0 * kexp(-rate * dur);
         beta = 0.0;
     }
     e = n2 + k.q * n0 * (lam0 - lam);
 }
 
-// Chemistry of cell i (one per thread) with half-step duration `dt`;
-// partial slot pblk of r.partials.
-__device__ __forceinline__ void react_cells(const fvl::React& r, const Grid& g, double dt, long i, long pblk) {
-    const int nf = g.dim + 2;
+// Chemistry of cells start, start + stride, ... with half-step duration
+// `dt`; partial slot pblk of r.partials. Each thread walks several cells
+// (react_blocks), so a block folds its partials once per 256 x kReactCells
+// cells; the source sum runs per thread in cell order, then over the block.
+// The next cell's inputs are loaded before this cell's kinetics, so their
+// latency overlaps the arithmetic; outputs are written from registers.
+template <int DIM>
+__device__ __forceinline__ void react_cells_dim(const fvl::React& r, const Grid& g, double dt, long start,
+                                                long stride, long pblk) {
+    constexpr int NF = DIM + 2;
     const long N = (long)g.nx * g.ny * g.nz;
+    const real* __restrict__ U = (const real*)r.U;
+    const real* __restrict__ L = (const real*)r.lam;
+    const real* __restrict__ B = (const real*)r.beta;
+    const real* __restrict__ S = (const real*)r.Ustart;
+    const real* __restrict__ O = (const real*)r.Uold;
+    real* __restrict__ Uo = (real*)r.Uout;
+    real* __restrict__ Lo = (real*)r.lamout;
+    real* __restrict__ Bo = (real*)r.betaout;
+    const Kin k{r.model, r.q_heat, r.activation_energy, r.pre_exponential};
     double acc[5] = {0, 0, 0, 0, 0}, mx[3] = {0, 0, 0}, src = 0;
     int bad = 0;
-    if (i < N) {
-        const real* U = (const real*)r.U;
-        double u[5];
-        for (int c = 0; c < nf; ++c) u[c] = (double)U[c * N + i];
-        const double E = u[nf - 1];
+    real nu[NF], nl = 0, nb = 1, ns = 0, no = 0;
+    auto fetch = [&](long i) {
+#pragma unroll
+        for (int c = 0; c < NF; ++c) nu[c] = U[c * N + i];
+        nl = L[i];
+        if (B) nb = B[i];
+        if (S) {
+            ns = S[(NF - 1) * N + i];
+            no = O[(NF - 1) * N + i];
+        }
+    };
+    if (start < N) fetch(start);
+    for (long i = start; i < N; i += stride) {
+        real cu[NF];
+#pragma unroll
+        for (int c = 0; c < NF; ++c) cu[c] = nu[c];
+        const real cl = nl, cb = nb, cs = ns, co = no;
+        if (i + stride < N) fetch(i + stride);
+        double u[NF];
+#pragma unroll
+        for (int c = 0; c < NF; ++c) u[c] = (double)cu[c];
+        const double E = u[NF - 1];
         double n2;
-        if (g.dim == 3) n2 = E - (0.5 * (u[2] * u[2] + u[3] * u[3])) / u[0];
-        else if (g.dim == 2) n2 = E - (0.5 * (u[2] * u[2])) / u[0];
+        if (DIM == 3) n2 = E - (0.5 * (u[2] * u[2] + u[3] * u[3])) / u[0];
+        else if (DIM == 2) n2 = E - (0.5 * (u[2] * u[2])) / u[0];
         else n2 = E;
-        const double lam0 = (double)((const real*)r.lam)[i];
-        const double beta0 = r.beta ? (double)((const real*)r.beta)[i] : 1.0;
+        const double lam0 = (double)cl;
+        const double beta0 = B ? (double)cb : 1.0;
         bool ok = isfinite(u[0]) && isfinite(u[1]) && isfinite(n2) && u[0] > 1e-12 &&
                   n2 - (0.5 * (u[1] * u[1])) / u[0] > 1e-12;
         ok = ok && isfinite(lam0) && lam0 >= 0.0 && lam0 <= 1.0 && isfinite(beta0) && beta0 >= 0.0 && beta0 <= 1.0;
         if (!ok) bad |= fvl::F_REACT_BAD;
-        Kin k{r.model, r.q_heat, r.activation_energy, r.pre_exponential};
         double lm, bm, em, lam, beta, e;
         advance(k, lam0, beta0, u[0], n2, rate_at(k, lam0, u[0], u[1], n2), 0.5 * dt, lm, bm, em);
         advance(k, lam0, beta0, u[0], n2, rate_at(k, lm, u[0], u[1], em), dt, lam, beta, e);
-        const double Enew = E + (e - n2);
-        real* Uo = (real*)r.Uout;
-        for (int c = 0; c < nf - 1; ++c) Uo[c * N + i] = U[c * N + i];
-        Uo[(nf - 1) * N + i] = (real)Enew;
-        ((real*)r.lamout)[i] = (real)lam;
-        if (r.betaout) ((real*)r.betaout)[i] = (real)beta;
-        if (r.Ustart) {
-            // (old - start + state - hydro) of the energy row (torchSession.py:513-515).
-            const double es = (double)((const real*)r.Ustart)[(nf - 1) * N + i];
-            const double eo = (double)((const real*)r.Uold)[(nf - 1) * N + i];
-            src = ((eo - es) + (double)(real)Enew) - E;
-        }
-        double o[5];
-        for (int c = 0; c < nf; ++c) o[c] = (double)Uo[c * N + i];
+        const real En = (real)(E + (e - n2));
+        const real ln = (real)lam;
+#pragma unroll
+        for (int c = 0; c < NF - 1; ++c) Uo[c * N + i] = cu[c];
+        Uo[(NF - 1) * N + i] = En;
+        Lo[i] = ln;
+        if (Bo) Bo[i] = (real)beta;
+        // (old - start + state - hydro) of the energy row (torchSession.py:513-515).
+        if (S) src += (((double)co - (double)cs) + (double)En) - E;
         double s2 = 0;
-        for (int c = 1; c < nf - 1; ++c) s2 = c == 1 ? o[c] * o[c] : s2 + o[c] * o[c];
-        if (!(o[0] > 1e-12 && o[nf - 1] - (0.5 * s2) / o[0] > 1e-12 && isfinite(o[nf - 1])))
+#pragma unroll
+        for (int c = 1; c < NF - 1; ++c) s2 = c == 1 ? u[c] * u[c] : s2 + u[c] * u[c];
+        if (!(u[0] > 1e-12 && (double)En - (0.5 * s2) / u[0] > 1e-12 && isfinite((double)En)))
             bad |= fvl::F_STATE_BAD;
-        if (r.want_cfl && g.dim == 3) {
+        if (DIM == 3 && r.want_cfl) {
             // The stored state and fraction, as a standalone CFL pass reads them.
             real so[5];
-            for (int c = 0; c < 5; ++c) so[c] = Uo[c * N + i];
-            const double ls = (double)(real)lam;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) so[c] = cu[c];
+            so[4] = En;
+            const double ls = (double)ln;
             wave3(so, (4.5 - ls) / (3.5 - ls), mx);
         }
     }
-    block_partials<256>(acc, mx, src, r.partials, r.flags, bad, pblk);
+    block_partials<256, false, true>(acc, mx, src, r.partials, r.flags, bad, pblk);
+}
+
+__device__ __forceinline__ void react_cells(const fvl::React& r, const Grid& g, double dt, long start, long stride,
+                                            long pblk) {
+    if (g.dim == 3) react_cells_dim<3>(r, g, dt, start, stride, pblk);
+    else if (g.dim == 2) react_cells_dim<2>(r, g, dt, start, stride, pblk);
+    else react_cells_dim<1>(r, g, dt, start, stride, pblk);
 }
 
 __global__ void __launch_bounds__(256) k_react(fvl::React r, Grid g) {
-    react_cells(r, g, r.dt, blockIdx.x * (long)blockDim.x + threadIdx.x, blockIdx.x);
+    react_cells(r, g, r.dt, blockIdx.x * (long)blockDim.x + threadIdx.x, gridDim.x * (long)blockDim.x, blockIdx.x);
 }
 
 // Work-list form over reactive instances: phase 1 is an attempt of the first
@@ -681,9 +751,10 @@ __global__ void __launch_bounds__(256) k_batch_react(const fvl::BatchItem* items
     const double* k = it.step.coef;
     // Phase 1 runs until this step's dt is accepted; phase 2 once it is.
     if (phase == 1 ? (k[fvl::K_CHEM] == 0.0 || k[fvl::K_HYDRO] != 0.0) : k[fvl::K_HYDRO] == 0.0) return;
-    const int local = blockIdx.x - first[i];
-    react_cells(phase == 1 ? it.r1 : it.r2, it.grid, k[fvl::K_HALF], local * 256L + threadIdx.x,
-                (phase == 1 ? it.r1_first : it.r2_first) + local);
+    const int local = blockIdx.x - first[i], nblk = first[i + 1] - first[i];
+    // Batched reactive instances are 3D.
+    react_cells_dim<3>(phase == 1 ? it.r1 : it.r2, it.grid, k[fvl::K_HALF], local * 256L + threadIdx.x, nblk * 256L,
+                       (phase == 1 ? it.r1_first : it.r2_first) + local);
 }
 
 // CFL maxima of a stored 3D state (torchSession.py:356-360).
@@ -698,7 +769,7 @@ __global__ void __launch_bounds__(256) k_cfl3d(const real* U, const real* lam, d
         double gg = lam ? (direct ? (double)lam[i] : (4.5 - (double)lam[i]) / (3.5 - (double)lam[i])) : gamma;
         wave3(o, gg, mx);
     }
-    block_partials<256>(acc, mx, 0.0, partials, nullptr, 0);
+    block_partials<256, false, false>(acc, mx, 0.0, partials, nullptr, 0);
 }
 
 // Conserved totals (5 row sums) for drift reporting.
@@ -838,7 +909,7 @@ __global__ void __launch_bounds__(256) k_batch_cfl3d(const fvl::BatchItem* items
         const double gg = lam ? (4.5 - (double)lam[i]) / (3.5 - (double)lam[i]) : it.step.gamma;
         wave3(o, gg, mx);
     }
-    block_partials<256>(acc, mx, 0.0, it.step.partials, nullptr, 0, it.cfl_first + local);
+    block_partials<256, false, false>(acc, mx, 0.0, it.step.partials, nullptr, 0, it.cfl_first + local);
 }
 
 hipError_t batch_cfl3d(const fvl::BatchItem* items, const int* first, int n, int blocks, hipStream_t st) {
@@ -847,7 +918,11 @@ hipError_t batch_cfl3d(const fvl::BatchItem* items, const int* first, int n, int
     return hipGetLastError();
 }
 
-int react_blocks(const Grid& g) { return (int)(((long)g.nx * g.ny * g.nz + 255) / 256); }
+// Cells per thread of the chemistry passes.
+constexpr long kReactCells = 8;
+int react_blocks(const Grid& g) {
+    return (int)(((long)g.nx * g.ny * g.nz + 256 * kReactCells - 1) / (256 * kReactCells));
+}
 
 hipError_t react(const Grid& g, const fvl::React& r, hipStream_t st) {
     k_react<<<react_blocks(g), 256, 0, st>>>(r, g);
