@@ -76,7 +76,7 @@ struct torch_batch {
     int seg[4] = {0, 0, 0, 0};  // [3D | 2D | 1D] segment starts, seg[3] = n
     int lim[3][6] = {};         // limiter sub-segment starts within each dimension segment
     int rseg = 0;               // first reactive instance of the 3D segment (reactive ones last)
-    int rlim[6] = {};           // limiter sub-segments of the reactive 3D instances
+    int rlim[2][6] = {};        // limiter sub-segments of the reactive 3D instances, per kinetic model
     fvl::BatchItem* items = nullptr;
     fvl::BatchItem* pitems = nullptr;  // copies of the items of new instances (first dt)
     fvl::BatchCtl* ctl = nullptr;
@@ -86,7 +86,7 @@ struct torch_batch {
     fvl::BatchCtl* host_ctl = nullptr;  // pinned
     size_t cap_n = 0, cap_partials = 0, cap_tables = 0, cap_pitems = 0;
     Table hyd[3][5];          // second-order tables per dimension and limiter (inert)
-    Table hydr[5];            // second-order tables of reactive 3D instances per limiter
+    Table hydr[2][5];         // second-order tables of reactive 3D instances per model and limiter
     Table rx;                 // chemistry tables of reactive 3D instances
     Table l3, l2, l1, c1;     // first-order tables per dimension, 1D CFL
     Table p3, p2, p1;         // first-dt tables over pitems
@@ -190,12 +190,14 @@ hipError_t enqueue_3d(torch_batch* b, hipStream_t st) {
                                        L, 0, st);
         if (e != hipSuccess) return e;
     }
-    for (int L = 0; L < 5; ++L) {
-        const Table& t = b->hydr[L];
-        hipError_t e = b->ops->batch3d(b->items + b->rlim[L], b->tables + t.offset, t.count, t.blocks, b->bx, b->by,
-                                       L, 1, st);
-        if (e != hipSuccess) return e;
-    }
+    // Arrhenius instances transport one fraction, ZND ones two.
+    for (int m = 0; m < 2; ++m)
+        for (int L = 0; L < 5; ++L) {
+            const Table& t = b->hydr[m][L];
+            hipError_t e = b->ops->batch3d(b->items + b->rlim[m][L], b->tables + t.offset, t.count, t.blocks, b->bx,
+                                           b->by, L, m + 1, st);
+            if (e != hipSuccess) return e;
+        }
     return hipSuccess;
 }
 
@@ -272,6 +274,8 @@ int rebuild(torch_batch* b) {
         int sx = seg_of(x->grid.dim), sy = seg_of(y->grid.dim);
         if (sx != sy) return sx < sy;
         if (x->cfg.reactive != y->cfg.reactive) return x->cfg.reactive < y->cfg.reactive;
+        if (x->cfg.reactive && x->cfg.kinetic_model != y->cfg.kinetic_model)
+            return x->cfg.kinetic_model < y->cfg.kinetic_model;
         return x->cfg.limiter < y->cfg.limiter;
     });
     const int n = (int)v.size();
@@ -291,11 +295,14 @@ int rebuild(torch_batch* b) {
             b->lim[s][L] = i;
         }
     }
-    for (int L = 0; L <= 5; ++L) {
-        int i = b->rseg;
-        while (i < b->seg[1] && v[i]->cfg.limiter < L) ++i;
-        b->rlim[L] = i;
-    }
+    int mseg[3] = {b->rseg, b->rseg, b->seg[1]};  // [Arrhenius | ZND] within the reactive instances
+    while (mseg[1] < b->seg[1] && v[mseg[1]]->cfg.kinetic_model == 0) ++mseg[1];
+    for (int m = 0; m < 2; ++m)
+        for (int L = 0; L <= 5; ++L) {
+            int i = mseg[m];
+            while (i < mseg[m + 1] && v[i]->cfg.limiter < L) ++i;
+            b->rlim[m][L] = i;
+        }
     long slots = 0;
     for (Inst* p : v) {
         layout(b, p);
@@ -314,7 +321,8 @@ int rebuild(torch_batch* b) {
     std::vector<int> tb;
     for (int s = 0; s < 3; ++s)
         for (int L = 0; L < 5; ++L) b->hyd[s][L] = table(tb, v, b->lim[s][L], b->lim[s][L + 1], &Inst::nblk);
-    for (int L = 0; L < 5; ++L) b->hydr[L] = table(tb, v, b->rlim[L], b->rlim[L + 1], &Inst::nblk);
+    for (int m = 0; m < 2; ++m)
+        for (int L = 0; L < 5; ++L) b->hydr[m][L] = table(tb, v, b->rlim[m][L], b->rlim[m][L + 1], &Inst::nblk);
     b->rx = table(tb, v, b->rseg, b->seg[1], &Inst::nreact);
     b->l3 = table(tb, v, b->seg[0], b->seg[1], &Inst::nllf);
     b->l2 = table(tb, v, b->seg[1], b->seg[2], &Inst::nllf);

@@ -126,21 +126,22 @@ __device__ __forceinline__ void wave3(const real* o, double g, double (&mx)[3]) 
 }
 
 // Waves per SIMD the register allocation of a step kernel aims for. A
-// 256-thread FP32 tile fits three blocks per CU at <= 168 VGPRs, which hides
-// the z-march's load and barrier latency better than two; 256-thread FP64
-// and species-carrying tiles are held to two (<= 256 VGPRs, no accumulation
-// registers) so two blocks share a CU instead of one.
-constexpr int occupancy_of(int threads, bool spec = false) {
-    return threads == 256 ? (sizeof(real) == 4 && !spec ? 3 : 2) : 1;
+// 256-thread FP32 tile with at most one transported fraction fits three
+// blocks per CU at <= 168 VGPRs, which hides the z-march's load and barrier
+// latency better than two; 256-thread FP64 tiles and FP32 tiles carrying two
+// fractions are held to two (<= 256 VGPRs, no accumulation registers) so two
+// blocks share a CU instead of one.
+constexpr int occupancy_of(int threads, int nq = 0) {
+    return threads == 256 ? (sizeof(real) == 4 && nq < 2 ? 3 : 2) : 1;
 }
 
 // LDS footprint of one tile: the A plane with a two-cell halo, the y face
 // states and the y fluxes (separate buffers, so a plane needs two barriers:
 // A plane written, y face states written; the y fluxes written after the
 // second are read after the next plane's first).
-template <int BX, int BY, bool SPEC>
+template <int BX, int BY, int NQ_>
 struct Tile3 {
-    static constexpr int NQ = SPEC ? 2 : 0;
+    static constexpr int NQ = NQ_;
     static constexpr int PW = BX + 2, PH = BY + 2;  // plane: x in [-2, TX+1], y in [-2, TY+1]
     static constexpr int NYS = 6 + NQ;              // y-face payload: state, gamma, fractions
     static constexpr int NFY = 5 + NQ;
@@ -155,10 +156,10 @@ using fvl::coef_of;
 
 // One tile (tbx, tby) of one z chunk tbz; `pblk` is the index of this
 // block's slot in s.partials.
-template <int LIM, int BX, int BY, bool SPEC>
+template <int LIM, int BX, int BY, int NQ_>
 __device__ __forceinline__ void step3d_tile(const Step& s, const G3& g, int tbx, int tby, int tbz, long pblk,
                                             real* sbuf, real* sybuf, real* sfbuf, double* sacc) {
-    using T = Tile3<BX, BY, SPEC>;
+    using T = Tile3<BX, BY, NQ_>;
     constexpr int TX = BX - 2, TY = BY - 2;
     constexpr int NQ = T::NQ;
     constexpr int PW = T::PW, PH = T::PH;
@@ -176,7 +177,7 @@ __device__ __forceinline__ void step3d_tile(const Step& s, const G3& g, int tbx,
     const real* __restrict__ U = (const real*)s.U;
     real* __restrict__ Uo = (real*)s.Uout;
     const real* __restrict__ lamp = (const real*)s.lam;
-    const int nq = SPEC ? s.nq : 0;
+    const int nq = NQ;  // the launcher picks the instance whose s.nq equals NQ
 
     const bool in_active = x >= -1 && x <= nx && y >= -1 && y <= ny;
     const bool owns = lx >= 0 && lx < TX && ly >= 0 && ly < TY && x < nx && y < ny;
@@ -443,15 +444,15 @@ __device__ __forceinline__ void step3d_tile(const Step& s, const G3& g, int tbx,
     block_partials<BX * BY>(acc, mx, 0.0, s.partials, s.flags, bad, pblk);
 }
 
-template <int LIM, int BX, int BY, bool SPEC>
-__global__ void __launch_bounds__(BX* BY, occupancy_of(BX* BY, SPEC)) k_step3d(Step s, G3 g) {
-    using T = Tile3<BX, BY, SPEC>;
+template <int LIM, int BX, int BY, int NQ>
+__global__ void __launch_bounds__(BX* BY, occupancy_of(BX* BY, NQ)) k_step3d(Step s, G3 g) {
+    using T = Tile3<BX, BY, NQ>;
     __shared__ real sbuf[T::BUF];
     __shared__ real sy[T::YS];
     __shared__ real sf[T::FLUX];
     __shared__ double sacc[5 * BX * BY];
     const long b = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
-    step3d_tile<LIM, BX, BY, SPEC>(s, g, blockIdx.x, blockIdx.y, blockIdx.z, b, sbuf, sy, sf, sacc);
+    step3d_tile<LIM, BX, BY, NQ>(s, g, blockIdx.x, blockIdx.y, blockIdx.z, b, sbuf, sy, sf, sacc);
 }
 
 // Work-list form: blocks of many instances of different sizes and
@@ -459,9 +460,9 @@ __global__ void __launch_bounds__(BX* BY, occupancy_of(BX* BY, SPEC)) k_step3d(S
 // instance i with first[i] <= b < first[i+1]; its tile follows from that
 // instance's tile counts. Instances that are idle this step (coef active flag
 // clear) only publish empty partials.
-template <int LIM, int BX, int BY, bool SPEC>
-__global__ void __launch_bounds__(BX* BY, occupancy_of(BX* BY, SPEC)) k_batch3d(const fvl::BatchItem* items, const int* first, int n) {
-    using T = Tile3<BX, BY, SPEC>;
+template <int LIM, int BX, int BY, int NQ>
+__global__ void __launch_bounds__(BX* BY, occupancy_of(BX* BY, NQ)) k_batch3d(const fvl::BatchItem* items, const int* first, int n) {
+    using T = Tile3<BX, BY, NQ>;
     __shared__ real sbuf[T::BUF];
     __shared__ real sy[T::YS];
     __shared__ real sf[T::FLUX];
@@ -479,7 +480,7 @@ __global__ void __launch_bounds__(BX* BY, occupancy_of(BX* BY, SPEC)) k_batch3d(
         block_partials<BX * BY, false, false>(acc, mx, 0.0, s.partials, s.flags, 0, local);
         return;
     }
-    step3d_tile<LIM, BX, BY, SPEC>(s, g, tbx, tby, tbz, local, sbuf, sy, sf, sacc);
+    step3d_tile<LIM, BX, BY, NQ>(s, g, tbx, tby, tbz, local, sbuf, sy, sf, sacc);
 }
 
 // First-order LLF pass over the whole grid, run only when the second-order
@@ -817,32 +818,33 @@ int step3d_blocks(const Grid& g, const fvl::Launch& l, int zchunk) {
     return ((g.nx + tx - 1) / tx) * ((g.ny + ty - 1) / ty) * ((g.nz + zchunk - 1) / zchunk);
 }
 
-template <int BX, int BY, bool SPEC>
+template <int BX, int BY, int NQ>
 static hipError_t launch_lim(const Grid& g, const Step& s, hipStream_t st) {
     dim3 grid((g.nx + BX - 3) / (BX - 2), (g.ny + BY - 3) / (BY - 2), (g.nz + s.zchunk - 1) / s.zchunk);
     dim3 block(BX, BY);
     G3 q = g3(g);
     switch (s.limiter) {
-        case VANLEER: k_step3d<VANLEER, BX, BY, SPEC><<<grid, block, 0, st>>>(s, q); break;
-        case ALBADA: k_step3d<ALBADA, BX, BY, SPEC><<<grid, block, 0, st>>>(s, q); break;
-        case MINBEE: k_step3d<MINBEE, BX, BY, SPEC><<<grid, block, 0, st>>>(s, q); break;
-        case SUPERBEE: k_step3d<SUPERBEE, BX, BY, SPEC><<<grid, block, 0, st>>>(s, q); break;
-        default: k_step3d<UNOS, BX, BY, SPEC><<<grid, block, 0, st>>>(s, q); break;
+        case VANLEER: k_step3d<VANLEER, BX, BY, NQ><<<grid, block, 0, st>>>(s, q); break;
+        case ALBADA: k_step3d<ALBADA, BX, BY, NQ><<<grid, block, 0, st>>>(s, q); break;
+        case MINBEE: k_step3d<MINBEE, BX, BY, NQ><<<grid, block, 0, st>>>(s, q); break;
+        case SUPERBEE: k_step3d<SUPERBEE, BX, BY, NQ><<<grid, block, 0, st>>>(s, q); break;
+        default: k_step3d<UNOS, BX, BY, NQ><<<grid, block, 0, st>>>(s, q); break;
     }
     return hipGetLastError();
 }
 
-template <bool SPEC>
+template <int NQ>
 static hipError_t launch_shape(const Grid& g, const Step& s, const fvl::Launch& l) {
-    if (l.block_x == 64 && l.block_y == 4) return launch_lim<64, 4, SPEC>(g, s, l.stream);
-    if (l.block_x == 64 && l.block_y == 8) return launch_lim<64, 8, SPEC>(g, s, l.stream);
-    if (l.block_x == 32 && l.block_y == 8) return launch_lim<32, 8, SPEC>(g, s, l.stream);
-    if (l.block_x == 32 && l.block_y == 16) return launch_lim<32, 16, SPEC>(g, s, l.stream);
+    if (l.block_x == 64 && l.block_y == 4) return launch_lim<64, 4, NQ>(g, s, l.stream);
+    if (l.block_x == 64 && l.block_y == 8) return launch_lim<64, 8, NQ>(g, s, l.stream);
+    if (l.block_x == 32 && l.block_y == 8) return launch_lim<32, 8, NQ>(g, s, l.stream);
+    if (l.block_x == 32 && l.block_y == 16) return launch_lim<32, 16, NQ>(g, s, l.stream);
     return hipErrorInvalidValue;
 }
 
 hipError_t step3d(const Grid& g, const Step& s, const fvl::Launch& l) {
-    return s.nq > 0 ? launch_shape<true>(g, s, l) : launch_shape<false>(g, s, l);
+    if (s.nq == 2) return launch_shape<2>(g, s, l);
+    return s.nq == 1 ? launch_shape<1>(g, s, l) : launch_shape<0>(g, s, l);
 }
 
 int llf3d_blocks(const Grid& g) {
@@ -855,35 +857,36 @@ hipError_t llf3d(const Grid& g, const Step& s, hipStream_t st) {
     return hipGetLastError();
 }
 
-template <int BX, int BY, bool SPEC>
+template <int BX, int BY, int NQ>
 static void launch_batch3d(const fvl::BatchItem* items, const int* first, int n, int blocks, int limiter,
                            hipStream_t st) {
     const dim3 block(BX, BY);
     switch (limiter) {
-        case VANLEER: k_batch3d<VANLEER, BX, BY, SPEC><<<blocks, block, 0, st>>>(items, first, n); break;
-        case ALBADA: k_batch3d<ALBADA, BX, BY, SPEC><<<blocks, block, 0, st>>>(items, first, n); break;
-        case MINBEE: k_batch3d<MINBEE, BX, BY, SPEC><<<blocks, block, 0, st>>>(items, first, n); break;
-        case SUPERBEE: k_batch3d<SUPERBEE, BX, BY, SPEC><<<blocks, block, 0, st>>>(items, first, n); break;
-        default: k_batch3d<UNOS, BX, BY, SPEC><<<blocks, block, 0, st>>>(items, first, n); break;
+        case VANLEER: k_batch3d<VANLEER, BX, BY, NQ><<<blocks, block, 0, st>>>(items, first, n); break;
+        case ALBADA: k_batch3d<ALBADA, BX, BY, NQ><<<blocks, block, 0, st>>>(items, first, n); break;
+        case MINBEE: k_batch3d<MINBEE, BX, BY, NQ><<<blocks, block, 0, st>>>(items, first, n); break;
+        case SUPERBEE: k_batch3d<SUPERBEE, BX, BY, NQ><<<blocks, block, 0, st>>>(items, first, n); break;
+        default: k_batch3d<UNOS, BX, BY, NQ><<<blocks, block, 0, st>>>(items, first, n); break;
     }
 }
 
-template <bool SPEC>
+template <int NQ>
 static hipError_t batch_shape(const fvl::BatchItem* items, const int* first, int n, int blocks, int bx, int by,
                               int limiter, hipStream_t st) {
-    if (bx == 64 && by == 4) launch_batch3d<64, 4, SPEC>(items, first, n, blocks, limiter, st);
-    else if (bx == 64 && by == 8) launch_batch3d<64, 8, SPEC>(items, first, n, blocks, limiter, st);
-    else if (bx == 32 && by == 8) launch_batch3d<32, 8, SPEC>(items, first, n, blocks, limiter, st);
-    else if (bx == 32 && by == 16) launch_batch3d<32, 16, SPEC>(items, first, n, blocks, limiter, st);
+    if (bx == 64 && by == 4) launch_batch3d<64, 4, NQ>(items, first, n, blocks, limiter, st);
+    else if (bx == 64 && by == 8) launch_batch3d<64, 8, NQ>(items, first, n, blocks, limiter, st);
+    else if (bx == 32 && by == 8) launch_batch3d<32, 8, NQ>(items, first, n, blocks, limiter, st);
+    else if (bx == 32 && by == 16) launch_batch3d<32, 16, NQ>(items, first, n, blocks, limiter, st);
     else return hipErrorInvalidValue;
     return hipGetLastError();
 }
 
 hipError_t batch3d(const fvl::BatchItem* items, const int* first, int n, int blocks, int bx, int by, int limiter,
-                   int spec, hipStream_t st) {
+                   int nq, hipStream_t st) {
     if (blocks <= 0) return hipSuccess;
-    return spec ? batch_shape<true>(items, first, n, blocks, bx, by, limiter, st)
-                : batch_shape<false>(items, first, n, blocks, bx, by, limiter, st);
+    if (nq == 2) return batch_shape<2>(items, first, n, blocks, bx, by, limiter, st);
+    return nq == 1 ? batch_shape<1>(items, first, n, blocks, bx, by, limiter, st)
+                   : batch_shape<0>(items, first, n, blocks, bx, by, limiter, st);
 }
 
 hipError_t batch_llf3d(const fvl::BatchItem* items, const int* first, int n, int blocks, hipStream_t st) {

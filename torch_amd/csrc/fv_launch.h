@@ -332,7 +332,7 @@ struct Launch {
     hipError_t convert_out(const void* src, double* dst, long n, hipStream_t st);         \
     /* batches: items/first in device memory; first[n] = total blocks */                 \
     hipError_t batch3d(const fvl::BatchItem* items, const int* first, int n, int blocks,   \
-                       int bx, int by, int limiter, int spec, hipStream_t st);             \
+                       int bx, int by, int limiter, int nq, hipStream_t st);               \
     hipError_t batch_llf3d(const fvl::BatchItem* items, const int* first, int n,          \
                            int blocks, hipStream_t st);                                    \
     hipError_t batch_cfl3d(const fvl::BatchItem* items, const int* first, int n,          \
