@@ -222,6 +222,12 @@ __device__ __forceinline__ void step3d_tile(const Step& s, const G3& g, int tbx,
 
     load5(U, N, g, x, y, z0 - 2, Ap);
     load5(U, N, g, x, y, z0 - 1, Ac);
+    // FP64 tiles load their ring cell one plane ahead (after the slopes, so
+    // the load overlaps the plane's arithmetic): 2-6% faster there, while
+    // in FP32 the five extra registers cost as much as the latency saved.
+    constexpr bool kRingAhead = sizeof(real) == sizeof(double);
+    real rg[5] = {0, 0, 0, 0, 0};
+    if (kRingAhead && tid < nring) load5(U, N, g, bx0 + rx, by0 + ry, z0 - 1, rg);
     const bool direct = s.lam_is_gamma != 0;
     if (lamp) {
         gp = gamma_from(load1(lamp, g, x, y, z0 - 2), direct);
@@ -242,10 +248,9 @@ __device__ __forceinline__ void step3d_tile(const Step& s, const G3& g, int tbx,
 #pragma unroll
         for (int c = 0; c < 5; ++c) sA[c][ly + 2][lx + 2] = Ac[c];
         if (tid < nring) {
-            real r[5];
-            load5(U, N, g, bx0 + rx, by0 + ry, k, r);
+            if (!kRingAhead) load5(U, N, g, bx0 + rx, by0 + ry, k, rg);
 #pragma unroll
-            for (int c = 0; c < 5; ++c) sA[c][ry + 2][rx + 2] = r[c];
+            for (int c = 0; c < 5; ++c) sA[c][ry + 2][rx + 2] = rg[c];
         }
         __syncthreads();
 
@@ -266,6 +271,7 @@ __device__ __forceinline__ void step3d_tile(const Step& s, const G3& g, int tbx,
             sy[c] = limited<LIM, 3>(Ac[c] - sA[c][ly + 1][lx + 2], sA[c][ly + 3][lx + 2] - Ac[c]);
             sz[c] = limited<LIM, 3>(Ac[c] - Ap[c], An[c] - Ac[c]);
         }
+        if (kRingAhead && k < z1 && tid < nring) load5(U, N, g, bx0 + rx, by0 + ry, k + 1, rg);
         // Half-step predictor (torchSolver3D.py:140-151).
         real P[5];
 #pragma unroll
