@@ -498,7 +498,12 @@ hipError_t step2d(const Grid& g, const Step& s, const fvl::Launch& l) {
 
 // The first-order pass checks the update flag on the device, so it can be
 // queued unconditionally; its partials are read only when it ran.
-int llf2d_blocks(const Grid& g) { return (int)(((long)g.nx * g.ny + 255) / 256); }
+// At most 256 grid-striding blocks: the pass is idle unless a second-order
+// update failed (see llf3d_blocks).
+int llf2d_blocks(const Grid& g) {
+    long b = ((long)g.nx * g.ny + 255) / 256;
+    return (int)(b < 256 ? b : 256);
+}
 
 hipError_t llf2d(const Grid& g, const Step& s, hipStream_t st) {
     k_llf2d<<<llf2d_blocks(g), 256, 0, st>>>(s, g2(g));

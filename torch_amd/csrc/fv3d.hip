@@ -853,9 +853,13 @@ hipError_t step3d(const Grid& g, const Step& s, const fvl::Launch& l) {
     return s.nq == 1 ? launch_shape<1>(g, s, l) : launch_shape<0>(g, s, l);
 }
 
+// Blocks of the first-order pass: it runs only after a failed second-order
+// pass and otherwise every block just reads the flag word and exits, which
+// in a batch's replay costs time per block; 256 grid-striding blocks per
+// instance keep that small.
 int llf3d_blocks(const Grid& g) {
     long b = ((long)g.nx * g.ny * g.nz + 255) / 256;
-    return (int)(b < 2048 ? b : 2048);
+    return (int)(b < 256 ? b : 256);
 }
 
 hipError_t llf3d(const Grid& g, const Step& s, hipStream_t st) {
