@@ -210,7 +210,7 @@ def run_served(seed, count, t_run, budget_gib, round_steps, precision="fp32", op
     return out
 
 
-def run_reactive(seed, count, t_run, steps, warmup, precision="fp32"):
+def run_reactive(seed, count, t_run, steps, warmup, precision="fp32", tile=None):
     """The served ignition requests admitted together into one batch of fixed
     membership (no arrivals or retirements), stepped by graph replays: the
     reactive kernels' throughput without admission work."""
@@ -227,6 +227,10 @@ def run_reactive(seed, count, t_run, steps, warmup, precision="fp32"):
             raise RuntimeError("reactive batch instance not admitted")
         ids.append(ident)
     batch = server.batch
+    if tile == "tune":
+        tile, _ = batch.tune(reps=2)
+    elif tile:
+        batch.set_tiles(*tile)
     batch.run(warmup)
     before = [batch.info(i) for i in ids]
     t0 = time.perf_counter()
@@ -238,7 +242,8 @@ def run_reactive(seed, count, t_run, steps, warmup, precision="fp32"):
     retries = sum(b["retries"] - a["retries"] for a, b in zip(before, after))
     done = sum(b["steps"] - a["steps"] for a, b in zip(before, after))
     return dict(value=round(work / wall, 1), ms_per_replay=round(1e3 * wall / steps, 4), replays=steps,
-                steps=done, retries=retries, cells=sum(b["cells"] for b in after), dtype=precision.replace("fp", "f"))
+                steps=done, retries=retries, cells=sum(b["cells"] for b in after), dtype=precision.replace("fp", "f"),
+                tile=list(tile) if tile else None)
 
 
 # HBM bytes per launch of the 3D step kernel on the 256^3 Sod tube with the
@@ -652,6 +657,8 @@ def main(argv=None):
     ap.add_argument("--serve-round", type=int, default=4, help="steps between admission checks")
     ap.add_argument("--serve-closed-only", action="store_true", help="skip the open-loop (Poisson) run")
     ap.add_argument("--reactive-steps", type=int, default=20, help="graph replays of the fixed reactive batch")
+    ap.add_argument("--reactive-tile", type=_tile, default=None,
+                    help="BX,BY,ZCHUNK of the fixed reactive batch ('tune' runs the autotuner)")
     ap.add_argument("--metric-n", type=int, default=96, help="cells per axis of the metric/face workload")
     ap.add_argument("--metric-steps", type=int, default=20)
     ap.add_argument("--amr-roots", type=_pair, default=(512, 64), help="NX,NY roots of the adaptive workload")
@@ -720,7 +727,7 @@ def main(argv=None):
     if "reactive" in legs:
         served = served or {}
         served["reactive_batch"] = run_reactive(args.serve_seed, args.serve_count, args.serve_t_run,
-                                                args.reactive_steps, 3)
+                                                args.reactive_steps, 3, tile=args.reactive_tile)
         served["reactive_batch"]["workload"] = (
             f"the {args.serve_count} served ignition requests admitted together into one FP32 batch (fixed "
             f"membership), {args.reactive_steps} graph replays after 3")

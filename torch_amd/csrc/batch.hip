@@ -16,6 +16,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <string>
 #include <vector>
 
 #include "capi_common.h"
@@ -108,8 +109,12 @@ void free_inst(Inst* p) {
     (void)hipFree(p->U[0]);
     (void)hipFree(p->U[1]);
     for (auto& kind : p->slot)
-        for (void* q : kind) (void)hipFree(q);
+        for (void*& q : kind) {
+            (void)hipFree(q);
+            q = nullptr;
+        }
     (void)hipFree(p->scratch);
+    p->U[0] = p->U[1] = p->scratch = nullptr;
     delete p;
 }
 
@@ -265,6 +270,60 @@ int read_ctl(torch_batch* b) {
     return TORCH_OK;
 }
 
+// Host-side lifetime check of a rebuild: every device pointer an item holds
+// must lie inside memory the batch holds right now (an instance's buffers or
+// the batch's own arrays), so no item, and no graph captured from the items,
+// refers to a freed allocation. Costs one pass over the items' pointers.
+struct Span {
+    const char* lo;
+    const char* hi;
+};
+
+int check_items(const torch_batch* b, const std::vector<fvl::BatchItem>& items) {
+    std::vector<Span> live;
+    auto add = [&](const void* q, size_t bytes) {
+        if (q) live.push_back({(const char*)q, (const char*)q + bytes});
+    };
+    const size_t e = b->ops->elem;
+    for (const Inst* p : b->inst) {
+        const size_t state = (size_t)p->nf * p->ncell * e, row = (size_t)p->ncell * e;
+        add(p->U[0], state);
+        add(p->U[1], state);
+        for (int j = 0; j < 4; ++j) {
+            add(p->slot[0][j], state);
+            add(p->slot[1][j], row);
+            add(p->slot[2][j], row);
+        }
+        if (p->scratch) add(p->scratch, b->ops->step1d_scratch(p->grid));
+    }
+    add(b->partials, b->cap_partials);
+    add(b->flags, b->cap_n * kFlagStride * sizeof(int));
+    add(b->ctl, b->cap_n * sizeof(fvl::BatchCtl));
+    auto ok = [&](const void* q) {
+        if (!q) return true;
+        const char* c = (const char*)q;
+        for (const Span& sp : live)
+            if (c >= sp.lo && c < sp.hi) return true;
+        return false;
+    };
+    for (size_t i = 0; i < items.size(); ++i) {
+        const fvl::BatchItem& it = items[i];
+        const fvl::Step& s = it.step;
+        bool good = ok(s.U) && ok(s.Uout) && ok(s.lam) && ok(s.q[0]) && ok(s.q[1]) && ok(s.qout[0]) &&
+                    ok(s.qout[1]) && ok(s.F[0]) && ok(s.F[1]) && ok(s.F[2]) && ok(s.flags) && ok(s.partials) &&
+                    ok(s.coef) && ok(it.scratch);
+        for (const fvl::React* r : {&it.r1, &it.r2})
+            good = good && ok(r->U) && ok(r->Uout) && ok(r->lam) && ok(r->lamout) && ok(r->beta) &&
+                   ok(r->betaout) && ok(r->Ustart) && ok(r->Uold) && ok(r->flags) && ok(r->partials);
+        for (int k = 0; k < 3; ++k)
+            for (int j = 0; j < 4; ++j) good = good && ok(it.slot[k][j]);
+        if (!good)
+            return fail(TORCH_HIP_ERROR, "internal: batch item " + std::to_string(i) +
+                                             " refers to memory the batch does not hold");
+    }
+    return TORCH_OK;
+}
+
 // Device tables, items and control records for the current membership and
 // tile choice; derives the first dt of new instances; captures the graph.
 int rebuild(torch_batch* b) {
@@ -413,6 +472,7 @@ int rebuild(torch_batch* b) {
         p->ctl.ticket = 0;
         b->host_ctl[i] = p->ctl;
     }
+    if ((rc = check_items(b, items))) return rc;
     hipStream_t st = b->ctx->stream;
     HIP_TRY(hipMemcpyAsync(b->items, items.data(), n * sizeof(fvl::BatchItem), hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(b->ctl, b->host_ctl, n * sizeof(fvl::BatchCtl), hipMemcpyHostToDevice, st));
