@@ -175,7 +175,7 @@ def run_uneven(seed, count, steps, precisions=("fp32", "fp64"), tile=None):
     return out, err, sum(cells.values())
 
 
-def run_served(seed, count, t_run, budget_gib, round_steps, precision="fp32", open_loop=True):
+def run_served(seed, count, t_run, budget_gib, round_steps, precision="fp32", open_loop=True, policy="backfill"):
     """Served reactive ignition solves: a closed-loop run (all requests at
     t=0) measures capacity, then Poisson arrivals at 80% of it give the
     p50/p99 time-to-solution. Each request's initial arrays arrive with it
@@ -186,7 +186,7 @@ def run_served(seed, count, t_run, budget_gib, round_steps, precision="fp32", op
     specs = served_requests(seed=seed, count=count, t_run=t_run)
     budget = int(budget_gib * (1 << 30))
     out = {}
-    server = Server(precision=precision, hbm_budget=budget, round_steps=round_steps)
+    server = Server(precision=precision, hbm_budget=budget, round_steps=round_steps, policy=policy)
     closed = summary(server.serve([ignition_request(s, 0.0) for s in specs], prebuild=True))
     server.close()
     capacity = closed["completed"] / closed["makespan_s"]
@@ -200,7 +200,7 @@ def run_served(seed, count, t_run, budget_gib, round_steps, precision="fp32", op
     rate = 0.8 * capacity
     arrivals = np.cumsum(rng.exponential(1.0 / rate, size=count))
     arrivals -= arrivals[0]
-    server = Server(precision=precision, hbm_budget=budget, round_steps=round_steps)
+    server = Server(precision=precision, hbm_budget=budget, round_steps=round_steps, policy=policy)
     opened = summary(server.serve([ignition_request(s, a) for s, a in zip(specs, arrivals)], prebuild=True))
     server.close()
     print(f"# served open loop at {rate:.2f} req/s: p50 {opened['p50_s']:.3f} s, p99 {opened['p99_s']:.3f} s",
@@ -656,6 +656,7 @@ def main(argv=None):
     ap.add_argument("--serve-budget-gib", type=float, default=4.0, help="HBM budget of the served batch")
     ap.add_argument("--serve-round", type=int, default=4, help="steps between admission checks")
     ap.add_argument("--serve-closed-only", action="store_true", help="skip the open-loop (Poisson) run")
+    ap.add_argument("--serve-policy", default="backfill", help="admission order: fifo, backfill or largest")
     ap.add_argument("--reactive-steps", type=int, default=20, help="graph replays of the fixed reactive batch")
     ap.add_argument("--reactive-tile", type=_tile, default=None,
                     help="BX,BY,ZCHUNK of the fixed reactive batch ('tune' runs the autotuner)")
@@ -718,7 +719,7 @@ def main(argv=None):
     served = None
     if "served" in legs:
         served = run_served(args.serve_seed, args.serve_count, args.serve_t_run, args.serve_budget_gib,
-                            args.serve_round, open_loop=not args.serve_closed_only)
+                            args.serve_round, open_loop=not args.serve_closed_only, policy=args.serve_policy)
         served["workload"] = (
             f"{args.serve_count} seeded (rng {args.serve_seed}) 3D ignition solves, n^3 with n in 32..160, "
             f"Arrhenius (ZND for ~10%), Ea/RT0 in [20,30], Q/RT0 in [10,50], Gaussian hot spot T/T0 in "

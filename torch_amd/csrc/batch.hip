@@ -27,6 +27,8 @@ namespace {
 
 constexpr int kFlagStride = 16;  // ints per instance flag word (one cache line each)
 constexpr int kAttempts = 3;     // chemistry attempts of a reactive step per graph replay
+constexpr int kSide = 8;         // side streams of a captured step's branches
+constexpr int kMarks = 64;       // fork/join events of one capture
 
 struct Inst {
     int id = 0;
@@ -93,6 +95,10 @@ struct torch_batch {
     Table p3, p2, p1;         // first-dt tables over pitems
     int pseg[4] = {0, 0, 0, 0};
     hipGraphExec_t graph = nullptr;
+    // Side streams and events that fork the captured step into parallel
+    // branches (work lists of different instances); capture only.
+    hipStream_t side[kSide] = {};
+    hipEvent_t mark[kMarks] = {};
     int bx = 32, by = 16, zchunk = 16;  // tile of the 3D work lists (by: see create)
     bool tiles_set = false;  // zchunk chosen by set_tiles or tune; else default_zchunk per instance
     bool profile = false;
@@ -206,6 +212,103 @@ hipError_t enqueue_3d(torch_batch* b, hipStream_t st) {
     return hipSuccess;
 }
 
+// Fork/join of a captured step: a branch starts on a side stream after the
+// work already enqueued on `from`, and `into` continues after the branch.
+// Captured, these events become graph edges, so work lists of different
+// instances become parallel graph branches (no shared buffers: each
+// instance owns its state, partial slots, flags and control record).
+struct Fork {
+    torch_batch* b;
+    int stream = 0, marks = 0;
+    hipError_t err = hipSuccess;
+    hipStream_t open(hipStream_t from) {
+        hipStream_t s = b->side[stream++ % kSide];
+        link(from, s);
+        return s;
+    }
+    void link(hipStream_t from, hipStream_t to) {
+        if (err != hipSuccess) return;
+        if (marks == kMarks) {
+            err = hipErrorOutOfMemory;
+            return;
+        }
+        hipEvent_t ev = b->mark[marks++];
+        err = hipEventRecord(ev, from);
+        if (err == hipSuccess) err = hipStreamWaitEvent(to, ev, 0);
+    }
+};
+
+// One step with each work list in a branch of its own: inert 3D lists from
+// the start, reactive 3D lists after the chemistry attempts, the 2D and 1D
+// pipelines beside the 3D one; every branch joins before the pass that
+// reads all of its segment (first-order 3D after the 3D lists, the post
+// kernel after everything).
+hipError_t enqueue_branched(torch_batch* b, hipStream_t st) {
+    const Ops* o = b->ops;
+    const int* T = b->tables;
+    fvl::BatchItem* I3 = b->items + b->seg[0];
+    fvl::BatchItem* I2 = b->items + b->seg[1];
+    fvl::BatchItem* I1 = b->items + b->seg[2];
+    fvl::BatchItem* IR = b->items + b->rseg;
+    const int nr = b->seg[1] - b->rseg;
+    Fork f{b};
+    hipError_t e;
+    std::vector<hipStream_t> open3;
+    for (int L = 0; L < 5; ++L) {
+        const Table& t = b->hyd[0][L];
+        if (t.blocks <= 0) continue;
+        hipStream_t s = f.open(st);
+        open3.push_back(s);
+        if ((e = o->batch3d(b->items + b->lim[0][L], T + t.offset, t.count, t.blocks, b->bx, b->by, L, 0, s)))
+            return e;
+    }
+    // 2D and 1D pipelines: their lists in parallel, then the segment's
+    // first-order (and 1D CFL) passes after the join.
+    std::vector<hipStream_t> tails;
+    for (int seg = 1; seg <= 2; ++seg) {
+        if (b->seg[seg + 1] == b->seg[seg]) continue;
+        hipStream_t head = f.open(st);
+        std::vector<hipStream_t> lists;
+        for (int L = 0; L < 5; ++L) {
+            const Table& t = b->hyd[seg][L];
+            if (t.blocks <= 0) continue;
+            hipStream_t s = f.open(head);
+            lists.push_back(s);
+            e = seg == 1 ? o->batch2d(b->items + b->lim[1][L], T + t.offset, t.count, t.blocks, L, s)
+                         : o->batch1d(b->items + b->lim[2][L], T + t.offset, t.count, t.blocks, L, s);
+            if (e) return e;
+        }
+        for (hipStream_t s : lists) f.link(s, head);
+        if (seg == 1) {
+            if ((e = o->batch_llf2d(I2, T + b->l2.offset, b->l2.count, b->l2.blocks, head))) return e;
+        } else {
+            if ((e = o->batch_llf1d(I1, T + b->l1.offset, b->l1.count, b->l1.blocks, head))) return e;
+            if ((e = o->batch_cfl1d(I1, T + b->c1.offset, b->c1.count, b->c1.blocks, 0, head))) return e;
+        }
+        tails.push_back(head);
+    }
+    for (int attempt = 0; attempt < (nr ? kAttempts : 0); ++attempt) {
+        if ((e = o->batch_react(IR, T + b->rx.offset, b->rx.count, b->rx.blocks, 1, st))) return e;
+        if ((e = o->batch_check(IR, b->ctl + b->rseg, nr, st))) return e;
+    }
+    for (int m = 0; m < 2; ++m)
+        for (int L = 0; L < 5; ++L) {
+            const Table& t = b->hydr[m][L];
+            if (t.blocks <= 0) continue;
+            hipStream_t s = f.open(st);
+            open3.push_back(s);
+            if ((e = o->batch3d(b->items + b->rlim[m][L], T + t.offset, t.count, t.blocks, b->bx, b->by, L, m + 1,
+                                s)))
+                return e;
+        }
+    for (hipStream_t s : open3) f.link(s, st);
+    if ((e = o->batch_llf3d(I3, T + b->l3.offset, b->l3.count, b->l3.blocks, st))) return e;
+    if ((e = o->batch_react(IR, T + b->rx.offset, b->rx.count, b->rx.blocks, 2, st))) return e;
+    for (hipStream_t s : tails) f.link(s, st);
+    if (f.err != hipSuccess) return f.err;
+    return o->batch_post(b->items, b->ctl, b->seg[3], 0, st);
+}
+
 hipError_t enqueue_step(torch_batch* b, hipStream_t st) {
     const Ops* o = b->ops;
     const int* T = b->tables;
@@ -244,7 +347,7 @@ int capture(torch_batch* b) {
     if (b->seg[3] == 0) return TORCH_OK;
     hipStream_t st = b->ctx->stream;
     HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    hipError_t e = enqueue_step(b, st);
+    hipError_t e = b->side[0] ? enqueue_branched(b, st) : enqueue_step(b, st);
     hipGraph_t g = nullptr;
     hipError_t e2 = hipStreamEndCapture(st, &g);
     if (e != hipSuccess || e2 != hipSuccess) {
@@ -524,6 +627,13 @@ int torch_batch_create(torch_ctx* ctx, int precision, size_t hbm_budget, torch_b
     b->ops = precision == TORCH_FP64 ? &kOps64 : &kOps32;
     b->budget = hbm_budget;
     b->by = precision == TORCH_FP32 ? 8 : 16;  // the single-session default tile
+    hipError_t e = hipSetDevice(ctx->device);
+    for (int i = 0; i < kSide && e == hipSuccess; ++i) e = hipStreamCreateWithFlags(&b->side[i], hipStreamNonBlocking);
+    for (int i = 0; i < kMarks && e == hipSuccess; ++i) e = hipEventCreateWithFlags(&b->mark[i], hipEventDisableTiming);
+    if (e != hipSuccess) {
+        torch_batch_destroy(b);
+        return fail(TORCH_HIP_ERROR, std::string("batch streams: ") + hipGetErrorString(e));
+    }
     *out = b;
     return TORCH_OK;
 }
@@ -544,6 +654,10 @@ int torch_batch_destroy(torch_batch* b) {
     (void)hipHostFree(b->host_ctl);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
+    for (hipEvent_t ev : b->mark)
+        if (ev) (void)hipEventDestroy(ev);
+    for (hipStream_t s : b->side)
+        if (s) (void)hipStreamDestroy(s);
     delete b;
     return TORCH_OK;
 }

@@ -39,6 +39,7 @@ class Request:
         self.status = None
         self.result = None
         self.retries = 0
+        self.skips = 0  # admission passes that found no budget for it
 
     @property
     def latency(self):
@@ -64,10 +65,23 @@ class Server:
     ``hbm_budget`` bounds the device bytes of admitted requests;
     ``round_steps`` is the number of steps between admission/retirement
     checks; ``keep_results`` downloads each finished state.
+
+    ``policy`` orders admission among the requests waiting for budget:
+    ``"fifo"`` admits in arrival order and stops at the first that does not
+    fit; ``"backfill"`` admits, in arrival order, every waiting request that
+    fits; ``"largest"`` does the same largest first (by cells). With
+    backfilling, a request passed over in ``max_skips`` rounds reserves the
+    budget: nothing behind it is admitted until it fits.
     """
 
+    POLICIES = ("fifo", "backfill", "largest")
+
     def __init__(self, precision="fp32", device=0, hbm_budget=0, round_steps=4, max_steps=100000,
-                 keep_results=False):
+                 keep_results=False, policy="backfill", max_skips=16):
+        if policy not in self.POLICIES:
+            raise ValueError(f"policy must be one of {self.POLICIES}")
+        self.policy = policy
+        self.max_skips = int(max_skips)
         self.batch = Batch(precision=precision, device=device, hbm_budget=hbm_budget)
         self.round_steps = int(round_steps)
         self.max_steps = int(max_steps)
@@ -144,15 +158,19 @@ class Server:
                     break
                 waiting.append(item)
                 block = False
-            # Admit in arrival order while the budget allows.
-            still = []
+            # Admission while the budget allows, in the policy's order.
+            if self.policy == "largest":
+                waiting.sort(key=lambda item: (-(item[0].cells or 0), item[0].arrival))
+            still, blocked = [], False
             for req, payload in waiting:
-                if still:
+                if blocked:
                     still.append((req, payload))
                     continue
                 ident = self._admit(req, payload)
                 if ident is None:
                     still.append((req, payload))
+                    req.skips += 1
+                    blocked = self.policy == "fifo" or req.skips > self.max_skips
                 else:
                     req.admitted = clock() - start
                     active[ident] = req
