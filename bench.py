@@ -53,6 +53,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+WORKLOAD_STEPS = 200  # the 3D Sod workload's length (200 steps at CFL 0.8)
 METRIC = "3D cell-updates/s FP32-batched vs FP64 plain; served p50/p99 solve latency"
 
 
@@ -247,16 +248,16 @@ def run_reactive(seed, count, t_run, steps, warmup, precision="fp32", tile=None)
 
 
 # HBM bytes per launch of the 3D step kernel on the 256^3 Sod tube with the
-# default tiles (FP32 32x8, FP64 32x16, z chunks of 24, and of 16), from rocprofv3
-# FETCH_SIZE and WRITE_SIZE counter runs scaled by a 1 GiB device copy
-# (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000); the commands and per-counter
-# values are kept with the profile summaries.
-TRAFFIC = {("f32", 256, (32, 8, 24)): 1503564928, ("f64", 256, (32, 16, 24)): 2208221664,
-           ("f32", 256, (32, 8, 16)): 1563068992, ("f64", 256, (32, 16, 16)): 2288424448}
+# default tiles (FP32 32x8, FP64 32x16, z chunks of 24), from rocprofv3
+# FETCH_SIZE and WRITE_SIZE counter runs over the 200-step workload (mean per
+# dispatch) scaled by a 1 GiB device copy (FETCH_SIZE x 2.000, WRITE_SIZE x
+# 1.000); the commands and per-counter values are kept with the profile
+# summaries.
+TRAFFIC = {("f32", 256, (32, 8, 24)): 1464121217, ("f64", 256, (32, 16, 24)): 2203313824}
 
 # Wave-level VALU instructions per launch of the same kernels (rocprofv3
-# --pmc SQ_INSTS_VALU, median over the dispatches of a 200-step run).
-VALU_INSTS = {}
+# --pmc SQ_INSTS_VALU, mean over the dispatches of the same 200-step runs).
+VALU_INSTS = {("f32", 256, (32, 8, 24)): 240896000, ("f64", 256, (32, 16, 24)): 298922000}
 
 # VALU issue peak of MI355X: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz lane
 # slots per second (a wave64 instruction holds a SIMD for 4 cycles; FP32
@@ -698,6 +699,19 @@ def main(argv=None):
         say(f"# fp32 batched: {r32['ms_per_step']:.3f} ms/step, kernel {r32['kernel_ms']:.3f} ms, tile {r32['tile']}")
         r64b = run_batched_sod(n, args.steps, args.warmup, precision="fp64")
         say(f"# fp64 batched: {r64b['ms_per_step']:.3f} ms/step, kernel {r64b['kernel_ms']:.3f} ms, tile {r64b['tile']}")
+    full = None
+    if args.steps < WORKLOAD_STEPS and ("batched" in legs or "fp64" in legs):
+        # The workload's own length, beside the K-step window (early steps
+        # cross more uniform flow, where the step's shortcuts apply).
+        full = {"steps": WORKLOAD_STEPS, "warmup": 0}
+        if "batched" in legs:
+            f32 = run_batched_sod(n, WORKLOAD_STEPS, 0, tile=args.tile)
+            full["fp32_batched"] = {"value": round(f32["value"], 1), "ms_per_step": round(f32["ms_per_step"], 4)}
+        if "fp64" in legs:
+            f64 = run_precision(n, "fp64", WORKLOAD_STEPS, 0)
+            full["fp64_plain"] = {"value": round(f64["value"], 1), "ms_per_step": round(f64["ms_per_step"], 4),
+                                  "kernel_ms": round(f64["kernel_ms"], 4)}
+        say(f"# full {WORKLOAD_STEPS}-step run: {full}")
     if "error" in legs:
         err = fp32_error(n, args.error_steps)
         say(f"# fp32 vs fp64 after {args.error_steps} steps: L1 {err['l1_rel']} Linf {err['linf_rel']} "
@@ -810,6 +824,7 @@ def main(argv=None):
         "fp32_over_fp64": round(head["value"] / r64["value"], 3) if head and r64 else None,
         "fp32_l1_rel": err["l1_rel"] if err else None,
         "fp32_error": err,
+        "full_length": full,
         "batched": uneven,
         "served": served,
         "metric_face": metric,

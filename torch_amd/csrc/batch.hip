@@ -28,7 +28,7 @@ namespace {
 constexpr int kFlagStride = 16;  // ints per instance flag word (one cache line each)
 constexpr int kAttempts = 3;     // chemistry attempts of a reactive step per graph replay
 constexpr int kSide = 8;         // side streams of a captured step's branches
-constexpr int kMarks = 64;       // fork/join events of one capture
+constexpr int kMarks = 64;       // fork/join events, reused round-robin within a capture
 
 struct Inst {
     int id = 0;
@@ -228,11 +228,9 @@ struct Fork {
     }
     void link(hipStream_t from, hipStream_t to) {
         if (err != hipSuccess) return;
-        if (marks == kMarks) {
-            err = hipErrorOutOfMemory;
-            return;
-        }
-        hipEvent_t ev = b->mark[marks++];
+        // Each event is recorded and waited on back to back, so the pool is
+        // reused round-robin (a capture takes the event's nodes at the wait).
+        hipEvent_t ev = b->mark[marks++ % kMarks];
         err = hipEventRecord(ev, from);
         if (err == hipSuccess) err = hipStreamWaitEvent(to, ev, 0);
     }
