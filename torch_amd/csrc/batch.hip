@@ -37,7 +37,7 @@ struct Inst {
     long ncell = 0;
     int nf = 0;
     void* U[2] = {nullptr, nullptr};   // inert: the two state buffers
-    void* slot[3][4] = {};             // reactive: [U, lambda, beta] x 4 rotating buffers
+    void* slot[3][4] = {};             // reactive: [U, lambda, beta] x kReactSlots rotating buffers
     void* scratch = nullptr;  // 1D interface fluxes
     size_t bytes = 0;
     double volume = 0;
@@ -59,7 +59,7 @@ struct Table {
 size_t state_bytes(const torch_config& c, size_t elem) {
     long ncell = (long)c.n[0] * (c.dim >= 2 ? c.n[1] : 1) * (c.dim >= 3 ? c.n[2] : 1);
     size_t b = 2 * (size_t)(c.dim + 2) * ncell * elem;
-    if (c.reactive) b = 4 * (size_t)(c.dim + 2 + 1 + (c.kinetic_model == 1)) * ncell * elem;
+    if (c.reactive) b = fvl::kReactSlots * (size_t)(c.dim + 2 + 1 + (c.kinetic_model == 1)) * ncell * elem;
     if (c.dim == 1) b += elem * (size_t)(3 * (c.n[0] + 1) + 64);
     // partial slots (default tiles) and bookkeeping
     size_t slots = (size_t)(ncell / 256 + 8) * (c.reactive ? 5 : 3);
@@ -410,7 +410,7 @@ int check_items(const torch_batch* b, const std::vector<fvl::BatchItem>& items) 
     for (size_t i = 0; i < items.size(); ++i) {
         const fvl::BatchItem& it = items[i];
         const fvl::Step& s = it.step;
-        bool good = ok(s.U) && ok(s.Uout) && ok(s.lam) && ok(s.q[0]) && ok(s.q[1]) && ok(s.qout[0]) &&
+        bool good = ok(s.U) && ok(s.Ue) && ok(s.Uout) && ok(s.lam) && ok(s.q[0]) && ok(s.q[1]) && ok(s.qout[0]) &&
                     ok(s.qout[1]) && ok(s.F[0]) && ok(s.F[1]) && ok(s.F[2]) && ok(s.flags) && ok(s.partials) &&
                     ok(s.coef) && ok(it.scratch);
         for (const fvl::React* r : {&it.r1, &it.r2})
@@ -686,7 +686,7 @@ int torch_batch_add(torch_batch* b, const torch_config* config, const double* U,
     hipStream_t st = b->ctx->stream;
     if (c.reactive) {
         const bool znd = c.kinetic_model == 1;
-        for (int j = 0; j < 4 && !rc; ++j) {
+        for (int j = 0; j < fvl::kReactSlots && !rc; ++j) {
             rc = alloc(&p->slot[0][j], p->nf * p->ncell * e);
             if (!rc) rc = alloc(&p->slot[1][j], p->ncell * e);
             if (!rc && znd) rc = alloc(&p->slot[2][j], p->ncell * e);

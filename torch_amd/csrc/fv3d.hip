@@ -35,12 +35,14 @@ __device__ __forceinline__ long cell(int x, int y, int z, int nx, int ny) {
 }
 
 // 5 components of ghost-mapped cell (x, y, z); reflective ghosts negate the
-// momentum normal to that wall.
-__device__ __forceinline__ void load5(const real* __restrict__ U, long N, const G3& g, int x, int y, int z,
-                                      real* a) {
+// momentum normal to that wall. The energy row comes from Ue (U itself for
+// a whole state).
+__device__ __forceinline__ void load5(const real* __restrict__ U, const real* __restrict__ Ue, long N, const G3& g,
+                                      int x, int y, int z, real* a) {
     long i = cell(ghost_src(x, g.nx, g.bcx), ghost_src(y, g.ny, g.bcy), ghost_src(z, g.nz, g.bcz), g.nx, g.ny);
 #pragma unroll
-    for (int c = 0; c < 5; ++c) a[c] = U[c * N + i];
+    for (int c = 0; c < 4; ++c) a[c] = U[c * N + i];
+    a[4] = Ue[4 * N + i];
     if (ghost_flip(x, g.nx, g.bcx)) a[1] = -a[1];
     if (ghost_flip(y, g.ny, g.bcy)) a[2] = -a[2];
     if (ghost_flip(z, g.nz, g.bcz)) a[3] = -a[3];
@@ -175,6 +177,7 @@ __device__ __forceinline__ void step3d_tile(const Step& s, const G3& g, int tbx,
     const long N = (long)nx * ny * nz;
     const int z0 = tbz * s.zchunk, z1 = min(z0 + s.zchunk, nz);
     const real* __restrict__ U = (const real*)s.U;
+    const real* __restrict__ Ue = s.Ue ? (const real*)s.Ue : U;
     real* __restrict__ Uo = (real*)s.Uout;
     const real* __restrict__ lamp = (const real*)s.lam;
     const int nq = NQ;  // the launcher picks the instance whose s.nq equals NQ
@@ -220,14 +223,14 @@ __device__ __forceinline__ void step3d_tile(const Step& s, const G3& g, int tbx,
     real mxr[3] = {0, 0, 0};
     int bad = 0;
 
-    load5(U, N, g, x, y, z0 - 2, Ap);
-    load5(U, N, g, x, y, z0 - 1, Ac);
+    load5(U, Ue, N, g, x, y, z0 - 2, Ap);
+    load5(U, Ue, N, g, x, y, z0 - 1, Ac);
     // FP64 tiles load their ring cell one plane ahead (after the slopes, so
     // the load overlaps the plane's arithmetic): 2-6% faster there, while
     // in FP32 the five extra registers cost as much as the latency saved.
     constexpr bool kRingAhead = sizeof(real) == sizeof(double);
     real rg[5] = {0, 0, 0, 0, 0};
-    if (kRingAhead && tid < nring) load5(U, N, g, bx0 + rx, by0 + ry, z0 - 1, rg);
+    if (kRingAhead && tid < nring) load5(U, Ue, N, g, bx0 + rx, by0 + ry, z0 - 1, rg);
     const bool direct = s.lam_is_gamma != 0;
     if (lamp) {
         gp = gamma_from(load1(lamp, g, x, y, z0 - 2), direct);
@@ -239,7 +242,7 @@ __device__ __forceinline__ void step3d_tile(const Step& s, const G3& g, int tbx,
     }
 
     for (int k = z0 - 1; k <= z1; ++k) {
-        load5(U, N, g, x, y, k + 1, An);
+        load5(U, Ue, N, g, x, y, k + 1, An);
         if (lamp) gn = gamma_from(load1(lamp, g, x, y, k + 1), direct);
         for (int j = 0; j < NQ; ++j) if (j < nq) qn[j] = load1((const real*)s.q[j], g, x, y, k + 1);
 
@@ -248,7 +251,7 @@ __device__ __forceinline__ void step3d_tile(const Step& s, const G3& g, int tbx,
 #pragma unroll
         for (int c = 0; c < 5; ++c) sA[c][ly + 2][lx + 2] = Ac[c];
         if (tid < nring) {
-            if (!kRingAhead) load5(U, N, g, bx0 + rx, by0 + ry, k, rg);
+            if (!kRingAhead) load5(U, Ue, N, g, bx0 + rx, by0 + ry, k, rg);
 #pragma unroll
             for (int c = 0; c < 5; ++c) sA[c][ry + 2][rx + 2] = rg[c];
         }
@@ -271,7 +274,7 @@ __device__ __forceinline__ void step3d_tile(const Step& s, const G3& g, int tbx,
             sy[c] = limited<LIM, 3>(Ac[c] - sA[c][ly + 1][lx + 2], sA[c][ly + 3][lx + 2] - Ac[c]);
             sz[c] = limited<LIM, 3>(Ac[c] - Ap[c], An[c] - Ac[c]);
         }
-        if (kRingAhead && k < z1 && tid < nring) load5(U, N, g, bx0 + rx, by0 + ry, k + 1, rg);
+        if (kRingAhead && k < z1 && tid < nring) load5(U, Ue, N, g, bx0 + rx, by0 + ry, k + 1, rg);
         // Half-step predictor (torchSolver3D.py:140-151).
         real P[5];
 #pragma unroll
@@ -516,9 +519,10 @@ __device__ __forceinline__ void llf3d_cells(const Step& s, const G3& g, long sta
     for (long i = start; i < N; i += stride) {
         const int x = (int)(i % nx), y = (int)((i / nx) % ny), z = (int)(i / ((long)nx * ny));
         const real* U = (const real*)s.U;
+        const real* Ue = s.Ue ? (const real*)s.Ue : U;
         const real* lamp = (const real*)s.lam;
         real Ac[5], o[5], rqv[2];
-        load5(U, N, g, x, y, z, Ac);
+        load5(U, Ue, N, g, x, y, z, Ac);
         const bool direct = s.lam_is_gamma != 0;
         const real gcen = lamp ? gamma_from(load1(lamp, g, x, y, z), direct) : (real)s.gamma;
         real qcen[2] = {0, 0};
@@ -532,8 +536,8 @@ __device__ __forceinline__ void llf3d_cells(const Step& s, const G3& g, long sta
         for (int m = 0; m < 3; ++m) {
             const int dx = m == 0, dy = m == 1, dz = m == 2;
             real Lo[5], Hi[5], Fl[5], Fh[5];
-            load5(U, N, g, x - dx, y - dy, z - dz, Lo);
-            load5(U, N, g, x + dx, y + dy, z + dz, Hi);
+            load5(U, Ue, N, g, x - dx, y - dy, z - dz, Lo);
+            load5(U, Ue, N, g, x + dx, y + dy, z + dz, Hi);
             real glo = lamp ? gamma_from(load1(lamp, g, x - dx, y - dy, z - dz), direct) : (real)s.gamma;
             real ghi = lamp ? gamma_from(load1(lamp, g, x + dx, y + dy, z + dz), direct) : (real)s.gamma;
             llf_face(Lo, Ac, glo, gcen, m + 1, Fl);
@@ -664,14 +668,17 @@ __device__ __forceinline__ void react_cells_dim(const fvl::React& r, const Grid&
                                                 long stride, long pblk) {
     constexpr int NF = DIM + 2;
     const long N = (long)g.nx * g.ny * g.nz;
-    const real* __restrict__ U = (const real*)r.U;
-    const real* __restrict__ L = (const real*)r.lam;
-    const real* __restrict__ B = (const real*)r.beta;
+    // No restrict: an energy-only update may write its input buffers in
+    // place (each cell's outputs are stored after its inputs were read).
+    const real* U = (const real*)r.U;
+    const real* L = (const real*)r.lam;
+    const real* B = (const real*)r.beta;
     const real* __restrict__ S = (const real*)r.Ustart;
     const real* __restrict__ O = (const real*)r.Uold;
-    real* __restrict__ Uo = (real*)r.Uout;
-    real* __restrict__ Lo = (real*)r.lamout;
-    real* __restrict__ Bo = (real*)r.betaout;
+    real* Uo = (real*)r.Uout;
+    real* Lo = (real*)r.lamout;
+    real* Bo = (real*)r.betaout;
+    const bool all_rows = !r.energy_only;
     const Kin k{r.model, r.q_heat, r.activation_energy, r.pre_exponential};
     double acc[5] = {0, 0, 0, 0, 0}, mx[3] = {0, 0, 0}, src = 0;
     int bad = 0;
@@ -712,8 +719,10 @@ __device__ __forceinline__ void react_cells_dim(const fvl::React& r, const Grid&
         advance(k, lam0, beta0, u[0], n2, rate_at(k, lm, u[0], u[1], em), dt, lam, beta, e);
         const real En = (real)(E + (e - n2));
         const real ln = (real)lam;
+        if (all_rows) {
 #pragma unroll
-        for (int c = 0; c < NF - 1; ++c) Uo[c * N + i] = cu[c];
+            for (int c = 0; c < NF - 1; ++c) Uo[c * N + i] = cu[c];
+        }
         Uo[(NF - 1) * N + i] = En;
         Lo[i] = ln;
         if (Bo) Bo[i] = (real)beta;

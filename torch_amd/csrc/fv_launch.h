@@ -123,6 +123,9 @@ struct Amr {
 
 struct Step {
     const void* U;     // input state, (dim+2) fields of ncell values
+    // Energy row of the input state at Ue + (dim+1) * ncell, or null when it
+    // is U's own (a reactive step's first half-step rewrites only that row).
+    const void* Ue;
     void* Uout;        // output state
     const void* lam;   // per-cell lambda for gamma(lambda), or null
     int lam_is_gamma;  // lam holds gamma itself (functional calls with a gamma array)
@@ -190,6 +193,9 @@ struct React {
     int* flags;
     double* partials;
     int want_cfl;
+    // Write only the energy row of Uout (its other rows are U's, unchanged
+    // by the chemistry): Uout may then be U itself, an in-place update.
+    int energy_only;
 };
 
 // One instance of a batch (device memory). Partial slots of an instance:
@@ -215,22 +221,29 @@ struct BatchItem {
 enum CoefSlot { K_HYDRO = 9, K_HALF = 10, K_CHEM = 11, NCOEF = 12 };
 
 // Pointers of a reactive instance for the step whose committed buffer is
-// `cur` (A): R(dt/2) A->B, hydro B->C with species, R(dt/2) C->D with the
-// source integral against A and B (torchSession.py:398-515).
+// `cur` (A), over three rotating buffers: R(dt/2) writes the energy row and
+// fractions of B (density and momenta stay A's: the chemistry does not
+// change them), hydro reads {A's density and momenta, B's energy} and
+// writes C with species, R(dt/2) updates C's energy row and fractions in
+// place with the source integral against A and B (torchSession.py:398-515).
+// C becomes the next step's A.
+constexpr int kReactSlots = 3;
 __host__ __device__ inline void set_reactive_slots(BatchItem& it, int cur) {
-    const int A = cur, B = (cur + 1) % 4, C = (cur + 2) % 4, D = (cur + 3) % 4;
+    const int A = cur, B = (cur + 1) % kReactSlots, C = (cur + 2) % kReactSlots;
     void* const* U = it.slot[0];
     void* const* L = it.slot[1];
     void* const* Bt = it.slot[2];
     it.r1.U = U[A];
     it.r1.Uout = U[B];
+    it.r1.energy_only = 1;
     it.r1.lam = L[A];
     it.r1.lamout = L[B];
     it.r1.beta = it.znd ? Bt[A] : nullptr;
     it.r1.betaout = it.znd ? Bt[B] : nullptr;
     it.r1.Ustart = nullptr;
     it.r1.Uold = nullptr;
-    it.step.U = U[B];
+    it.step.U = U[A];
+    it.step.Ue = U[B];
     it.step.Uout = U[C];
     it.step.lam = L[B];
     it.step.q[0] = L[B];
@@ -238,11 +251,12 @@ __host__ __device__ inline void set_reactive_slots(BatchItem& it, int cur) {
     it.step.q[1] = it.znd ? Bt[B] : nullptr;
     it.step.qout[1] = it.znd ? Bt[C] : nullptr;
     it.r2.U = U[C];
-    it.r2.Uout = U[D];
+    it.r2.Uout = U[C];
+    it.r2.energy_only = 1;
     it.r2.lam = L[C];
-    it.r2.lamout = L[D];
+    it.r2.lamout = L[C];
     it.r2.beta = it.znd ? Bt[C] : nullptr;
-    it.r2.betaout = it.znd ? Bt[D] : nullptr;
+    it.r2.betaout = it.znd ? Bt[C] : nullptr;
     it.r2.Ustart = U[A];
     it.r2.Uold = U[B];
 }

@@ -206,7 +206,7 @@ __global__ void __launch_bounds__(256) k_batch_post(fvl::BatchItem* items, fvl::
         c.steps += 1;
         c.fallbacks += fb ? 1 : 0;
         if (c.reactive) {
-            c.cur = (c.cur + 3) % 4;  // D of this step is A of the next
+            c.cur = (c.cur + 2) % fvl::kReactSlots;  // C of this step is A of the next
             fvl::set_reactive_slots(it, c.cur);
         } else {
             const void* u = it.step.U;
