@@ -96,14 +96,16 @@ def run_precision(n, precision, steps, warmup, tile=None):
     )
 
 
-def run_batched_sod(n, steps, warmup, precision="fp32", tile=None):
-    """The Sod instance as a batch of one: graph-replayed steps, tuned tile."""
+def run_batched_sod(n, steps, warmup, precision="fp32", tile=None, end_pass=False):
+    """The Sod instance as a batch of one: graph-replayed steps, tuned tile.
+    With ``end_pass`` the step kernel is also timed on the state the timed
+    steps end with (the instance stays active for it)."""
     from torch_amd.torchBatch import Batch
 
     batch = Batch(precision=precision)
     ident = batch.add_problem(sod_primitive(n), [1.0, 1.0, 1.0], t_end=OPTIONS["t_end"], cfl=OPTIONS["cfl"],
                               limiter=OPTIONS["limiter"], boundaries=OPTIONS["boundaries"],
-                              max_steps=warmup + steps)
+                              max_steps=warmup + steps + (1 if end_pass else 0))
     if tile == "tune":
         tile, _ = batch.tune(reps=3)
     else:
@@ -115,11 +117,12 @@ def run_batched_sod(n, steps, warmup, precision="fp32", tile=None):
     batch.run(steps)
     wall = time.perf_counter() - t0
     info = batch.info(ident)
+    kernel_end = batch.time_pass(reps=10) if end_pass else None
     batch.close()
     if info["steps"] != warmup + steps or info["status"] != 0:
         raise RuntimeError(f"batched Sod instance stopped early: {info}")
     return dict(wall=wall, ms_per_step=1e3 * wall / steps, value=n ** 3 * steps / wall, kernel_ms=kernel_ms,
-                tile=tile, fallbacks=info["fallbacks"])
+                tile=tile, fallbacks=info["fallbacks"], kernel_ms_end=kernel_end)
 
 
 def run_uneven(seed, count, steps, precisions=("fp32", "fp64"), tile=None):
@@ -705,8 +708,12 @@ def main(argv=None):
         # cross more uniform flow, where the step's shortcuts apply).
         full = {"steps": WORKLOAD_STEPS, "warmup": 0}
         if "batched" in legs:
-            f32 = run_batched_sod(n, WORKLOAD_STEPS, 0, tile=args.tile)
-            full["fp32_batched"] = {"value": round(f32["value"], 1), "ms_per_step": round(f32["ms_per_step"], 4)}
+            f32 = run_batched_sod(n, WORKLOAD_STEPS, 0, tile=args.tile, end_pass=True)
+            t32 = f32["tile"] if f32["tile"] else DEFAULT_TILE["fp32"]
+            full["fp32_batched"] = {
+                "value": round(f32["value"], 1), "ms_per_step": round(f32["ms_per_step"], 4),
+                "kernel_ms_start": round(f32["kernel_ms"], 4), "kernel_ms_end": round(f32["kernel_ms_end"], 4),
+                "roofline_end": roofline(n, 40, f32["kernel_ms_end"], "f32", t32)}
         if "fp64" in legs:
             f64 = run_precision(n, "fp64", WORKLOAD_STEPS, 0)
             full["fp64_plain"] = {"value": round(f64["value"], 1), "ms_per_step": round(f64["ms_per_step"], 4),
