@@ -22,7 +22,8 @@
   in FP64 against the oracle's reactive session (torchSession.py:374-522,
   the dt-halving retry included) at every step: 1e-12, widened only where
   the oracle itself moves by more under half-ulp changes of exp (ignition
-  amplifies rounding).
+  amplifies rounding); and served requests in the FP32 form taking the
+  FP64 method's steps and dt halvings, within 5e-5 relative L1 per field.
 """
 
 import numpy as np
@@ -299,3 +300,46 @@ def test_served_fp64_request_matches_the_oracle(model, monkeypatch):
         assert np.array_equal(served.result[1], sim.lam)
     finally:
         sim.close()
+
+
+@pytest.mark.parametrize("index", [13, 23, 50])
+def test_served_fp32_request_follows_the_fp64_steps(index):
+    """Served requests (rng 355: 13 is n=49 Arrhenius, 23 n=54 Arrhenius, 50
+    n=39 ZND) run to t_run in the FP32 form and the FP64 plain method.
+
+    After ignition nearly every FP64 step rejects its first chemistry
+    attempt: the heating raises the fastest cell's speed by more than the
+    check's 1e-12 tolerance. The FP32 form stores the heated energy in FP32,
+    where heating below half an ulp vanishes; its attempts report the exact
+    speed increment instead, so it must take the same number of steps and
+    the same dt halvings as FP64 (a check on the stored heated state alone
+    accepts many of these attempts, and the two runs then part by ~1e-2).
+    Bar: relative L1 per field within 5e-5 (measured 3.8e-6..3.6e-5, largest
+    on request 23's momenta, which start from rest; ignition amplifies
+    FP32 rounding), the fraction within 5e-6 mean absolute.
+    """
+    from torch_amd import Simulation
+    from torch_amd.torchServe import ignition_request
+    from torch_amd.workloads import served_requests
+
+    spec = served_requests(seed=355, count=64)[index]
+    prim, opts = ignition_request(spec, 0.0).build()
+    out = {}
+    for precision in ("fp64", "fp32"):
+        sim = Simulation(prim, [1.0] * 3, t_end=opts["t_end"], reactive=True, kinetic_model=opts["kinetic_model"],
+                         lambda_init=opts["lambda_init"], constants=opts["constants"], cfl=opts["cfl"],
+                         limiter=opts["limiter"], precision=precision)
+        try:
+            retries = 0
+            while not sim.completed:
+                retries += sim.step().retries
+            out[precision] = (sim.state.copy(), sim.lam.copy(), sim.steps, retries)
+        finally:
+            sim.close()
+    (s32, l32, n32, r32), (s64, l64, n64, r64) = out["fp32"], out["fp64"]
+    l1 = [float(np.abs(s32[c] - s64[c]).sum() / np.abs(s64[c]).sum()) for c in range(5)]
+    lam_l1 = float(np.abs(l32 - l64).mean())
+    say(f"request {index} n={spec['n']} {spec['model']}: {n64} steps, {r64} halvings (FP32 {n32}, {r32}); "
+        f"L1 {['%.2e' % v for v in l1]}, fraction {lam_l1:.2e}")
+    assert n32 == n64 and r32 == r64
+    assert max(l1) <= 5e-5 and lam_l1 <= 5e-6

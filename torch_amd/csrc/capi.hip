@@ -336,6 +336,7 @@ int run_react(torch_state* s, int in, int out, double dt, int start, int old, bo
     r.flags = s->flags;
     r.partials = region(s, 2);
     r.want_cfl = want_cfl && s->cfg.dim == 3;
+    r.attempt = want_cfl && start < 0;
     if (!r.U || !r.Uout || !r.lam || !r.lamout || (s->cfg.kinetic_model == 1 && (!r.beta || !r.betaout)) ||
         (start >= 0 && (!r.Ustart || !r.Uold)))
         return fail(TORCH_INVALID, "internal: chemistry buffers missing for this state");

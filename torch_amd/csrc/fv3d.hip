@@ -628,15 +628,42 @@ __device__ __forceinline__ double kexp(double x) {
     const double n = floor(y);
     return ldexp((double)__builtin_amdgcn_exp2f((float)(y - n)), (int)n);
 }
+// 1/b refined like kdiv's reciprocal; kquot(a, b, ir) = a / b through it.
+__device__ __forceinline__ double krcp(double b) {
+    double r = __builtin_amdgcn_rcp(b);
+    double e = fma(-b, r, 1.0);
+    r = fma(e, r, r);
+    e = fma(-b, r, 1.0);
+    return fma(e, r, r);
+}
+__device__ __forceinline__ double kquot(double a, double, double ir) { return a * ir; }
 #else
 __device__ __forceinline__ double kdiv(double a, double b) { return a / b; }
 __device__ __forceinline__ double kexp(double x) { return exp(x); }
+__device__ __forceinline__ double krcp(double b) { return 1.0 / b; }
+__device__ __forceinline__ double kquot(double a, double b, double) { return a / b; }
 #endif
 
-__device__ __forceinline__ double rate_at(const Kin& k, double lam, double n0, double n1, double n2) {
+// Rate constant A exp(-Ea / (8.314 T)) of a cell with density rho (floored),
+// velocity u = n1 / rho, reduced energy n2 and fraction lam.
+__device__ __forceinline__ double rate_at(const Kin& k, double lam, double rho, double u, double n2) {
+#ifdef FV_FAST
+    // gamma - 1 = 1 / (3.5 - lam), so with neither floor active (p > 1e-12,
+    // T > 1e-12) the exponent -Ea rho R d / (8.314 e M), with e the internal
+    // energy, d = 3.5 - lam, M the molar mass and R = 8314.46, takes one
+    // division instead of the three of gamma, T and the exponent; M is linear
+    // in lam (18.016 - 6.00533.. lam). Cells at a floor take the literal
+    // sequence below.
+    {
+        const double d = 3.5 - lam;
+        const double e = n2 - (0.5 * rho) * (u * u);
+        const double num = e * fma(lam, (2.0 / 3.0) * 2.016 + (1.0 / 3.0) * 32.0 - 18.016, 18.016);
+        const double den = rho * d;
+        if (e > 1e-12 * d && num > (1e-12 * 8314.46) * den)
+            return k.a * kexp(kdiv(-k.ea * (8314.46 / 8.314) * den, num));
+    }
+#endif
     double g = kdiv(4.5 - lam, 3.5 - lam);
-    double rho = fmax(n0, 1e-12);
-    double u = kdiv(n1, rho);
     double p = fmax((n2 - (0.5 * rho) * (u * u)) * (g - 1.0), 1e-12);
     double rho2 = fmax(rho, 1e-12);
     double molar = lam * (2.0 / 3.0) * 2.016 + lam * (1.0 / 3.0) * 32.0 + (1.0 - lam) * 18.016;
@@ -656,6 +683,25 @@ __device__ __forceinline__ void advance(const Kin& k, double lam0, double beta0,
     }
     e = n2 + k.q * n0 * (lam0 - lam);
 }
+
+#ifdef FV_FAST
+// Change of the sound speed a = sqrt(g (g - 1) e / rho), g = 1 + 1/(3.5 - lam),
+// when the chemistry adds de to the internal energy e and moves the fraction
+// from lo to ln, free of cancellation: with go, gn the two gammas,
+// d(a^2) = [(gn - go)(gn + go - 1) e + gn (gn - 1) de] / rho where
+// gn - go = (ln - lo) / ((3.5 - ln)(3.5 - lo)), and da = d(a^2) / (a + a').
+// Exactly zero when the cell did not react. ir = 1/rho.
+__device__ __forceinline__ double heated_sound_speed_change(double rho, double e, double de, double lo, double ln,
+                                                            double ir) {
+    const double dlo = 3.5 - lo, dln = 3.5 - ln;
+    const double go = 1.0 + krcp(dlo), gn = 1.0 + krcp(dln);
+    const double dg = kdiv(ln - lo, dln * dlo);
+    const double da2 = ((dg * (gn + go - 1.0)) * e + (gn * (gn - 1.0)) * de) * ir;
+    const double a2 = (go * (go - 1.0)) * e * ir;
+    const float ao = __builtin_amdgcn_sqrtf((float)a2), an = __builtin_amdgcn_sqrtf((float)fmax(a2 + da2, 0.0));
+    return da2 == 0.0 ? 0.0 : (double)((float)da2 / (ao + an));
+}
+#endif
 
 // Chemistry of cells start, start + stride, ... with half-step duration
 // `dt`; partial slot pblk of r.partials. Each thread walks several cells
@@ -704,19 +750,24 @@ __device__ __forceinline__ void react_cells_dim(const fvl::React& r, const Grid&
 #pragma unroll
         for (int c = 0; c < NF; ++c) u[c] = (double)cu[c];
         const double E = u[NF - 1];
+        // Every quotient by the density goes through one reciprocal in the
+        // FP32 form (a density at or below the floor fails `ok` anyway); the
+        // FP64 plain method divides each time, as the reference does.
+        const double rho = fmax(u[0], 1e-12), ir = krcp(rho);
         double n2;
-        if (DIM == 3) n2 = E - (0.5 * (u[2] * u[2] + u[3] * u[3])) / u[0];
-        else if (DIM == 2) n2 = E - (0.5 * (u[2] * u[2])) / u[0];
+        if (DIM == 3) n2 = E - kquot(0.5 * (u[2] * u[2] + u[3] * u[3]), u[0], ir);
+        else if (DIM == 2) n2 = E - kquot(0.5 * (u[2] * u[2]), u[0], ir);
         else n2 = E;
         const double lam0 = (double)cl;
         const double beta0 = B ? (double)cb : 1.0;
         bool ok = isfinite(u[0]) && isfinite(u[1]) && isfinite(n2) && u[0] > 1e-12 &&
-                  n2 - (0.5 * (u[1] * u[1])) / u[0] > 1e-12;
+                  n2 - kquot(0.5 * (u[1] * u[1]), u[0], ir) > 1e-12;
         ok = ok && isfinite(lam0) && lam0 >= 0.0 && lam0 <= 1.0 && isfinite(beta0) && beta0 >= 0.0 && beta0 <= 1.0;
         if (!ok) bad |= fvl::F_REACT_BAD;
+        const double vel = kquot(u[1], rho, ir);
         double lm, bm, em, lam, beta, e;
-        advance(k, lam0, beta0, u[0], n2, rate_at(k, lam0, u[0], u[1], n2), 0.5 * dt, lm, bm, em);
-        advance(k, lam0, beta0, u[0], n2, rate_at(k, lm, u[0], u[1], em), dt, lam, beta, e);
+        advance(k, lam0, beta0, u[0], n2, rate_at(k, lam0, rho, vel, n2), 0.5 * dt, lm, bm, em);
+        advance(k, lam0, beta0, u[0], n2, rate_at(k, lm, rho, vel, em), dt, lam, beta, e);
         const real En = (real)(E + (e - n2));
         const real ln = (real)lam;
         if (all_rows) {
@@ -731,7 +782,7 @@ __device__ __forceinline__ void react_cells_dim(const fvl::React& r, const Grid&
         double s2 = 0;
 #pragma unroll
         for (int c = 1; c < NF - 1; ++c) s2 = c == 1 ? u[c] * u[c] : s2 + u[c] * u[c];
-        if (!(u[0] > 1e-12 && (double)En - (0.5 * s2) / u[0] > 1e-12 && isfinite((double)En)))
+        if (!(u[0] > 1e-12 && (double)En - kquot(0.5 * s2, u[0], ir) > 1e-12 && isfinite((double)En)))
             bad |= fvl::F_STATE_BAD;
         if (DIM == 3 && r.want_cfl) {
             // The stored state and fraction, as a standalone CFL pass reads them.
@@ -740,6 +791,28 @@ __device__ __forceinline__ void react_cells_dim(const fvl::React& r, const Grid&
             for (int c = 0; c < 4; ++c) so[c] = cu[c];
             so[4] = En;
             const double ls = (double)ln;
+#ifdef FV_FAST
+            if (r.attempt) {
+                // The attempt is accepted when the heated state's speeds do
+                // not exceed the committed state's (which set dt) by more
+                // than 1e-12 relative. Stored in FP32, the heated energy
+                // loses any heating below half an ulp (~6e-8), which the
+                // FP64 method sees and rejects. So report the committed
+                // cell's stored-state speeds plus the exact increment of
+                // its sound speed: with no chemistry the increment is zero
+                // and the maxima are those that set dt.
+                so[4] = cu[4];
+                real v[3];
+                speeds3(so, (real)((4.5 - lam0) / (3.5 - lam0)), v);
+                const double da = heated_sound_speed_change(u[0], n2 - kquot(0.5 * (u[1] * u[1]), u[0], ir),
+                                                            e - n2, lam0, lam, ir);
+#pragma unroll
+                for (int m = 0; m < 3; ++m) {
+                    const double w = (double)v[m] + da;
+                    mx[m] = (w != w || w > mx[m]) ? w : mx[m];
+                }
+            } else
+#endif
             wave3(so, (4.5 - ls) / (3.5 - ls), mx);
         }
     }

@@ -196,6 +196,10 @@ struct React {
     // Write only the energy row of Uout (its other rows are U's, unchanged
     // by the chemistry): Uout may then be U itself, an in-place update.
     int energy_only;
+    // A first half-step attempt, whose maxima only decide whether dt is
+    // accepted (the FP32 form then reports the committed cell's speeds plus
+    // the chemistry's exact speed increment; see react_cells_dim).
+    int attempt;
 };
 
 // One instance of a batch (device memory). Partial slots of an instance:
@@ -242,6 +246,7 @@ __host__ __device__ inline void set_reactive_slots(BatchItem& it, int cur) {
     it.r1.betaout = it.znd ? Bt[B] : nullptr;
     it.r1.Ustart = nullptr;
     it.r1.Uold = nullptr;
+    it.r1.attempt = 1;
     it.step.U = U[A];
     it.step.Ue = U[B];
     it.step.Uout = U[C];
@@ -259,6 +264,7 @@ __host__ __device__ inline void set_reactive_slots(BatchItem& it, int cur) {
     it.r2.betaout = it.znd ? Bt[C] : nullptr;
     it.r2.Ustart = U[A];
     it.r2.Uold = U[B];
+    it.r2.attempt = 0;
 }
 
 // Per-instance control record of a batch (device memory). The host fills
