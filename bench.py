@@ -25,7 +25,10 @@ workload's length). The JSON line reports:
   steps each, FP32 and FP64 batches) with its throughput and FP32 error;
 - served: seeded 3D ignition requests through the continuous-batching
   server (closed loop for capacity, then Poisson arrivals at 80% of it):
-  p50/p99 time-to-solution and aggregate cell-updates/s;
+  p50/p99 time-to-solution and aggregate cell-updates/s, and six requests
+  from the small end (n <= 60; five Arrhenius, one ZND) served in FP32 and
+  FP64 to their end time (steps, dt halvings, per-field relative L1 and Linf
+  of FP32 against FP64);
 - metric_face: the metric/face scheme (curvilinear grid, face boundaries,
   SSPRK2 MUSCL-Rusanov) on a cylindrical-sector blast, FP64 and FP32, with
   its own CPU baseline;
@@ -211,6 +214,45 @@ def run_served(seed, count, t_run, budget_gib, round_steps, precision="fp32", op
           file=sys.stderr, flush=True)
     out["open_loop"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in opened.items()}
     out["arrival_rate_per_s"] = round(rate, 3)
+    return out
+
+
+def served_fp32_error(seed, count, t_run, n_max=60):
+    """Requests from the small end of the served workload (the first five
+    Arrhenius and the first ZND request with n <= n_max) served to their end
+    time in the FP32 form and by the FP64 method: per
+    request the steps and dt halvings of each and the FP32 state's relative
+    L1 and Linf per field against the FP64 one (the FP64 method matches the
+    oracle's reactive session; the tests hold that)."""
+    from torch_amd.torchServe import Server, ignition_request
+    from torch_amd.workloads import served_requests
+
+    small = [s for s in served_requests(seed=seed, count=count, t_run=t_run) if s["n"] <= n_max]
+    specs = [s for s in small if s["model"] == "arrhenius"][:5] + [s for s in small if s["model"] == "znd"][:1]
+    got = {}
+    for precision in ("fp64", "fp32"):
+        server = Server(precision=precision, hbm_budget=0, round_steps=8, keep_results=True)
+        try:
+            reqs = server.serve([ignition_request(s, 0.0) for s in specs])
+        finally:
+            server.close()
+        got[precision] = {r.ident: r for r in reqs}
+    rows = []
+    for spec in specs:
+        a, b = got["fp32"][spec["index"]], got["fp64"][spec["index"]]
+        x, y = a.result[0], b.result[0]
+        l1 = [float(np.sum(np.abs(u - v)) / max(np.sum(np.abs(v)), 1e-300)) for u, v in zip(x, y)]
+        linf = [float(np.max(np.abs(u - v)) / max(np.max(np.abs(v)), 1e-300)) for u, v in zip(x, y)]
+        rows.append(dict(index=spec["index"], n=spec["n"], model=spec["model"], steps=[b.steps, a.steps],
+                         halvings=[b.retries, a.retries], l1_rel=[float(f"{e:.3e}") for e in l1],
+                         linf_rel=[float(f"{e:.3e}") for e in linf]))
+    worst = lambda key: [max(r[key][c] for r in rows) for c in range(5)]
+    out = dict(requests=rows, max_l1_rel_per_field=worst("l1_rel"), max_linf_rel_per_field=worst("linf_rel"),
+               same_steps_and_halvings=all(r["steps"][0] == r["steps"][1] and r["halvings"][0] == r["halvings"][1]
+                                           for r in rows))
+    print(f"# served fp32 vs fp64 ({len(rows)} requests, n <= {n_max}): L1 per field "
+          f"{out['max_l1_rel_per_field']}, Linf {out['max_linf_rel_per_field']}, "
+          f"same steps and halvings: {out['same_steps_and_halvings']}", file=sys.stderr, flush=True)
     return out
 
 
@@ -746,6 +788,7 @@ def main(argv=None):
             f"Arrhenius (ZND for ~10%), Ea/RT0 in [20,30], Q/RT0 in [10,50], Gaussian hot spot T/T0 in "
             f"[1.5,3], run length {args.serve_t_run} (induction ~0.1 of it), VanLeer, CFL 0.4, open "
             f"boundaries; FP32 batch, HBM budget {args.serve_budget_gib} GiB, {args.serve_round} steps/round")
+        served["fp32_vs_fp64"] = served_fp32_error(args.serve_seed, args.serve_count, args.serve_t_run)
     if "reactive" in legs:
         served = served or {}
         served["reactive_batch"] = run_reactive(args.serve_seed, args.serve_count, args.serve_t_run,

@@ -298,7 +298,8 @@ int compute_maxima(torch_state* s) {
     const void* lam = s->cfg.gamma_mode ? s->lam[s->cur] : nullptr;
     hipStream_t st = s->ctx->stream;
     HIP_TRY(clear_flags(s->flags, st));
-    HIP_TRY(s->ops->cfl_cells(s->grid, s->U[s->cur], lam, s->cfg.gamma, region(s, 3), s->cfg.gamma_mode == 2, st));
+    HIP_TRY(s->ops->cfl_cells(s->grid, s->U[s->cur], lam, s->cfg.gamma, region(s, 3), s->cfg.gamma_mode == 2,
+                              nullptr, st));
     HIP_TRY(fvl::finalize(region(s, 3), s->ops->cfl_blocks(s->grid), s->scalars + 3 * fvl::NSLOT, st));
     int rc = sync_read(s);
     if (rc) return rc;
@@ -307,11 +308,13 @@ int compute_maxima(torch_state* s) {
     return TORCH_OK;
 }
 
-// 1D/2D CFL maxima of slot `u` (lambda of slot `l`) into region 3.
-hipError_t run_cfl(torch_state* s, int u, int l) {
+// 1D/2D CFL maxima of slot `u` (lambda of slot `l`) into region 3, raised
+// by a reactive attempt's `increments` when given (FP32 form).
+hipError_t run_cfl(torch_state* s, int u, int l, const void* increments = nullptr) {
     const void* lam = s->cfg.gamma_mode ? s->lam[l] : nullptr;
     hipStream_t st = s->ctx->stream;
-    hipError_t e = s->ops->cfl_cells(s->grid, s->U[u], lam, s->cfg.gamma, region(s, 3), s->cfg.gamma_mode == 2, st);
+    hipError_t e = s->ops->cfl_cells(s->grid, s->U[u], lam, s->cfg.gamma, region(s, 3), s->cfg.gamma_mode == 2,
+                                     increments, st);
     if (e != hipSuccess) return e;
     return fvl::finalize(region(s, 3), s->ops->cfl_blocks(s->grid), s->scalars + 3 * fvl::NSLOT, st);
 }
@@ -337,13 +340,18 @@ int run_react(torch_state* s, int in, int out, double dt, int start, int old, bo
     r.partials = region(s, 2);
     r.want_cfl = want_cfl && s->cfg.dim == 3;
     r.attempt = want_cfl && start < 0;
+    // A 2D or 1D attempt of the FP32 form leaves its per-cell increments in
+    // state slot (in + 3) mod 4 (the second half-step's output, unused until
+    // then) and checks the committed state's maxima raised by them.
+    if (r.attempt && s->cfg.dim != 3 && s->cfg.precision == TORCH_FP32) r.increments = s->U[(in + 3) % 4];
     if (!r.U || !r.Uout || !r.lam || !r.lamout || (s->cfg.kinetic_model == 1 && (!r.beta || !r.betaout)) ||
         (start >= 0 && (!r.Ustart || !r.Uold)))
         return fail(TORCH_INVALID, "internal: chemistry buffers missing for this state");
     hipStream_t st = s->ctx->stream;
     HIP_TRY(s->ops->react(s->grid, r, st));
     HIP_TRY(fvl::finalize(region(s, 2), s->ops->react_blocks(s->grid), s->scalars + 2 * fvl::NSLOT, st));
-    if (want_cfl && s->cfg.dim != 3) HIP_TRY(run_cfl(s, out, out));
+    if (r.increments) HIP_TRY(run_cfl(s, in, in, r.increments));
+    else if (want_cfl && s->cfg.dim != 3) HIP_TRY(run_cfl(s, out, out));
     return TORCH_OK;
 }
 

@@ -53,7 +53,8 @@ struct Ops {
     hipError_t (*step1d)(const fvl::Grid&, const fvl::Step&, void*, hipStream_t);
     size_t (*step1d_scratch)(const fvl::Grid&);
     int (*step1d_blocks)(const fvl::Grid&);
-    hipError_t (*cfl_cells)(const fvl::Grid&, const void*, const void*, double, double*, int, hipStream_t);
+    hipError_t (*cfl_cells)(const fvl::Grid&, const void*, const void*, double, double*, int, const void*,
+                            hipStream_t);
     int (*cfl_blocks)(const fvl::Grid&);
     hipError_t (*react)(const fvl::Grid&, const fvl::React&, hipStream_t);
     int (*react_blocks)(const fvl::Grid&);

@@ -17,20 +17,27 @@ using namespace fv;
 using fvl::Grid;
 using fvl::Step;
 
-struct P1 {
-    real rho, v, p;
+// Primitive state, sound speed and face wave speeds in the arithmetic type
+// T of the caller: the storage precision, or double for the exact change a
+// reactive attempt's heating makes to the CFL maxima (FP32 form).
+template <typename T>
+struct P1T {
+    T rho, v, p;
 };
+using P1 = P1T<real>;
 
-__device__ __forceinline__ P1 prim1(const real* u, real g) {
-    P1 q;
-    q.rho = rmax(u[0], (real)1e-12);
+template <typename T>
+__device__ __forceinline__ P1T<T> prim1(const T* u, T g) {
+    P1T<T> q;
+    q.rho = fmax(u[0], (T)1e-12);
     q.v = u[1] / q.rho;
-    q.p = rmax((u[2] - ((real)0.5 * q.rho) * (q.v * q.v)) * (g - (real)1), (real)1e-12);
+    q.p = fmax((u[2] - ((T)0.5 * q.rho) * (q.v * q.v)) * (g - (T)1), (T)1e-12);
     return q;
 }
 
-__device__ __forceinline__ real sound1(const P1& q, real g) {
-    return sqrt((g * rmax(q.p, (real)0)) / rmax(q.rho, (real)1e-12));
+template <typename T>
+__device__ __forceinline__ T sound1(const P1T<T>& q, T g) {
+    return sqrt((g * fmax(q.p, (T)0)) / fmax(q.rho, (T)1e-12));
 }
 
 __device__ __forceinline__ void eflux1(const real* u, real g, real* f) {
@@ -57,33 +64,34 @@ __device__ __forceinline__ real gamma1(const Step& s, int n, int bc, int j) {
 __device__ __forceinline__ real zero_bad(real x) { return isfinite(x) ? x : (real)0; }
 
 // Wave speeds S_L, S*, S_R at face f (between padded cells f and f+1).
-__device__ void waves1(const real* L, const real* R, real gl, real gr, int mode, real* S) {
-    P1 a = prim1(L, gl), b = prim1(R, gr);
-    real rl = rmax(a.rho, (real)1e-12), rr = rmax(b.rho, (real)1e-12);
-    real al = sound1(a, gl), ar = sound1(b, gr);
-    real sl, sr;
+template <typename T>
+__device__ void waves1(const T* L, const T* R, T gl, T gr, int mode, T* S) {
+    P1T<T> a = prim1(L, gl), b = prim1(R, gr);
+    T rl = fmax(a.rho, (T)1e-12), rr = fmax(b.rho, (T)1e-12);
+    T al = sound1(a, gl), ar = sound1(b, gr);
+    T sl, sr;
     if (mode == 0) {
-        real ps = nmax((real)0, (real)0.5 * (a.p + b.p) -
-                                    (real)0.5 * (b.v - a.v) * ((real)0.5 * (b.rho + a.rho)) * ((real)0.5 * (al + ar)));
-        real ql = (real)1, qr = (real)1;
-        if (ps > a.p) ql = sqrt((real)1 + (gl + (real)1) / ((real)2 * gl) * (ps / a.p - (real)1));
-        if (ps > b.p) qr = sqrt((real)1 + (gr + (real)1) / ((real)2 * gr) * (ps / b.p - (real)1));
+        T ps = nmax((T)0, (T)0.5 * (a.p + b.p) - (T)0.5 * (b.v - a.v) * ((T)0.5 * (b.rho + a.rho)) * ((T)0.5 * (al + ar)));
+        T ql = (T)1, qr = (T)1;
+        if (ps > a.p) ql = sqrt((T)1 + (gl + (T)1) / ((T)2 * gl) * (ps / a.p - (T)1));
+        if (ps > b.p) qr = sqrt((T)1 + (gr + (T)1) / ((T)2 * gr) * (ps / b.p - (T)1));
         sl = a.v - al * ql;
         sr = b.v + ar * qr;
     } else {
-        real elt = rl * ((real)0.5 * (a.v * a.v) + a.p / ((gl - (real)1) * rl));
-        real ert = rr * ((real)0.5 * (b.v * b.v) + b.p / ((gr - (real)1) * rr));
-        real sql = sqrt(rl), sqr = sqrt(rr);
-        real den = rmax(sql + sqr, (real)1e-12);
-        real ua = (sql * a.v + sqr * b.v) / den;
-        real ha = (sql * ((elt + a.p) / rl) + sqr * ((ert + b.p) / rr)) / den;
-        real ga = (real)0.5 * (gl + gr);
-        real aa = sqrt(rmax((ga - (real)1) * (ha - (real)0.5 * (ua * ua)), (real)1e-10));
+        T elt = rl * ((T)0.5 * (a.v * a.v) + a.p / ((gl - (T)1) * rl));
+        T ert = rr * ((T)0.5 * (b.v * b.v) + b.p / ((gr - (T)1) * rr));
+        T sql = sqrt(rl), sqr = sqrt(rr);
+        T den = fmax(sql + sqr, (T)1e-12);
+        T ua = (sql * a.v + sqr * b.v) / den;
+        T ha = (sql * ((elt + a.p) / rl) + sqr * ((ert + b.p) / rr)) / den;
+        T ga = (T)0.5 * (gl + gr);
+        T aa = sqrt(fmax((ga - (T)1) * (ha - (T)0.5 * (ua * ua)), (T)1e-10));
         sl = nmin(a.v - al, ua - aa);
         sr = nmax(b.v + ar, ua + aa);
     }
-    real den = guard14(rl * (sl - a.v) - rr * (sr - b.v));
-    real ss = (b.p - a.p + (rl * a.v) * (sl - a.v) - (rr * b.v) * (sr - b.v)) / den;
+    T den = rl * (sl - a.v) - rr * (sr - b.v);
+    den = fabs(den) < (T)1e-14 ? (T)1e-14 : den;
+    T ss = (b.p - a.p + (rl * a.v) * (sl - a.v) - (rr * b.v) * (sr - b.v)) / den;
     ss = nmin(nmax(ss, sl), sr);
     S[0] = sl;
     S[1] = ss;
@@ -319,8 +327,38 @@ __device__ void llf1d_tile(const Step& s, int n, int bc, int tile, real* Fs, Lds
     update1(s, n, bc, i0, (real)K.c[0], m, fvl::F_LLF_BAD);
 }
 
+#ifdef FV_FAST
+// A reactive attempt of the FP32 form (Step.heat: the chemistry's energy and
+// fraction increments per cell): the exact change the heating makes to face
+// f's |S_L|, |S*|, |S_R| and to |u| + a of its two cells, from the committed
+// and the heated states both evaluated in double. With no chemistry in
+// either cell the two evaluations are identical and every change is zero.
+__device__ void heat_change1(const Step& s, int n, int bc, int f, const real* L, const real* R, double* dS,
+                             double& da, double& db) {
+    const real* dE = (const real*)s.heat;
+    const real* dl = dE + n;
+    const int il = ghost_src(f - 1, n, bc), ir = ghost_src(f, n, bc);
+    const double ll = s.lam ? (double)((const real*)s.lam)[il] : 0.0;
+    const double lr = s.lam ? (double)((const real*)s.lam)[ir] : 0.0;
+    const double llh = ll + (double)dl[il], lrh = lr + (double)dl[ir];
+    auto gam = [&](double l) { return s.lam ? (4.5 - l) / (3.5 - l) : s.gamma; };
+    const double Lc[3] = {L[0], L[1], L[2]}, Rc[3] = {R[0], R[1], R[2]};
+    const double Lh[3] = {L[0], L[1], (double)L[2] + (double)dE[il]};
+    const double Rh[3] = {R[0], R[1], (double)R[2] + (double)dE[ir]};
+    double Sc[3], Sh[3];
+    waves1(Lc, Rc, gam(ll), gam(lr), s.wavespeed, Sc);
+    waves1(Lh, Rh, gam(llh), gam(lrh), s.wavespeed, Sh);
+    for (int c = 0; c < 3; ++c) dS[c] = fabs(Sh[c]) - fabs(Sc[c]);
+    const P1T<double> ac = prim1(Lc, gam(ll)), ah = prim1(Lh, gam(llh));
+    const P1T<double> bc0 = prim1(Rc, gam(lr)), bh = prim1(Rh, gam(lrh));
+    da = (fabs(ah.v) + sound1(ah, gam(llh))) - (fabs(ac.v) + sound1(ac, gam(ll)));
+    db = (fabs(bh.v) + sound1(bh, gam(lrh))) - (fabs(bc0.v) + sound1(bc0, gam(lr)));
+}
+#endif
+
 // CFL maxima of faces [f0, f0+256): max |S| (slot 0) and max |u|+a over the
-// padded cells (slot 1).
+// padded cells (slot 1); for a reactive attempt of the FP32 form (Step.heat)
+// those of the committed state raised by the heating's exact change.
 __device__ void cfl1d_faces(const Step& s, int n, int bc, int f0, double* partials, long pblk) {
     __shared__ double red[4][2];
     const int f = f0 + threadIdx.x;
@@ -331,12 +369,16 @@ __device__ void cfl1d_faces(const Step& s, int n, int bc, int f0, double* partia
         cell1((const real*)s.U, n, bc, f + 1, R);
         real gl = gamma1(s, n, bc, f), gr = gamma1(s, n, bc, f + 1);
         waves1(L, R, gl, gr, s.wavespeed, S);
-        for (int c = 0; c < 3; ++c) m0 = fmax(m0, (double)fabs(S[c]));
+        double dS[3] = {0, 0, 0}, da = 0, db = 0;
+#ifdef FV_FAST
+        if (s.heat) heat_change1(s, n, bc, f, L, R, dS, da, db);
+#endif
+        for (int c = 0; c < 3; ++c) m0 = fmax(m0, (double)fabs(S[c]) + dS[c]);
         P1 a = prim1(L, gl);
-        m1 = (double)(fabs(a.v) + sound1(a, gl));
+        m1 = (double)(fabs(a.v) + sound1(a, gl)) + da;
         if (f == n) {
             P1 b = prim1(R, gr);
-            m1 = fmax(m1, (double)(fabs(b.v) + sound1(b, gr)));
+            m1 = fmax(m1, (double)(fabs(b.v) + sound1(b, gr)) + db);
         }
     }
     m0 = wave_max(m0);

@@ -406,8 +406,11 @@ __global__ void __launch_bounds__(256) k_batch_cfl2d(const fvl::BatchItem* items
 
 // CFL maxima over the padded 2D state: interior cells and, with an Inflow
 // boundary, the inlet ghost state at gamma 1.4.
+// With `dspeed`, each cell's speeds plus its sound-speed increment (the
+// inflow state has none): the FP32 form's check of a reactive attempt, over
+// the committed state whose own maxima set dt.
 __global__ void __launch_bounds__(256) k_cfl2d(const real* U, const real* lam, double gamma, G2 g,
-                                               double* partials, int direct) {
+                                               double* partials, int direct, const real* dspeed) {
     const long N = (long)g.nx * g.ny;
     const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
     double acc[5] = {0, 0, 0, 0, 0}, mx[3] = {0, 0, 0};
@@ -424,6 +427,10 @@ __global__ void __launch_bounds__(256) k_cfl2d(const real* U, const real* lam, d
         Prim2 p = prim2(u, gg);
         mx[0] = (double)(fabs(p.ux) + p.a);
         mx[1] = (double)(fabs(p.vy) + p.a);
+        if (dspeed && i < N) {
+            mx[0] += (double)dspeed[i];
+            mx[1] += (double)dspeed[i];
+        }
     }
     partials2<256>(acc, mx, partials, nullptr, 0);
 }
@@ -511,9 +518,9 @@ hipError_t llf2d(const Grid& g, const Step& s, hipStream_t st) {
 }
 
 hipError_t cfl2d(const Grid& g, const void* U, const void* lam, double gamma, double* partials, int direct,
-                 hipStream_t st) {
-    k_cfl2d<<<(unsigned)(((long)g.nx * g.ny + 256) / 256), 256, 0, st>>>((const real*)U, (const real*)lam, gamma,
-                                                                          g2(g), partials, direct);
+                 const void* dspeed, hipStream_t st) {
+    k_cfl2d<<<(unsigned)(((long)g.nx * g.ny + 256) / 256), 256, 0, st>>>(
+        (const real*)U, (const real*)lam, gamma, g2(g), partials, direct, (const real*)dspeed);
     return hipGetLastError();
 }
 

@@ -784,6 +784,17 @@ __device__ __forceinline__ void react_cells_dim(const fvl::React& r, const Grid&
         for (int c = 1; c < NF - 1; ++c) s2 = c == 1 ? u[c] * u[c] : s2 + u[c] * u[c];
         if (!(u[0] > 1e-12 && (double)En - kquot(0.5 * s2, u[0], ir) > 1e-12 && isfinite((double)En)))
             bad |= fvl::F_STATE_BAD;
+#ifdef FV_FAST
+        // 2D and 1D attempts: the increments go to the CFL pass over the
+        // committed state (the same check as the 3D maxima below).
+        if (DIM == 2 && r.increments)
+            ((real*)r.increments)[i] = (real)heated_sound_speed_change(
+                u[0], n2 - kquot(0.5 * (u[1] * u[1]), u[0], ir), e - n2, lam0, lam, ir);
+        if (DIM == 1 && r.increments) {
+            ((real*)r.increments)[i] = (real)(e - n2);
+            ((real*)r.increments)[N + i] = (real)(lam - lam0);
+        }
+#endif
         if (DIM == 3 && r.want_cfl) {
             // The stored state and fraction, as a standalone CFL pass reads them.
             real so[5];
@@ -1037,9 +1048,16 @@ int cfl_blocks(const Grid& g) {
     return (int)(((long)g.nx * g.ny * g.nz + 255) / 256);
 }
 
+// `increments` (FP32 form, 2D and 1D, else null): the chemistry's per-cell
+// increments of a reactive attempt; the maxima are then those of U (the
+// committed state) raised by the exact change the heating makes to them.
 hipError_t cfl_cells(const Grid& g, const void* U, const void* lam, double gamma, double* partials, int direct,
-                     hipStream_t st) {
-    if (g.dim == 2) return cfl2d(g, U, lam, gamma, partials, direct, st);
+                     const void* increments, hipStream_t st) {
+#ifndef FV_FAST
+    if (increments) return hipErrorInvalidValue;
+#endif
+    if (g.dim == 2) return cfl2d(g, U, lam, gamma, partials, direct, increments, st);
+    if (g.dim == 3 && increments) return hipErrorInvalidValue;
     if (g.dim == 1) {
         Step s;
         memset(&s, 0, sizeof(s));
@@ -1049,6 +1067,7 @@ hipError_t cfl_cells(const Grid& g, const void* U, const void* lam, double gamma
         s.gamma = gamma;
         s.wavespeed = g.wavespeed;
         s.partials = partials;
+        s.heat = increments;
         return cfl1d(g, s, st);
     }
     k_cfl3d<<<cfl_blocks(g), 256, 0, st>>>((const real*)U, (const real*)lam, gamma, g, partials, direct);

@@ -150,6 +150,9 @@ struct Step {
     // of c/hc/w: [dt/h x3, 0.5*dt/h x3, dt*vol/h x3, hydro gate, ...]
     // (enum CoefSlot).
     const double* coef;
+    // 1D CFL pass of an FP32-form reactive attempt: the chemistry's energy
+    // and fraction increments per cell (two rows of n values), or null.
+    const void* heat;
 };
 
 // Step coefficients: from the host (Step.c/hc/w) or, for graph-replayed
@@ -200,6 +203,10 @@ struct React {
     // accepted (the FP32 form then reports the committed cell's speeds plus
     // the chemistry's exact speed increment; see react_cells_dim).
     int attempt;
+    // Attempts of the FP32 form in 2D and 1D, for the CFL pass over the
+    // committed state: each cell's sound-speed increment (2D, one row of
+    // ncell values) or its energy and fraction increments (1D, two rows).
+    void* increments;
 };
 
 // One instance of a batch (device memory). Partial slots of an instance:
@@ -333,7 +340,8 @@ struct Launch {
     hipError_t llf2d(const fvl::Grid& g, const fvl::Step& s, hipStream_t st);             \
     int llf2d_blocks(const fvl::Grid& g);                                                 \
     hipError_t cfl2d(const fvl::Grid& g, const void* U, const void* lam, double gamma,    \
-                     double* partials, int lam_is_gamma, hipStream_t st);                 \
+                     double* partials, int lam_is_gamma, const void* increments,           \
+                     hipStream_t st);                                                     \
     int cfl2d_blocks(const fvl::Grid& g);                                                 \
     hipError_t cfl1d(const fvl::Grid& g, const fvl::Step& s, hipStream_t st);             \
     int cfl1d_blocks(const fvl::Grid& g);                                                 \
@@ -341,7 +349,8 @@ struct Launch {
     size_t step1d_scratch(const fvl::Grid& g);                                            \
     int step1d_blocks(const fvl::Grid& g);                                                \
     hipError_t cfl_cells(const fvl::Grid& g, const void* U, const void* lam, double gamma,\
-                         double* partials, int lam_is_gamma, hipStream_t st);            \
+                         double* partials, int lam_is_gamma, const void* increments,      \
+                         hipStream_t st);                                                 \
     int cfl_blocks(const fvl::Grid& g);                                                   \
     hipError_t react(const fvl::Grid& g, const fvl::React& r, hipStream_t st);            \
     int react_blocks(const fvl::Grid& g);                                                 \
