@@ -23,7 +23,7 @@
   the dt-halving retry included) at every step: 1e-12, widened only where
   the oracle itself moves by more under half-ulp changes of exp (ignition
   amplifies rounding); and served requests in the FP32 form taking the
-  FP64 method's steps and dt halvings, within 5e-5 relative L1 per field.
+  FP64 method's steps and dt halvings, within 3e-5 relative L1 per field.
 """
 
 import numpy as np
@@ -314,9 +314,9 @@ def test_served_fp32_request_follows_the_fp64_steps(index):
     speed increment instead, so it must take the same number of steps and
     the same dt halvings as FP64 (a check on the stored heated state alone
     accepts many of these attempts, and the two runs then part by ~1e-2).
-    Bar: relative L1 per field within 5e-5 (measured 3.8e-6..3.6e-5, largest
-    on request 23's momenta, which start from rest; ignition amplifies
-    FP32 rounding), the fraction within 5e-6 mean absolute.
+    Bar: relative L1 per field within 3e-5 (measured 2.2e-6..1.3e-5, largest
+    on request 23's momenta, which start from rest; ignition amplifies FP32
+    rounding), the fraction within 5e-6 mean absolute.
     """
     from torch_amd import Simulation
     from torch_amd.torchServe import ignition_request
@@ -342,7 +342,7 @@ def test_served_fp32_request_follows_the_fp64_steps(index):
     say(f"request {index} n={spec['n']} {spec['model']}: {n64} steps, {r64} halvings (FP32 {n32}, {r32}); "
         f"L1 {['%.2e' % v for v in l1]}, fraction {lam_l1:.2e}")
     assert n32 == n64 and r32 == r64
-    assert max(l1) <= 5e-5 and lam_l1 <= 5e-6
+    assert max(l1) <= 3e-5 and lam_l1 <= 5e-6
 
 
 def sliced_ignition(spec, dim):
@@ -371,10 +371,11 @@ def test_sliced_request_fp32_follows_the_fp64_steps(dim):
     double before and after the heating), so both take the same steps and dt
     halvings; checked on the stored heated state instead, the FP32 runs take
     166 steps with 72 halvings (2D, FP64: 241 and 224) and 521 with 434 (1D,
-    FP64: 601 and 596). Bar: 2e-5 relative L1 per field (measured 4.0e-6 in
-    2D, 9.0e-6 in 1D; on request 9's 1D slice, with equal steps and
-    halvings, 3.0e-4 on the momentum, as the ignition amplifies FP32
-    rounding)."""
+    FP64: 601 and 596). The advance factor exp(-rate dt) is evaluated to
+    ~1e-13 (the FP32 float exp2 split lost the heat of weakly reacting
+    cells: request 9's 1D slice then ended 3.0e-4 from FP64 with equal
+    steps). Bar: 2e-5 relative L1 per field (measured 2.6e-6 in 2D, 3.3e-6
+    in 1D; 2.3e-5 on request 9's 1D slice)."""
     from torch_amd import Simulation
     from torch_amd.workloads import served_requests
 

@@ -320,7 +320,8 @@ hipError_t run_cfl(torch_state* s, int u, int l, const void* increments = nullpt
 }
 
 // Chemistry half-step slot `in` -> slot `out` (state, lambda, beta).
-int run_react(torch_state* s, int in, int out, double dt, int start, int old, bool want_cfl) {
+int run_react(torch_state* s, int in, int out, double dt, int start, int old, bool want_cfl,
+              void* speed_increments = nullptr) {
     fvl::React r;
     memset(&r, 0, sizeof(r));
     r.U = s->U[in];
@@ -342,15 +343,23 @@ int run_react(torch_state* s, int in, int out, double dt, int start, int old, bo
     r.attempt = want_cfl && start < 0;
     // A 2D or 1D attempt of the FP32 form leaves its per-cell increments in
     // state slot (in + 3) mod 4 (the second half-step's output, unused until
-    // then) and checks the committed state's maxima raised by them.
-    if (r.attempt && s->cfg.dim != 3 && s->cfg.precision == TORCH_FP32) r.increments = s->U[(in + 3) % 4];
+    // then) and checks the committed state's maxima raised by them; a metric
+    // attempt passes its own buffer for the sound-speed increments.
+    if (r.attempt && s->cfg.dim != 3 && s->cfg.precision == TORCH_FP32) {
+        r.increments = s->U[(in + 3) % 4];
+        r.increment_kind = s->cfg.dim == 1 ? fvl::INC_HEAT : fvl::INC_SPEED;
+    }
+    if (speed_increments) {
+        r.increments = speed_increments;
+        r.increment_kind = fvl::INC_SPEED;
+    }
     if (!r.U || !r.Uout || !r.lam || !r.lamout || (s->cfg.kinetic_model == 1 && (!r.beta || !r.betaout)) ||
         (start >= 0 && (!r.Ustart || !r.Uold)))
         return fail(TORCH_INVALID, "internal: chemistry buffers missing for this state");
     hipStream_t st = s->ctx->stream;
     HIP_TRY(s->ops->react(s->grid, r, st));
     HIP_TRY(fvl::finalize(region(s, 2), s->ops->react_blocks(s->grid), s->scalars + 2 * fvl::NSLOT, st));
-    if (r.increments) HIP_TRY(run_cfl(s, in, in, r.increments));
+    if (want_cfl && r.increments) HIP_TRY(run_cfl(s, in, in, r.increments));
     else if (want_cfl && s->cfg.dim != 3) HIP_TRY(run_cfl(s, out, out));
     return TORCH_OK;
 }
@@ -435,11 +444,13 @@ int boundary_error(int flags) {
 }
 
 // cfl_dt of slot u (torchFiniteVolume.py:66-98).
-int metric_dt(torch_state* s, int u, double* dt) {
+// CFL dt of slot `u`; with `increments` (FP32 reactive attempts) the rates
+// of `u`, the committed state, raised by the chemistry's sound-speed change.
+int metric_dt(torch_state* s, int u, double* dt, const void* increments = nullptr) {
     hipStream_t st = s->ctx->stream;
     HIP_TRY(clear_flags(s->flags, st));
     HIP_TRY(s->ops->metric_cfl(s->grid, s->metric, s->U[u], frac(s, u, 0), s->cfg.gamma, s->cfg.reactive,
-                               region(s, 3), s->flags, st));
+                               region(s, 3), s->flags, increments, st));
     HIP_TRY(fvl::finalize(region(s, 3), s->ops->metric_blocks(s->grid), s->scalars + 3 * fvl::NSLOT, st));
     int rc = sync_read(s);
     if (rc) return rc;
@@ -529,7 +540,8 @@ int metric_fast_step(torch_state* s, double time, double t_end, torch_step_info*
     double* rate = s->scalars + 3 * fvl::NSLOT;
     double* dtp = s->scalars + 3 * fvl::NSLOT + fvl::NSLOT - 1;
     HIP_TRY(clear_flags(s->flags, st));
-    HIP_TRY(s->ops->metric_cfl(s->grid, s->metric, s->U[A], nullptr, s->cfg.gamma, 0, region(s, 3), s->flags, st));
+    HIP_TRY(s->ops->metric_cfl(s->grid, s->metric, s->U[A], nullptr, s->cfg.gamma, 0, region(s, 3), s->flags,
+                               nullptr, st));
     HIP_TRY(fvl::finalize(region(s, 3), s->ops->metric_blocks(s->grid), rate, st));
     k_metric_dt<<<1, 64, 0, st>>>(rate, s->cfg.cfl, time, t_end, dtp);
     HIP_TRY(hipGetLastError());
@@ -1110,12 +1122,16 @@ int torch_state_metric_step(torch_state* s, double time, double t_end, torch_ste
         int old = A;
         if (reactive) {
             HIP_TRY(clear_flags(s->flags, s->ctx->stream));
-            if ((rc = run_react(s, A, B, dt / 2, -1, -1, false))) return rc;
+            // The FP32 form checks the committed state's rates raised by the
+            // heating's exact change (increments in slot C, free until the
+            // stage below), as its Cartesian sessions do.
+            void* incr = s->cfg.precision == TORCH_FP32 ? s->U[C] : nullptr;
+            if ((rc = run_react(s, A, B, dt / 2, -1, -1, false, incr))) return rc;
             if ((rc = sync_read(s))) return rc;
             if (host_flags(s) & fvl::F_REACT_BAD)
                 return fail(TORCH_INVALID, "Reaction input outside its domain (state, lambda or beta)");
             double allowed;
-            if ((rc = metric_dt(s, B, &allowed))) return rc;
+            if ((rc = incr ? metric_dt(s, A, &allowed, incr) : metric_dt(s, B, &allowed))) return rc;
             if (dt > allowed * (1 + 1e-12)) {
                 dt = py_min(dt / 2, allowed);
                 info->retries += 1;

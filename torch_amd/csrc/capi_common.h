@@ -81,7 +81,7 @@ struct Ops {
     // metric/face scheme
     hipError_t (*metric_stage)(const fvl::Grid&, const fvl::Metric&, const fvl::MStage&, hipStream_t);
     hipError_t (*metric_cfl)(const fvl::Grid&, const fvl::Metric&, const void*, const void*, double, int, double*,
-                             int*, hipStream_t);
+                             int*, const void*, hipStream_t);
     hipError_t (*metric_source)(const fvl::Grid&, const fvl::Metric&, const void*, const void*, const void*,
                                 const void*, double*, hipStream_t);
     hipError_t (*rusanov_faces)(int, int, long, const void*, const void*, const void*, const void*, void*,

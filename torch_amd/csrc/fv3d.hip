@@ -611,7 +611,9 @@ struct Kin {
 // the quotient) and evaluates exp(x) as 2^n 2^f with x log2(e) = n + f split
 // in double: 2^f (0 <= f < 1) by the hardware exp2 and 2^n exactly by
 // ldexp. Its relative error is about 1e-7, the level of the FP32 storage,
-// and does not grow with the exponent's magnitude.
+// and does not grow with the exponent's magnitude: enough for the rate
+// A exp(-Ea/RT), which enters multiplied. The advance factor exp(-rate t),
+// which enters as 1 - exp(-rate t), takes kexp_fine instead.
 #ifdef FV_FAST
 __device__ __forceinline__ double kdiv(double a, double b) {
     double r = __builtin_amdgcn_rcp(b);
@@ -628,6 +630,32 @@ __device__ __forceinline__ double kexp(double x) {
     const double n = floor(y);
     return ldexp((double)__builtin_amdgcn_exp2f((float)(y - n)), (int)n);
 }
+// exp(x) to about an ulp, for the advance factor exp(-rate t): the heat
+// release takes 1 - exp(-x), where kexp's ~1e-7 would swamp a weakly reacting
+// cell's share (exp(-1e-12) would come out as 1: no heat at all). x = n ln2
+// + r with |r| <= ln2/2 (ln2 as a 32-bit head, whose products with n are
+// exact, plus a tail), exp(r) by its Taylor series through r^10/10! (the rest
+// is below 2.3e-13 relative, and vanishes with r, so a tiny x keeps every
+// digit of 1 - exp(x)), scaled by 2^n.
+__device__ __forceinline__ double kexp_fine(double x) {
+    if (x != x) return x;
+    x = fmin(fmax(x, -746.0), 710.0);
+    const double n = rint(x * 1.4426950408889634);
+    double r = fma(-n, 0.6931471803691238, x);
+    r = fma(-n, 1.9082149292705877e-10, r);
+    double p = 2.755731922398589e-07;
+    p = fma(p, r, 2.7557319223985893e-06);
+    p = fma(p, r, 2.48015873015873e-05);
+    p = fma(p, r, 0.0001984126984126984);
+    p = fma(p, r, 0.001388888888888889);
+    p = fma(p, r, 0.008333333333333333);
+    p = fma(p, r, 0.041666666666666664);
+    p = fma(p, r, 0.16666666666666666);
+    p = fma(p, r, 0.5);
+    p = fma(p, r, 1.0);
+    p = fma(p, r, 1.0);
+    return ldexp(p, (int)n);
+}
 // 1/b refined like kdiv's reciprocal; kquot(a, b, ir) = a / b through it.
 __device__ __forceinline__ double krcp(double b) {
     double r = __builtin_amdgcn_rcp(b);
@@ -640,6 +668,7 @@ __device__ __forceinline__ double kquot(double a, double, double ir) { return a 
 #else
 __device__ __forceinline__ double kdiv(double a, double b) { return a / b; }
 __device__ __forceinline__ double kexp(double x) { return exp(x); }
+__device__ __forceinline__ double kexp_fine(double x) { return exp(x); }
 __device__ __forceinline__ double krcp(double b) { return 1.0 / b; }
 __device__ __forceinline__ double kquot(double a, double b, double) { return a / b; }
 #endif
@@ -675,10 +704,10 @@ __device__ __forceinline__ void advance(const Kin& k, double lam0, double beta0,
                                         double dur, double& lam, double& beta, double& e) {
     if (k.model == 1) {
         double active = fmax(0.0, dur - kdiv(beta0, fmax(rate, 1e-300)));
-        lam = lam0 * kexp(-5.0 * rate * active);
+        lam = lam0 * kexp_fine(-5.0 * rate * active);
         beta = fmin(fmax(beta0 - rate * dur, 0.0), 1.0);
     } else {
-        lam = lam0 * kexp(-rate * dur);
+        lam = lam0 * kexp_fine(-rate * dur);
         beta = 0.0;
     }
     e = n2 + k.q * n0 * (lam0 - lam);
@@ -785,14 +814,17 @@ __device__ __forceinline__ void react_cells_dim(const fvl::React& r, const Grid&
         if (!(u[0] > 1e-12 && (double)En - kquot(0.5 * s2, u[0], ir) > 1e-12 && isfinite((double)En)))
             bad |= fvl::F_STATE_BAD;
 #ifdef FV_FAST
-        // 2D and 1D attempts: the increments go to the CFL pass over the
-        // committed state (the same check as the 3D maxima below).
-        if (DIM == 2 && r.increments)
-            ((real*)r.increments)[i] = (real)heated_sound_speed_change(
-                u[0], n2 - kquot(0.5 * (u[1] * u[1]), u[0], ir), e - n2, lam0, lam, ir);
-        if (DIM == 1 && r.increments) {
-            ((real*)r.increments)[i] = (real)(e - n2);
-            ((real*)r.increments)[N + i] = (real)(lam - lam0);
+        // Attempts outside 3D Cartesian sessions: the increments go to the
+        // CFL pass over the committed state (the same check as the 3D maxima
+        // below).
+        if (r.increments) {
+            if (r.increment_kind == fvl::INC_HEAT) {
+                ((real*)r.increments)[i] = (real)(e - n2);
+                ((real*)r.increments)[N + i] = (real)(lam - lam0);
+            } else {
+                ((real*)r.increments)[i] = (real)heated_sound_speed_change(
+                    u[0], n2 - kquot(0.5 * (u[1] * u[1]), u[0], ir), e - n2, lam0, lam, ir);
+            }
         }
 #endif
         if (DIM == 3 && r.want_cfl) {

@@ -203,11 +203,14 @@ struct React {
     // accepted (the FP32 form then reports the committed cell's speeds plus
     // the chemistry's exact speed increment; see react_cells_dim).
     int attempt;
-    // Attempts of the FP32 form in 2D and 1D, for the CFL pass over the
-    // committed state: each cell's sound-speed increment (2D, one row of
-    // ncell values) or its energy and fraction increments (1D, two rows).
+    // Attempts of the FP32 form outside 3D Cartesian sessions, for the CFL
+    // pass over the committed state: each cell's sound-speed increment
+    // (INC_SPEED, one row of ncell values: 2D Cartesian and metric grids) or
+    // its energy and fraction increments (INC_HEAT, two rows: 1D Cartesian).
     void* increments;
+    int increment_kind;
 };
+enum IncrementKind { INC_SPEED = 1, INC_HEAT = 2 };
 
 // One instance of a batch (device memory). Partial slots of an instance:
 // [second-order | first-order | CFL pass | first chemistry half | second].
@@ -390,7 +393,7 @@ struct Launch {
                             hipStream_t st);                                              \
     hipError_t metric_cfl(const fvl::Grid& g, const fvl::Metric& m, const void* U,         \
                           const void* lam, double gamma, int reactive, double* partials,   \
-                          int* flags, hipStream_t st);                                     \
+                          int* flags, const void* increments, hipStream_t st);             \
     hipError_t metric_source(const fvl::Grid& g, const fvl::Metric& m, const void* start,  \
                              const void* old, const void* state, const void* hydro,        \
                              double* partials, hipStream_t st);                            \

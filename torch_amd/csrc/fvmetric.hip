@@ -595,10 +595,45 @@ __device__ __forceinline__ real face_rate(const Ctx& c, const fvl::Metric& m, co
     return nmax(a_lo, a_hi);
 }
 
-// CFL rate: sum over directions of max(A_lo s_lo, A_hi s_hi) / V.
+#ifdef FV_FAST
+// The exact change of face_rate, in double, when each entry's sound speed
+// rises by its increment `inc` (per cell; a periodic ghost takes its source
+// cell's, an inflow ghost none, any other ghost its edge cell's, exact for
+// walls and extrapolation and a first-order estimate for stagnation inlets
+// and pressure outlets). Zero when no entry's cell reacted.
+template <int DIM, int D>
+__device__ __forceinline__ double face_rate_change(const Ctx& c, const fvl::Metric& m, const fvl::Grid& g, long i,
+                                                   const int* idx, int& bad, const real* inc) {
+    constexpr int NF = DIM + 2;
+    const Line L = line_of<D>(g, i, idx);
+    const int at = idx[D];
+    auto raise = [&](int jj) -> double {
+        if (jj >= 1 && jj <= L.n) return (double)inc[L.base + (jj - 1) * L.stride];
+        const bool low = jj == 0;
+        const MFace& f = m.face[2 * D + (low ? 0 : 1)];
+        if (f.type == FACE_INFLOW) return 0.0;
+        if (f.type == FACE_PERIODIC) return (double)inc[L.base + (low ? L.n - 1 : 0) * L.stride];
+        return (double)inc[L.base + (low ? 0 : L.n - 1) * L.stride];
+    };
+    auto speed = [&](int jj) {
+        const Pad<NF> e = padded<NF, D>(c, m, L, jj, bad);
+        return (double)(fabs(e.p[D + 1]) + sqrt((e.g * e.p[NF - 1]) / e.p[0]));
+    };
+    auto mx = [](double a, double b) { return (a > b || a != a) ? a : b; };
+    const double s0 = speed(at), s1 = speed(at + 1), s2 = speed(at + 2);
+    const double h0 = s0 + raise(at), h1 = s1 + raise(at + 1), h2 = s2 + raise(at + 2);
+    const double alo = area_at<DIM, D>(m, idx, at), ahi = area_at<DIM, D>(m, idx, at + 1);
+    return mx(alo * mx(h0, h1), ahi * mx(h1, h2)) - mx(alo * mx(s0, s1), ahi * mx(s1, s2));
+}
+#endif
+
+// CFL rate: sum over directions of max(A_lo s_lo, A_hi s_hi) / V. With
+// `inc` (an FP32 reactive attempt over the committed state): that rate
+// raised by the heating's exact change (face_rate_change).
 template <int DIM>
 __global__ void __launch_bounds__(MT) k_metric_cfl(fvl::Grid g, fvl::Metric m, const real* U, const real* lam,
-                                                   double gamma, int reactive, double* partials, int* flags) {
+                                                   double gamma, int reactive, double* partials, int* flags,
+                                                   const real* inc) {
     const long N = (long)g.nx * g.ny * g.nz;
     const long i = blockIdx.x * (long)MT + threadIdx.x;
     double rate = 0;
@@ -621,6 +656,14 @@ __global__ void __launch_bounds__(MT) k_metric_cfl(fvl::Grid g, fvl::Metric m, c
         if (DIM >= 2) r = r + face_rate<DIM, (DIM >= 2 ? 1 : 0)>(c, m, g, i, idx, bad) / V;
         if (DIM >= 3) r = r + face_rate<DIM, (DIM >= 3 ? 2 : 0)>(c, m, g, i, idx, bad) / V;
         rate = (double)r;
+#ifdef FV_FAST
+        if (inc) {
+            double d = face_rate_change<DIM, 0>(c, m, g, i, idx, bad, inc);
+            if (DIM >= 2) d += face_rate_change<DIM, (DIM >= 2 ? 1 : 0)>(c, m, g, i, idx, bad, inc);
+            if (DIM >= 3) d += face_rate_change<DIM, (DIM >= 3 ? 2 : 0)>(c, m, g, i, idx, bad, inc);
+            rate = rate + d / volume_at<DIM>(m, idx);
+        }
+#endif
     }
     if (bad) atomicOr(flags, bad);
     block_max(rate, partials);
@@ -1063,15 +1106,19 @@ hipError_t metric_stage(const fvl::Grid& g, const fvl::Metric& m, const fvl::MSt
 }
 
 hipError_t metric_cfl(const fvl::Grid& g, const fvl::Metric& m, const void* U, const void* lam, double gamma,
-                      int reactive, double* partials, int* flags, hipStream_t st) {
+                      int reactive, double* partials, int* flags, const void* increments, hipStream_t st) {
+#ifndef FV_FAST
+    if (increments) return hipErrorInvalidValue;
+#endif
     const int blocks = metric_blocks(g);
     const real* u = (const real*)U;
     const real* l = (const real*)lam;
+    const real* inc = (const real*)increments;
     hipError_t e = metric_prims(g, m, U, lam, nullptr, 0, gamma, reactive, st);
     if (e != hipSuccess) return e;
-    if (g.dim == 1) k_metric_cfl<1><<<blocks, MT, 0, st>>>(g, m, u, l, gamma, reactive, partials, flags);
-    else if (g.dim == 2) k_metric_cfl<2><<<blocks, MT, 0, st>>>(g, m, u, l, gamma, reactive, partials, flags);
-    else k_metric_cfl<3><<<blocks, MT, 0, st>>>(g, m, u, l, gamma, reactive, partials, flags);
+    if (g.dim == 1) k_metric_cfl<1><<<blocks, MT, 0, st>>>(g, m, u, l, gamma, reactive, partials, flags, inc);
+    else if (g.dim == 2) k_metric_cfl<2><<<blocks, MT, 0, st>>>(g, m, u, l, gamma, reactive, partials, flags, inc);
+    else k_metric_cfl<3><<<blocks, MT, 0, st>>>(g, m, u, l, gamma, reactive, partials, flags, inc);
     return hipGetLastError();
 }
 
