@@ -639,6 +639,10 @@ __device__ __forceinline__ double kexp(double x) {
 // digit of 1 - exp(x)), scaled by 2^n.
 __device__ __forceinline__ double kexp_fine(double x) {
     if (x != x) return x;
+    // Weakly reacting cells, most of a grid: |x| < 2^-10 takes the series
+    // through x^4/24 directly (the rest is below 1e-17), so a wave of them
+    // skips the reduction and the long polynomial.
+    if (fabs(x) < 0x1p-10) return fma(fma(fma(fma(x, 1.0 / 24.0, 1.0 / 6.0), x, 0.5), x, 1.0), x, 1.0);
     x = fmin(fmax(x, -746.0), 710.0);
     const double n = rint(x * 1.4426950408889634);
     double r = fma(-n, 0.6931471803691238, x);
