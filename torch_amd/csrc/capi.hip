@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "capi_common.h"
@@ -77,15 +78,50 @@ void make_grid(const torch_config& c, fvl::Grid* out, double* volume) {
     }
 }
 
+// Pinned staging of FP32 uploads: the host converts the float64 input into
+// it (on up to 8 threads for large arrays) and one DMA moves it. Grown on
+// demand and kept for the process; every upload completes before the lock
+// is released, so the buffer is free again when the next one takes it.
+namespace {
+std::mutex g_stage_mu;
+float* g_stage = nullptr;
+long g_stage_n = 0;
+
+void convert_to_float(const double* src, float* dst, long n) {
+    const long per = 1L << 20;
+    const int threads = (int)std::min<long>(8, (n + per - 1) / per);
+    auto part = [&](int t) {
+        const long lo = n * t / threads, hi = n * (t + 1) / threads;
+        for (long i = lo; i < hi; ++i) dst[i] = (float)src[i];
+    };
+    if (threads <= 1) {
+        part(0);
+        return;
+    }
+    std::vector<std::thread> pool;
+    for (int t = 1; t < threads; ++t) pool.emplace_back(part, t);
+    part(0);
+    for (auto& th : pool) th.join();
+}
+}  // namespace
+
 int upload_as(const Ops* ops, hipStream_t st, void* dst, const double* src, long n) {
+    if (n <= 0) return TORCH_OK;
     if (ops->elem == sizeof(double)) {
         HIP_TRY(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyHostToDevice, st));
-    } else {
-        std::vector<float> tmp(n);
-        for (long i = 0; i < n; ++i) tmp[i] = (float)src[i];
-        HIP_TRY(hipMemcpyAsync(dst, tmp.data(), n * sizeof(float), hipMemcpyHostToDevice, st));
         HIP_TRY(hipStreamSynchronize(st));
+        return TORCH_OK;
     }
+    std::lock_guard<std::mutex> lock(g_stage_mu);
+    if (n > g_stage_n) {
+        if (g_stage) (void)hipHostFree(g_stage);
+        g_stage = nullptr;
+        g_stage_n = 0;
+        HIP_TRY(hipHostMalloc((void**)&g_stage, (size_t)n * sizeof(float), hipHostMallocDefault));
+        g_stage_n = n;
+    }
+    convert_to_float(src, g_stage, n);
+    HIP_TRY(hipMemcpyAsync(dst, g_stage, n * sizeof(float), hipMemcpyHostToDevice, st));
     HIP_TRY(hipStreamSynchronize(st));
     return TORCH_OK;
 }

@@ -88,6 +88,7 @@ class Server:
         self.keep_results = keep_results
         self.rounds = 0
         self.device_ms = 0.0
+        self.timing = {}
 
     def payload(self, prim, opts):
         """Host-side admission data of a request: (config, state, lam, beta, t_end)."""
@@ -130,6 +131,11 @@ class Server:
         """
         pending = sorted(requests, key=lambda r: r.arrival)
         built = {id(r): self.payload(*r.build()) for r in pending} if prebuild else {}
+        # Host time of this call by phase (seconds): admissions, rounds right
+        # after a membership change (they re-capture the step graph) and the
+        # others, retirements (progress reads, downloads, removals).
+        self.timing = dict(admit_s=0.0, changed_rounds=0, changed_s=0.0, steady_rounds=0, steady_s=0.0,
+                           retire_s=0.0)
         ready = queue.Queue()
         start = clock()
 
@@ -145,6 +151,7 @@ class Server:
         worker.start()
         waiting, active = [], {}
         arrivals_done = False
+        retired = False
         while not arrivals_done or waiting or active:
             # Collect prepared arrivals; wait for one only when idle.
             block = not active and not waiting and not arrivals_done
@@ -162,6 +169,8 @@ class Server:
             if self.policy == "largest":
                 waiting.sort(key=lambda item: (-(item[0].cells or 0), item[0].arrival))
             still, blocked = [], False
+            t_admit = clock()
+            changed = retired
             for req, payload in waiting:
                 if blocked:
                     still.append((req, payload))
@@ -174,6 +183,8 @@ class Server:
                 else:
                     req.admitted = clock() - start
                     active[ident] = req
+                    changed = True
+            self.timing["admit_s"] += clock() - t_admit
             waiting = still
             if waiting and not active:
                 raise RuntimeError("a request does not fit the HBM budget on an empty device")
@@ -181,9 +192,15 @@ class Server:
                 continue
             t0 = clock()
             self.batch.run(self.round_steps)
-            self.device_ms += 1e3 * (clock() - t0)
+            ran = clock() - t0
+            self.device_ms += 1e3 * ran
             self.rounds += 1
+            kind = "changed" if changed else "steady"
+            self.timing[kind + "_rounds"] += 1
+            self.timing[kind + "_s"] += ran
             now = clock() - start
+            t_retire = clock()
+            retired = False
             for ident in list(active):
                 info = self.batch.info(ident)
                 if info["active"]:
@@ -196,6 +213,8 @@ class Server:
                 if self.keep_results:
                     req.result = self.batch._batch.download(ident)
                 self.batch.remove(ident)
+                retired = True
+            self.timing["retire_s"] += clock() - t_retire
         worker.join()
         return pending
 
