@@ -196,6 +196,7 @@ def run_served(seed, count, t_run, budget_gib, round_steps, precision="fp32", op
     server = Server(precision=precision, hbm_budget=budget, round_steps=round_steps, policy=policy)
     closed = summary(server.serve([ignition_request(s, 0.0) for s in specs], prebuild=True))
     phases = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in server.timing.items()}
+    phases["graph_instantiated"], phases["graph_updated"] = server.batch._batch.graph_counts()
     server.close()
     capacity = closed["completed"] / closed["makespan_s"]
     print(f"# served closed loop: {closed['makespan_s']:.2f} s for {count} requests, "
@@ -204,7 +205,8 @@ def run_served(seed, count, t_run, budget_gib, round_steps, precision="fp32", op
     print(f"# served closed loop phases: admissions {phases['admit_s']:.2f} s, {phases['changed_rounds']} rounds "
           f"after a membership change {phases['changed_s']:.2f} s ({per('changed'):.1f} ms each), "
           f"{phases['steady_rounds']} steady rounds {phases['steady_s']:.2f} s ({per('steady'):.1f} ms each), "
-          f"retirements {phases['retire_s']:.2f} s", file=sys.stderr, flush=True)
+          f"retirements {phases['retire_s']:.2f} s; step graph instantiated {phases['graph_instantiated']} times, "
+          f"updated in place {phases['graph_updated']} times", file=sys.stderr, flush=True)
     out["closed_loop"] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in closed.items()}
     out["closed_loop"]["host_phases"] = phases
     out["capacity_per_s"] = round(capacity, 3)

@@ -322,6 +322,10 @@ int torch_batch_time_pass(torch_batch* batch, int reps, double* ms);
 /* Measurement hook: HIP events around the graph replays of each run. Returns
  * the accumulated milliseconds and steps since the last call, then resets. */
 int torch_batch_profile(torch_batch* batch, int enable, double* ms, long* steps);
+/* Measurement hook: how often the step graph was instantiated and how often
+ * an existing one was updated in place (same work lists) since the batch
+ * was created. */
+int torch_batch_graph_counts(torch_batch* batch, long* instantiated, long* updated);
 
 #ifdef __cplusplus
 }

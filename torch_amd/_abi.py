@@ -31,7 +31,8 @@ EXPORTS = (
     "torch_state_totals", "torch_state_profile", "torch_hllc_faces", "torch_hllc_faces_static",
     "torch_batch_footprint", "torch_batch_create", "torch_batch_destroy", "torch_batch_add",
     "torch_batch_remove", "torch_batch_seed", "torch_batch_run", "torch_batch_info", "torch_batch_download",
-    "torch_batch_tune", "torch_batch_set_tiles", "torch_batch_profile", "torch_batch_time_pass",
+    "torch_batch_tune", "torch_batch_set_tiles", "torch_batch_profile", "torch_batch_graph_counts",
+    "torch_batch_time_pass",
     "torch_state_set_metric", "torch_state_metric_dt", "torch_state_metric_rhs", "torch_state_metric_advance",
     "torch_state_metric_step", "torch_rusanov_faces", "torch_state_wave_maxima",
     "torch_amr_create", "torch_amr_destroy", "torch_amr_set_mesh", "torch_amr_upload", "torch_amr_download",
@@ -161,6 +162,8 @@ def load():
                                                     _dp]),
                 "torch_batch_set_tiles": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
                 "torch_batch_profile": (ctypes.c_int, [vp, ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_long)]),
+                "torch_batch_graph_counts": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_long),
+                                                            ctypes.POINTER(ctypes.c_long)]),
                 "torch_batch_time_pass": (ctypes.c_int, [vp, ctypes.c_int, _dp]),
                 "torch_state_set_metric": (ctypes.c_int, [vp, ctypes.c_int, _dp, ctypes.c_long,
                                                           ctypes.POINTER(Face)]),
@@ -590,6 +593,12 @@ class DeviceBatch:
         ms, n = ctypes.c_double(), ctypes.c_long()
         check(load().torch_batch_profile(self.handle, int(enable), ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
+
+    def graph_counts(self):
+        """(instantiations, in-place updates) of the step graph so far."""
+        built, updated = ctypes.c_long(), ctypes.c_long()
+        check(load().torch_batch_graph_counts(self.handle, ctypes.byref(built), ctypes.byref(updated)))
+        return built.value, updated.value
 
     def close(self):
         if getattr(self, "handle", None):

@@ -94,7 +94,14 @@ struct torch_batch {
     Table l3, l2, l1, c1;     // first-order tables per dimension, 1D CFL
     Table p3, p2, p1;         // first-dt tables over pitems
     int pseg[4] = {0, 0, 0, 0};
+    // The step graph. A membership or tile change marks it stale (graph_ok
+    // false) but keeps the executable: the next capture first tries to
+    // update it in place, which is far cheaper than instantiating a new one
+    // when the work lists are the same. A stale executable may name freed
+    // buffers; it is never launched before that update or its destruction.
     hipGraphExec_t graph = nullptr;
+    bool graph_ok = false;
+    long graph_builds = 0, graph_updates = 0;  // instantiations, in-place updates
     // Side streams and events that fork the captured step into parallel
     // branches (work lists of different instances); capture only.
     hipStream_t side[kSide] = {};
@@ -130,9 +137,12 @@ Inst* find(torch_batch* b, int id) {
     return nullptr;
 }
 
-void drop_graph(torch_batch* b) {
+void drop_graph(torch_batch* b) { b->graph_ok = false; }
+
+void free_graph(torch_batch* b) {
     if (b->graph) (void)hipGraphExecDestroy(b->graph);
     b->graph = nullptr;
+    b->graph_ok = false;
 }
 
 // Block counts of each pass for the current tile choice.
@@ -352,12 +362,28 @@ int capture(torch_batch* b) {
         if (g) (void)hipGraphDestroy(g);
         return fail(TORCH_HIP_ERROR, std::string("graph capture: ") + hipGetErrorString(e ? e : e2));
     }
+    if (b->graph) {
+        // Same work lists (nodes and edges): the kernels' grids and
+        // arguments are rewritten in place. Otherwise a new executable.
+        hipGraphNode_t refused = nullptr;
+        hipGraphExecUpdateResult how = hipGraphExecUpdateError;
+        if (hipGraphExecUpdate(b->graph, g, &refused, &how) == hipSuccess && how == hipGraphExecUpdateSuccess) {
+            (void)hipGraphDestroy(g);
+            b->graph_ok = true;
+            b->graph_updates += 1;
+            return TORCH_OK;
+        }
+        (void)hipGetLastError();
+        free_graph(b);
+    }
     e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (e != hipSuccess) {
         b->graph = nullptr;
         return fail(TORCH_HIP_ERROR, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
     }
+    b->graph_ok = true;
+    b->graph_builds += 1;
     return TORCH_OK;
 }
 
@@ -641,7 +667,7 @@ int torch_batch_destroy(torch_batch* b) {
     std::lock_guard<std::mutex> lock(b->ctx->mu);
     (void)hipSetDevice(b->ctx->device);
     (void)hipStreamSynchronize(b->ctx->stream);
-    drop_graph(b);
+    free_graph(b);
     for (Inst* p : b->inst) free_inst(p);
     (void)hipFree(b->items);
     (void)hipFree(b->ctl);
@@ -770,7 +796,7 @@ int torch_batch_run(torch_batch* b, int steps) {
     HIP_TRY(hipSetDevice(b->ctx->device));
     int rc;
     if (b->dirty && (rc = rebuild(b))) return rc;
-    if (!b->graph || steps == 0) return TORCH_OK;
+    if (!b->graph_ok || steps == 0) return TORCH_OK;
     hipStream_t st = b->ctx->stream;
     if (b->profile) HIP_TRY(hipEventRecord(b->ev0, st));
     for (int k = 0; k < steps; ++k) HIP_TRY(hipGraphLaunch(b->graph, st));
@@ -936,6 +962,14 @@ int torch_batch_profile(torch_batch* b, int enable, double* ms, long* steps) {
     b->prof_ms = 0;
     b->prof_steps = 0;
     b->profile = enable != 0;
+    return TORCH_OK;
+}
+
+int torch_batch_graph_counts(torch_batch* b, long* instantiated, long* updated) {
+    if (!b) return fail(TORCH_INVALID, "null argument");
+    std::lock_guard<std::mutex> lock(b->ctx->mu);
+    if (instantiated) *instantiated = b->graph_builds;
+    if (updated) *updated = b->graph_updates;
     return TORCH_OK;
 }
 
