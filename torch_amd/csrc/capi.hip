@@ -98,8 +98,15 @@ void convert_to_float(const double* src, float* dst, long n) {
         part(0);
         return;
     }
+    // A thread that cannot be started (a process limit) leaves its part to
+    // this one: no exception crosses the C ABI.
     std::vector<std::thread> pool;
-    for (int t = 1; t < threads; ++t) pool.emplace_back(part, t);
+    int started = 1;
+    try {
+        for (; started < threads; ++started) pool.emplace_back(part, started);
+    } catch (...) {
+    }
+    for (int t = started; t < threads; ++t) part(t);
     part(0);
     for (auto& th : pool) th.join();
 }
