@@ -78,21 +78,24 @@ void make_grid(const torch_config& c, fvl::Grid* out, double* volume) {
     }
 }
 
-// Pinned staging of FP32 uploads: the host converts the float64 input into
-// it (on up to 8 threads for large arrays) and one DMA moves it. Grown on
-// demand and kept for the process; every upload completes before the lock
-// is released, so the buffer is free again when the next one takes it.
+// Pinned staging of FP32 transfers: an upload converts the float64 input
+// into it (on up to 8 threads for large arrays) and one DMA moves it; a
+// download is one DMA into it, then the widening. Grown on demand and kept
+// for the process; every transfer completes before the lock is released,
+// so the buffer is free again when the next one takes it.
 namespace {
 std::mutex g_stage_mu;
 float* g_stage = nullptr;
 long g_stage_n = 0;
 
-void convert_to_float(const double* src, float* dst, long n) {
+// Elementwise conversion of n values on up to 8 threads (one per 2^20).
+template <typename From, typename To>
+void convert(const From* src, To* dst, long n) {
     const long per = 1L << 20;
     const int threads = (int)std::min<long>(8, (n + per - 1) / per);
     auto part = [&](int t) {
         const long lo = n * t / threads, hi = n * (t + 1) / threads;
-        for (long i = lo; i < hi; ++i) dst[i] = (float)src[i];
+        for (long i = lo; i < hi; ++i) dst[i] = (To)src[i];
     };
     if (threads <= 1) {
         part(0);
@@ -110,6 +113,17 @@ void convert_to_float(const double* src, float* dst, long n) {
     part(0);
     for (auto& th : pool) th.join();
 }
+
+// The staging buffer holds at least n floats (the caller holds g_stage_mu).
+int stage_at_least(long n) {
+    if (n <= g_stage_n) return TORCH_OK;
+    if (g_stage) (void)hipHostFree(g_stage);
+    g_stage = nullptr;
+    g_stage_n = 0;
+    HIP_TRY(hipHostMalloc((void**)&g_stage, (size_t)n * sizeof(float), hipHostMallocDefault));
+    g_stage_n = n;
+    return TORCH_OK;
+}
 }  // namespace
 
 int upload_as(const Ops* ops, hipStream_t st, void* dst, const double* src, long n) {
@@ -120,29 +134,27 @@ int upload_as(const Ops* ops, hipStream_t st, void* dst, const double* src, long
         return TORCH_OK;
     }
     std::lock_guard<std::mutex> lock(g_stage_mu);
-    if (n > g_stage_n) {
-        if (g_stage) (void)hipHostFree(g_stage);
-        g_stage = nullptr;
-        g_stage_n = 0;
-        HIP_TRY(hipHostMalloc((void**)&g_stage, (size_t)n * sizeof(float), hipHostMallocDefault));
-        g_stage_n = n;
-    }
-    convert_to_float(src, g_stage, n);
+    int rc = stage_at_least(n);
+    if (rc) return rc;
+    convert(src, g_stage, n);
     HIP_TRY(hipMemcpyAsync(dst, g_stage, n * sizeof(float), hipMemcpyHostToDevice, st));
     HIP_TRY(hipStreamSynchronize(st));
     return TORCH_OK;
 }
 
 int download_as(const Ops* ops, hipStream_t st, double* dst, const void* src, long n) {
+    if (n <= 0) return TORCH_OK;
     if (ops->elem == sizeof(double)) {
         HIP_TRY(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
-    } else {
-        std::vector<float> tmp(n);
-        HIP_TRY(hipMemcpyAsync(tmp.data(), src, n * sizeof(float), hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipStreamSynchronize(st));
-        for (long i = 0; i < n; ++i) dst[i] = tmp[i];
+        return TORCH_OK;
     }
+    std::lock_guard<std::mutex> lock(g_stage_mu);
+    int rc = stage_at_least(n);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(g_stage, src, n * sizeof(float), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    convert(g_stage, dst, n);
     return TORCH_OK;
 }
 
