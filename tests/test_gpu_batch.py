@@ -325,3 +325,26 @@ def test_reactive_batch_commit_and_continue():
     assert np.array_equal(sim.state, ref.state) and np.array_equal(sim.lam, ref.lam)
     assert np.array_equal(sim.source_integral, ref.source_integral)
     batch.close()
+
+
+def test_membership_changes_update_the_graph_in_place():
+    """Retiring an instance and admitting another with the same work lists
+    (3D, Superbee) rewrites the captured step graph in place instead of
+    instantiating it again (torch_batch_graph_counts: one instantiation, one
+    update), and every instance still equals its session stepped alone,
+    bitwise."""
+    from torch_amd.torchBatch import Batch
+    a, b = _sessions("fp32", seed=31, specs=[SPECS[7], SPECS[10]])
+    (c,) = _sessions("fp32", seed=32, specs=[SPECS[7]])
+    ref_a, ref_b, ref_c = _alone(a, 6), _alone(b, 3), _alone(c, 3)
+    batch = Batch(precision="fp32")
+    ia, ib = batch.add(a, max_steps=6), batch.add(b, max_steps=3)
+    batch.run(3)
+    _check_same(batch, ib, ref_b)
+    batch.remove(ib)
+    ic = batch.add(c, max_steps=3)
+    batch.run(3)
+    _check_same(batch, ia, ref_a)
+    _check_same(batch, ic, ref_c)
+    assert batch._batch.graph_counts() == (1, 1)
+    batch.close()
