@@ -339,7 +339,7 @@ __device__ __forceinline__ void step3d_tile(const Step& s, const G3& g, int tbx,
         real lq[2] = {0, 0};
         for (int j = 0; j < NQ; ++j) lq[j] = shfl_up1(qc[j]);
         if (plane_out && xface) {
-            hllc_cart5(lxr, xl, lg, gc, 0, Fx);
+            hllc_cart5<true>(lxr, xl, lg, gc, 0, Fx);
             for (int j = 0; j < NQ; ++j) if (j < nq) sfx[j] = Fx[0] * (Fx[0] >= (real)0 ? lq[j] : qc[j]);
         }
         // y face y-1/2 from the row below.
@@ -348,14 +348,14 @@ __device__ __forceinline__ void step3d_tile(const Step& s, const G3& g, int tbx,
             real below[5];
 #pragma unroll
             for (int c = 0; c < 5; ++c) below[c] = sY[c][row - 1][lane];
-            hllc_cart5(below, yl, sY[5][row - 1][lane], gc, 1, Fy);
+            hllc_cart5<true>(below, yl, sY[5][row - 1][lane], gc, 1, Fy);
             for (int j = 0; j < NQ; ++j) if (j < nq) sfy[j] = Fy[0] * (Fy[0] >= (real)0 ? sY[6 + j][row - 1][lane] : qc[j]);
         }
         // z face k-1/2 from the carried state of plane k-1.
         real Fz[5] = {0, 0, 0, 0, 0}, sfz[2] = {0, 0};
         const bool zface = owns && k >= z0;
         if (zface) {
-            hllc_cart5(zr, zl, gp, gc, 2, Fz);
+            hllc_cart5<true>(zr, zl, gp, gc, 2, Fz);
             for (int j = 0; j < NQ; ++j) if (j < nq) sfz[j] = Fz[0] * (Fz[0] >= (real)0 ? qp[j] : qc[j]);
         }
 #pragma unroll

@@ -251,7 +251,7 @@ __device__ __forceinline__ bool physical4(const real* s) {
 // rows [rho, m_n, m_t..., E]. Region choice reproduces the reference's
 // ordered masks L, *L, *R, R (last assignment wins); a face that matches
 // none (NaN speeds) yields NaN.
-template <int NV>
+template <int NV, bool CHECKED = false>
 __device__ __forceinline__ void hllc(const real* ul, const real* ur, real gl, real gr, real* out) {
     const int E = NV - 1;
     real gbar = (real)0.5 * (gl + gr);
@@ -286,8 +286,19 @@ __device__ __forceinline__ void hllc(const real* ul, const real* ur, real gl, re
     real h_roe = (sql * hl + sqr * hr) * inv;
     real ke_roe = (real)0.5 * (u_roe * u_roe + kv);
     real a_roe = rsqrt_(rmax((gbar - (real)1) * (h_roe - ke_roe), (real)1e-12));
-    real s_l = nmin(unl - al, u_roe - a_roe);
-    real s_r = nmax(unr + ar, u_roe + a_roe);
+    real s_l, s_r;
+    if (CHECKED && sizeof(real) == sizeof(float)) {
+        // CHECKED: both states passed physical5 (finite, density and internal
+        // energy above 1e-12) or the step is flagged and these fluxes are
+        // replaced by the first-order pass. The speeds are then never NaN
+        // (at worst +-inf in FP32), so the FP32 form takes the plain
+        // hardware min/max for the NaN-propagating ones.
+        s_l = fmin(unl - al, u_roe - a_roe);
+        s_r = fmax(unr + ar, u_roe + a_roe);
+    } else {
+        s_l = nmin(unl - al, u_roe - a_roe);
+        s_r = nmax(unr + ar, u_roe + a_roe);
+    }
     real den = guard14(rl * (s_l - unl) - rr * (s_r - unr));
     real s_star = dv(pr - pl + (rl * unl) * (s_l - unl) - (rr * unr) * (s_r - unr), den);
 
@@ -359,6 +370,7 @@ __device__ __forceinline__ void hllc_equal(const real* u, real g, real* out) {
 // axis `dir` (0 x, 1 y, 2 z): rotate to [rho, m_n, m_t(asc), E] and back
 // (torchFlux.py:36-39, 75-76). Faces between equal states (uniform regions)
 // take the solver's own flux of that state without the wave-speed work.
+template <bool CHECKED = false>
 __device__ __forceinline__ void hllc_cart5(const real* l, const real* r, real gl, real gr, int dir, real* f) {
     real a[5], b[5], o[5];
     int t0 = dir == 0 ? 2 : 1, t1 = dir == 2 ? 2 : 3;
@@ -367,7 +379,7 @@ __device__ __forceinline__ void hllc_cart5(const real* l, const real* r, real gl
     const bool equal = (a[0] == b[0]) & (a[1] == b[1]) & (a[2] == b[2]) & (a[3] == b[3]) & (a[4] == b[4]) &
                        (gl == gr);
     if (equal) hllc_equal<5>(a, gl, o);
-    else hllc<5>(a, b, gl, gr, o);
+    else hllc<5, CHECKED>(a, b, gl, gr, o);
     f[0] = o[0]; f[dir + 1] = o[1]; f[t0] = o[2]; f[t1] = o[3]; f[4] = o[4];
 }
 
