@@ -98,7 +98,11 @@ int torch_state_dt(torch_state* state, double* dt_limit);
  * max(|u_m| + a) per axis; 1D [max |S| of the wave-speed estimate,
  * max(|u| + a), 0], whose second entry gives hydro_dt_1d = cfl dx / max
  * (torchSolver.py:1049-1051) for the legacy reactive driver's CFL test
- * (torchSolver.py:1319-1331). */
+ * (torchSolver.py:1319-1331). Right after torch_state_react on a 1D or 2D
+ * Cartesian state of the FP32 form: the maxima of the state the half-step
+ * started from raised by the exact change its heating makes, which the
+ * FP32 storage rounds away when it is below half an ulp (the FP64 method's
+ * test then takes the same dt halvings). */
 int torch_state_wave_maxima(torch_state* state, double* maxima);
 
 /* Simulation.step (torchSession.py:374-522): CFL dt clipped to t_end - time,

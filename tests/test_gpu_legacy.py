@@ -79,3 +79,29 @@ def test_inert_sessions_match_legacy_drivers():
         else:
             r = run_torch_solver_3d(nx=8, ny=6, nz=4, t_end=0.02, cfl=0.4)
         np.testing.assert_allclose(s.state, r["u_final"], atol=1e-14)
+
+
+@pytest.mark.parametrize("model", ["arrhenius", "znd"])
+def test_run_simulation_wqr_fp32_takes_the_fp64_steps(model):
+    """The FP32 form of the reactive 1D driver (torchSolver.py:1319-1331)
+    repeats the FP64 method's dt halvings: a 50 atm / 0.1 atm tube over 200
+    cells to t = 1e-4 (about 1,150-1,400 steps). Heating below half an FP32
+    ulp of the energy raises |u| + a by more than the 1e-12 slack in FP64
+    only; measured on the committed state it is seen in both. Same step
+    count, each dt within 1% (the halved dt is the recomputed limit of a
+    state that carries FP32 rounding), final density within 5e-6 relative
+    L1 (measured 1.8e-6 and 1.2e-6; measured on the heated FP32 state, the
+    test skipped those halvings, took about 10% fewer steps and ended at
+    1.3e-5 to 1.9e-5)."""
+    from torch_amd import run_simulation_wqr
+    from torch_amd.torchExecution import DeviceExecution
+
+    kw = dict(l=1.0, n=200, rho_l=1.0, u_l=0.0, p_l=50 * 101325.0, rho_r=0.125, u_r=0.0, p_r=0.1 * 101325.0,
+              c_cfl=0.4, t_corte=1e-4, kinetic_model=model)
+    u64, t64, _ = run_simulation_wqr(**kw, execution=DeviceExecution(precision="fp64"))
+    u32, t32, _ = run_simulation_wqr(**kw, execution=DeviceExecution(precision="fp32"))
+    assert len(t32) == len(t64) > 1000
+    d64, d32 = np.diff(t64), np.diff(t32)
+    assert np.max(np.abs(d32 / d64 - 1.0)) <= 1e-2
+    rho64, rho32 = u64[0, :, -1], u32[0, :, -1]
+    assert np.abs(rho32 - rho64).sum() / np.abs(rho64).sum() <= 5e-6

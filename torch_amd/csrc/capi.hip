@@ -249,6 +249,16 @@ struct torch_state {
     double* host = nullptr;      // pinned readback: 4 x NSLOT doubles + flags
     bool maxima_valid = false;
     double maxima[3] = {0, 0, 0};
+    // Bumped by every change of the committed state. A 1D or 2D chemistry
+    // half-step of the FP32 form (torch_state_react) records the slot it
+    // started from and the buffer of its per-cell increments; until the
+    // next change, torch_state_wave_maxima reports that slot's maxima raised
+    // by the heating's exact change (the FP64 method's view of the heated
+    // state, whose sub-ulp heating the FP32 storage drops).
+    long version = 0;
+    int attempt_slot = -1;
+    long attempt_version = -1;
+    void* attempt_inc = nullptr;
     double volume = 0;
     bool profile = false, prof_pending = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -376,7 +386,7 @@ hipError_t run_cfl(torch_state* s, int u, int l, const void* increments = nullpt
 
 // Chemistry half-step slot `in` -> slot `out` (state, lambda, beta).
 int run_react(torch_state* s, int in, int out, double dt, int start, int old, bool want_cfl,
-              void* speed_increments = nullptr) {
+              void* increments = nullptr, int increment_kind = fvl::INC_SPEED) {
     fvl::React r;
     memset(&r, 0, sizeof(r));
     r.U = s->U[in];
@@ -399,14 +409,14 @@ int run_react(torch_state* s, int in, int out, double dt, int start, int old, bo
     // A 2D or 1D attempt of the FP32 form leaves its per-cell increments in
     // state slot (in + 3) mod 4 (the second half-step's output, unused until
     // then) and checks the committed state's maxima raised by them; a metric
-    // attempt passes its own buffer for the sound-speed increments.
+    // attempt or a standalone half-step passes its own buffer.
     if (r.attempt && s->cfg.dim != 3 && s->cfg.precision == TORCH_FP32) {
         r.increments = s->U[(in + 3) % 4];
         r.increment_kind = s->cfg.dim == 1 ? fvl::INC_HEAT : fvl::INC_SPEED;
     }
-    if (speed_increments) {
-        r.increments = speed_increments;
-        r.increment_kind = fvl::INC_SPEED;
+    if (increments) {
+        r.increments = increments;
+        r.increment_kind = increment_kind;
     }
     if (!r.U || !r.Uout || !r.lam || !r.lamout || (s->cfg.kinetic_model == 1 && (!r.beta || !r.betaout)) ||
         (start >= 0 && (!r.Ustart || !r.Uold)))
@@ -632,6 +642,7 @@ int metric_fast_step(torch_state* s, double time, double t_end, torch_step_info*
     }
     s->cur = C;
     s->maxima_valid = false;
+    s->version += 1;
     return TORCH_OK;
 }
 
@@ -835,6 +846,7 @@ int torch_state_upload(torch_state* s, const double* U, const double* lam, const
         }
     }
     s->maxima_valid = false;
+    s->version += 1;
     return rc;
 }
 
@@ -865,6 +877,17 @@ int torch_state_wave_maxima(torch_state* s, double* maxima) {
     if (!s || !maxima) return fail(TORCH_INVALID, "null argument");
     std::lock_guard<std::mutex> lock(s->ctx->mu);
     HIP_TRY(hipSetDevice(s->ctx->device));
+    if (s->attempt_slot >= 0 && s->attempt_version == s->version) {
+        // Right after an FP32 half-step: the maxima of the state it started
+        // from raised by the heating's exact change. Not cached: the maxima
+        // that set the next dt stay those of the stored state.
+        HIP_TRY(clear_flags(s->flags, s->ctx->stream));
+        HIP_TRY(run_cfl(s, s->attempt_slot, s->attempt_slot, s->attempt_inc));
+        int rc = sync_read(s);
+        if (rc) return rc;
+        for (int m = 0; m < 3; ++m) maxima[m] = s->host[3 * fvl::NSLOT + m];
+        return TORCH_OK;
+    }
     if (!s->maxima_valid) {
         int rc = compute_maxima(s);
         if (rc) return rc;
@@ -933,6 +956,7 @@ int torch_state_step(torch_state* s, double time, double t_end, torch_step_info*
     s->maxima_valid = true;
     for (int m = 0; m < 3; ++m) info->wave_max[m] = s->maxima[m];
     s->cur = c.reactive ? D : hout;
+    s->version += 1;
     return TORCH_OK;
 }
 
@@ -957,6 +981,7 @@ int torch_state_hydro(torch_state* s, double dt, int want_fluxes, torch_step_inf
     info->fallback = fb;
     for (int f = 0; f < s->nf; ++f) info->outward[f] = hv[fvl::S_OUT + f];
     s->cur = out;
+    s->version += 1;
     // 1D/2D scalar-gamma inert: run_hydro already folded the CFL maxima of
     // the new state (the pass compute_maxima would launch), so the next
     // dt needs no launch and no wait.
@@ -989,15 +1014,23 @@ int torch_state_react(torch_state* s, double dt) {
         return fail(TORCH_INVALID, "Reaction time and constants must be finite and nonnegative");
     std::lock_guard<std::mutex> lock(s->ctx->mu);
     HIP_TRY(hipSetDevice(s->ctx->device));
-    const int A = s->cur, B = (A + 1) % 4;
+    const int A = s->cur, B = (A + 1) % 4, C = (A + 2) % 4;
     HIP_TRY(clear_flags(s->flags, s->ctx->stream));
-    int rc = run_react(s, A, B, dt, -1, -1, false);
+    // FP32 form, 1D/2D Cartesian: the half-step's per-cell increments go to
+    // slot C, which nothing reads or writes before the state next changes.
+    const bool record = s->cfg.precision == TORCH_FP32 && s->cfg.dim != 3 && !s->has_metric;
+    int rc = run_react(s, A, B, dt, -1, -1, false, record ? s->U[C] : nullptr,
+                       s->cfg.dim == 1 ? fvl::INC_HEAT : fvl::INC_SPEED);
     if (!rc) rc = sync_read(s);
     if (rc) return rc;
     if (host_flags(s) & fvl::F_REACT_BAD)
         return fail(TORCH_INVALID, "Reaction requires a physical state and fractions in [0,1]");
     s->cur = B;
     s->maxima_valid = false;
+    s->version += 1;
+    s->attempt_slot = record ? A : -1;
+    s->attempt_version = s->version;
+    s->attempt_inc = record ? s->U[C] : nullptr;
     return TORCH_OK;
 }
 
@@ -1150,6 +1183,7 @@ int torch_state_metric_advance(torch_state* s, double dt, torch_step_info* info)
     info->dt = dt;
     s->cur = C;
     s->maxima_valid = false;
+    s->version += 1;
     return TORCH_OK;
 }
 
@@ -1229,6 +1263,7 @@ int torch_state_metric_step(torch_state* s, double time, double t_end, torch_ste
         }
         s->cur = final_slot;
         s->maxima_valid = false;
+        s->version += 1;
         return TORCH_OK;
     }
     return fail(TORCH_NONPHYSICAL, "Could not find a positive metric/face step");

@@ -9,7 +9,9 @@ returns: states (3, n, steps + 1), times, and for the reactive driver the
 lambda levels (steps + 1, n)). The loop keeps the drivers' own rules, which
 differ from Simulation.step: dt clamped with ``if t + dt > t_end``, a
 stall below 1e-15, ``t += dt``, and in the reactive driver a CFL test of
-the heated state against hydro_dt_1d alone (``torch_state_wave_maxima``).
+the heated state against hydro_dt_1d alone (``torch_state_wave_maxima``,
+which in the FP32 form measures the heating's exact change on the state
+the half-step started from, so the test rejects what the FP64 one does).
 """
 
 import numpy as np
