@@ -10,7 +10,8 @@ workload's length). The JSON line reports:
 
 - value: whole-job cell-updates/s of the FP32 batched form (FP32 bulk
   arithmetic, FP64 CFL/balance reductions; the instance stepped by graph
-  replays of the batch engine with the library's default tile);
+  replays of the batch engine with the FP32 form's 768-thread 32x24 tile,
+  the autotuner's fastest shape for one grid this large);
 - fp64_plain: the FP64 plain method through the session step loop;
 - fp64_batched: the FP64 plain method as a graph-replayed batch of one;
 - roofline: the 3D step kernel's algorithmic HBM bytes (40 B per FP32
@@ -112,7 +113,7 @@ def run_batched_sod(n, steps, warmup, precision="fp32", tile=None, end_pass=Fals
     if tile == "tune":
         tile, _ = batch.tune(reps=3)
     else:
-        tile = tile or DEFAULT_TILE[precision]
+        tile = tile or SOD_TILE[precision]
         batch.set_tiles(*tile)
     batch.run(warmup)
     kernel_ms = batch.time_pass(reps=10)  # while the instance is still stepping
@@ -302,24 +303,30 @@ def run_reactive(seed, count, t_run, steps, warmup, precision="fp32", tile=None)
 
 
 # HBM bytes per launch of the 3D step kernel on the 256^3 Sod tube with the
-# default tiles (FP32 32x8, FP64 32x16, z chunks of 24), from rocprofv3
+# Sod legs' tiles (FP32 32x24, FP64 32x16, z chunks of 24; and the FP32
+# library default 32x8), from rocprofv3
 # FETCH_SIZE and WRITE_SIZE counter runs over the 200-step workload (mean per
 # dispatch) scaled by a 1 GiB device copy (FETCH_SIZE x 2.000, WRITE_SIZE x
 # 1.000); the commands and per-counter values are kept with the profile
 # summaries.
-TRAFFIC = {("f32", 256, (32, 8, 24)): 1464121217, ("f64", 256, (32, 16, 24)): 2203313824}
+TRAFFIC = {("f32", 256, (32, 24, 24)): 1257390379, ("f32", 256, (32, 8, 24)): 1464121217,
+           ("f64", 256, (32, 16, 24)): 2203313824}
 
 # Wave-level VALU instructions per launch of the same kernels (rocprofv3
 # --pmc SQ_INSTS_VALU, mean over the dispatches of the same 200-step runs).
-VALU_INSTS = {("f32", 256, (32, 8, 24)): 240896000, ("f64", 256, (32, 16, 24)): 298922000}
+VALU_INSTS = {("f32", 256, (32, 24, 24)): 196339627, ("f32", 256, (32, 8, 24)): 240896000,
+              ("f64", 256, (32, 16, 24)): 298922000}
 
 # VALU issue peak of MI355X: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz lane
 # slots per second (a wave64 instruction holds a SIMD for 4 cycles; FP32
 # without packed math and FP64 both issue one lane operation per lane slot).
 VALU_PEAK = 256 * 4 * 16 * 2.4e9
 
-# The library's default tiles (BX, BY, z chunk) per precision.
-DEFAULT_TILE = {"fp32": (32, 8, 24), "fp64": (32, 16, 24)}
+# Tiles (BX, BY, z chunk) of the 256^3 Sod legs per precision: FP64 the
+# library default; FP32 its 768-thread 32x24 tile, the fastest shape of the
+# autotuner's sweep for one grid this large (the library's FP32 default,
+# 32x8, is the faster one for mixes of small grids).
+SOD_TILE = {"fp32": (32, 24, 24), "fp64": (32, 16, 24)}
 
 
 def roofline(n, bytes_per_cell, kernel_ms, dtype=None, tile=None):
@@ -743,10 +750,10 @@ def main(argv=None):
     r64 = r64b = r32s = r32 = err = uneven = cpu = None
     if "fp64" in legs:
         say(f"# 3D Sod {n}^3: FP64 plain, {args.warmup}+{args.steps} steps")
-        r64 = run_precision(n, "fp64", args.steps, args.warmup)
+        r64 = run_precision(n, "fp64", args.steps, args.warmup, tile=SOD_TILE["fp64"])
         say(f"# fp64 plain: {r64['ms_per_step']:.3f} ms/step, kernel {r64['kernel_ms']:.3f} ms")
     if "fp32" in legs:
-        r32s = run_precision(n, "fp32", args.steps, args.warmup)
+        r32s = run_precision(n, "fp32", args.steps, args.warmup, tile=SOD_TILE["fp32"])
         say(f"# fp32 session loop: {r32s['ms_per_step']:.3f} ms/step, kernel {r32s['kernel_ms']:.3f} ms")
     if "batched" in legs:
         r32 = run_batched_sod(n, args.steps, args.warmup, tile=args.tile)
@@ -760,7 +767,7 @@ def main(argv=None):
         full = {"steps": WORKLOAD_STEPS, "warmup": 0}
         if "batched" in legs:
             f32 = run_batched_sod(n, WORKLOAD_STEPS, 0, tile=args.tile, end_pass=True)
-            t32 = f32["tile"] if f32["tile"] else DEFAULT_TILE["fp32"]
+            t32 = f32["tile"] if f32["tile"] else SOD_TILE["fp32"]
             full["fp32_batched"] = {
                 "value": round(f32["value"], 1), "ms_per_step": round(f32["ms_per_step"], 4),
                 "kernel_ms_start": round(f32["kernel_ms"], 4), "kernel_ms_end": round(f32["kernel_ms_end"], 4),
@@ -867,10 +874,10 @@ def main(argv=None):
                                "gamma 1.4, full Simulation.step; FP32 form (f32 bulk, f64 reductions) as a "
                                "graph-replayed batch of one",
                    "cells": n ** 3, "tile": list(r32["tile"]) if r32 else None, "parallelism": "1 GPU"},
-        "roofline": roofline(n, 40, head["kernel_ms"], "f32", r32["tile"] if r32 else DEFAULT_TILE["fp32"]) if head else None,
+        "roofline": roofline(n, 40, head["kernel_ms"], "f32", r32["tile"] if r32 else SOD_TILE["fp32"]) if head else None,
         "fp64_plain": None if not r64 else {
             "value": round(r64["value"], 1), "ms_per_step": round(r64["ms_per_step"], 4), "dtype": "f64",
-            "kernel_ms": round(r64["kernel_ms"], 4), "roofline": roofline(n, 80, r64["kernel_ms"], "f64", DEFAULT_TILE["fp64"]),
+            "kernel_ms": round(r64["kernel_ms"], 4), "roofline": roofline(n, 80, r64["kernel_ms"], "f64", SOD_TILE["fp64"]),
             "fallbacks": r64["fallbacks"]},
         "fp64_batched": None if not r64b else {
             "value": round(r64b["value"], 1), "ms_per_step": round(r64b["ms_per_step"], 4), "dtype": "f64",

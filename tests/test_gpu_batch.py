@@ -230,11 +230,32 @@ def test_batch_autotune_keeps_results():
     ids = [batch.add(s) for s in sims]
     batch.run(2)
     (bx, by, zc), ms = batch.tune(reps=2)
-    assert (bx, by) in ((64, 4), (64, 8), (32, 8), (32, 16)) and zc in (8, 16, 32, 64) and ms > 0
+    assert (bx, by) in ((64, 4), (64, 8), (32, 8), (32, 16)) and zc in (8, 16, 24, 32, 64) and ms > 0
     batch.run(4)
     for s, i in zip(sims, ids):
         # the tile changes the order of the boundary sums, not the cell arithmetic
         _check_same(batch, i, _alone(s, 6), exact=False)
+        assert np.array_equal(batch.state(i), _alone(s, 6).state)
+    batch.close()
+
+
+def test_fp32_batch_autotune_and_wide_tile():
+    """The FP32 form's sweep includes its 768-thread 32x24 tile (the FP64
+    form has none and refuses it); whatever the tuner picks, every instance
+    steps exactly as it does alone on the default tile."""
+    from torch_amd.torchBatch import Batch
+    with pytest.raises(ValueError):
+        Batch(precision="fp64").set_tiles(32, 24, 24)
+    sims = _sessions("fp32", seed=14, specs=SPECS[7:])
+    batch = Batch(precision="fp32")
+    ids = [batch.add(s) for s in sims]
+    batch.run(2)
+    (bx, by, zc), ms = batch.tune(reps=2)
+    assert (bx, by) in ((64, 4), (64, 8), (32, 8), (32, 16), (32, 24)) and zc in (8, 16, 24, 32, 64) and ms > 0
+    batch.run(2)
+    batch.set_tiles(32, 24, 24)
+    batch.run(2)
+    for s, i in zip(sims, ids):
         assert np.array_equal(batch.state(i), _alone(s, 6).state)
     batch.close()
 

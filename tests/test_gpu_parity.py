@@ -236,3 +236,33 @@ def test_hydro3d_random_tiles_against_oracle(tam, limiter):
         for a, b, name in zip(fl, rf, "xyz"):
             assert_fields(a, b, what=f"tile {tile} F{name}")
         ex.close()
+
+
+@pytest.mark.parametrize("limiter", LIMITERS)
+def test_hydro3d_fp32_tiles_agree_bitwise(tam, limiter):
+    """The FP32 form's 768-thread tile (32x24) against its 256-thread tiles
+    on the same uneven grid (tile edges inside the domain in x, y and z):
+    the per-cell arithmetic does not depend on the tile, so states and face
+    fluxes are bitwise equal."""
+    rng = np.random.default_rng(11)
+    nz, ny, nx = 37, 53, 70
+    p = np.empty((5, nz, ny, nx))
+    p[0] = rng.uniform(0.5, 1.5, (nz, ny, nx))
+    p[1:4] = rng.uniform(-0.3, 0.3, (3, nz, ny, nx))
+    p[4] = rng.uniform(0.5, 1.5, (nz, ny, nx))
+    state = o3.to_cons(p, 1.4)
+    bcs = ("Periodic", "Reflective", "Open")
+    from torch_amd.torchSolver3D import hydro
+    from torch_amd import DeviceExecution
+
+    outs = []
+    for tile in ((32, 24, 24), (32, 24, 8), (32, 8, 24), (64, 4, 16)):
+        ex = DeviceExecution(precision="fp32")
+        ex.tile = tile
+        outs.append((tile,) + tuple(hydro(state, 1.4, (0.01, 0.02, 0.03), 1e-3, limiter, bcs, execution=ex)))
+        ex.close()
+    (t0, out0, fl0) = outs[0]
+    for tile, out, fl in outs[1:]:
+        assert np.array_equal(out, out0), (tile, t0)
+        for a, b in zip(fl, fl0):
+            assert np.array_equal(a, b), (tile, t0)

@@ -152,7 +152,7 @@ void layout(torch_batch* b, Inst* p) {
     if (g.dim == 3) {
         p->tiles[0] = (g.nx + b->bx - 3) / (b->bx - 2);
         p->tiles[1] = (g.ny + b->by - 3) / (b->by - 2);
-        p->zchunk = b->tiles_set ? b->zchunk : default_zchunk(g.nz);
+        p->zchunk = b->tiles_set ? b->zchunk : default_zchunk(g.nz, b->by);
         p->tiles[2] = (g.nz + p->zchunk - 1) / p->zchunk;
         p->nblk = p->tiles[0] * p->tiles[1] * p->tiles[2];
         p->nllf = o->llf3d_blocks(g);
@@ -854,20 +854,24 @@ int torch_batch_tune(torch_batch* b, int reps, int* block_x, int* block_y, int* 
     if (!b || reps <= 0) return fail(TORCH_INVALID, "invalid argument");
     std::lock_guard<std::mutex> lock(b->ctx->mu);
     HIP_TRY(hipSetDevice(b->ctx->device));
-    int rc;
+    int rc = TORCH_OK;
     if (b->dirty && (rc = rebuild(b))) return rc;
     hipStream_t st = b->ctx->stream;
     double best = -1;
     if (b->seg[1] > b->seg[0]) {
-        static const int shapes[4][2] = {{64, 4}, {64, 8}, {32, 8}, {32, 16}};
+        // 32x24 (768 threads, one block per CU) exists in the FP32 form only:
+        // FP64 tiles need more than the 168 VGPRs three waves per SIMD allow.
+        static const int shapes[5][2] = {{64, 4}, {64, 8}, {32, 8}, {32, 16}, {32, 24}};
         static const int chunks[5] = {8, 16, 24, 32, 64};
+        const int nshape = b->precision == TORCH_FP32 ? 5 : 4;
         int bbx = b->bx, bby = b->by, bz = b->zchunk;
         b->tiles_set = true;
         hipEvent_t e0, e1;
         HIP_TRY(hipEventCreate(&e0));
         HIP_TRY(hipEventCreate(&e1));
-        for (auto& sh : shapes)
+        for (int si = 0; si < nshape && !rc; ++si)
             for (int zc : chunks) {
+                const int* sh = shapes[si];
                 b->bx = sh[0];
                 b->by = sh[1];
                 b->zchunk = zc;
@@ -935,7 +939,8 @@ int torch_batch_time_pass(torch_batch* b, int reps, double* ms) {
 
 int torch_batch_set_tiles(torch_batch* b, int block_x, int block_y, int zchunk) {
     if (!b) return fail(TORCH_INVALID, "null argument");
-    const bool ok = (block_x == 64 && (block_y == 4 || block_y == 8)) || (block_x == 32 && (block_y == 8 || block_y == 16));
+    const bool ok = (block_x == 64 && (block_y == 4 || block_y == 8)) ||
+                    (block_x == 32 && (block_y == 8 || block_y == 16 || (block_y == 24 && b->precision == TORCH_FP32)));
     if (!ok || zchunk < 1) return fail(TORCH_INVALID, "unsupported tile shape");
     std::lock_guard<std::mutex> lock(b->ctx->mu);
     HIP_TRY(hipSetDevice(b->ctx->device));

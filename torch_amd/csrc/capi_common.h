@@ -25,12 +25,14 @@ struct Ops;
 int upload_as(const Ops* ops, hipStream_t st, void* dst, const double* src, long n);
 int download_as(const Ops* ops, hipStream_t st, double* dst, const void* src, long n);
 
-// Default z chunk of the 3D step for a grid nz planes deep, shared by
-// sessions and batches so that a batch of one folds its partials in the
-// same blocks as the session: 24 planes where the grid is deep enough to
-// fill the chip with such chunks (fastest in a 10-32 plane sweep of the
-// 256^3 Sod step in both precisions), else 16.
-inline int default_zchunk(int nz) { return nz >= 128 ? 24 : 16; }
+// Default z chunk of the 3D step for a grid nz planes deep and a tile
+// block_y threads tall, shared by sessions and batches so that a batch of
+// one folds its partials in the same blocks as the session: 24 planes where
+// the grid is deep enough to fill the chip with such chunks (fastest in a
+// 10-32 plane sweep of the 256^3 Sod step in both precisions), else 16;
+// the 32x24 FP32 tile (one block per CU) takes 24 at every depth (faster
+// than 16 on the 32^3..160^3 ignition mix).
+inline int default_zchunk(int nz, int block_y) { return (nz >= 128 || block_y == 24) ? 24 : 16; }
 
 #define HIP_TRY(expr)                                                                              \
     do {                                                                                           \

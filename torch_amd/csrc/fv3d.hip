@@ -132,7 +132,10 @@ __device__ __forceinline__ void wave3(const real* o, double g, double (&mx)[3]) 
 // blocks per CU at <= 168 VGPRs, which hides the z-march's load and barrier
 // latency better than two; 256-thread FP64 tiles and FP32 tiles carrying two
 // fractions are held to two (<= 256 VGPRs, no accumulation registers) so two
-// blocks share a CU instead of one.
+// blocks share a CU instead of one. Larger blocks run one per CU: the
+// 768-thread FP32 tile (32x24) then gets three waves per SIMD (<= 168
+// VGPRs; with two fractions 6 VGPRs spill, still faster than 32x8 at two
+// waves), 512-thread tiles two.
 constexpr int occupancy_of(int threads, int nq = 0) {
     return threads == 256 ? (sizeof(real) == 4 && nq < 2 ? 3 : 2) : 1;
 }
@@ -974,6 +977,10 @@ static hipError_t launch_shape(const Grid& g, const Step& s, const fvl::Launch& 
     if (l.block_x == 64 && l.block_y == 8) return launch_lim<64, 8, NQ>(g, s, l.stream);
     if (l.block_x == 32 && l.block_y == 8) return launch_lim<32, 8, NQ>(g, s, l.stream);
     if (l.block_x == 32 && l.block_y == 16) return launch_lim<32, 16, NQ>(g, s, l.stream);
+    if (l.block_x == 32 && l.block_y == 24) {
+        // The 768-thread tile exists in the FP32 form only.
+        if constexpr (sizeof(real) == sizeof(float)) return launch_lim<32, 24, NQ>(g, s, l.stream);
+    }
     return hipErrorInvalidValue;
 }
 
@@ -1016,7 +1023,11 @@ static hipError_t batch_shape(const fvl::BatchItem* items, const int* first, int
     else if (bx == 64 && by == 8) launch_batch3d<64, 8, NQ>(items, first, n, blocks, limiter, st);
     else if (bx == 32 && by == 8) launch_batch3d<32, 8, NQ>(items, first, n, blocks, limiter, st);
     else if (bx == 32 && by == 16) launch_batch3d<32, 16, NQ>(items, first, n, blocks, limiter, st);
-    else return hipErrorInvalidValue;
+    else if (bx == 32 && by == 24) {
+        // The 768-thread tile exists in the FP32 form only.
+        if constexpr (sizeof(real) == sizeof(float)) launch_batch3d<32, 24, NQ>(items, first, n, blocks, limiter, st);
+        else return hipErrorInvalidValue;
+    } else return hipErrorInvalidValue;
     return hipGetLastError();
 }
 
