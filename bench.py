@@ -303,19 +303,16 @@ def run_reactive(seed, count, t_run, steps, warmup, precision="fp32", tile=None)
 
 
 # HBM bytes per launch of the 3D step kernel on the 256^3 Sod tube with the
-# Sod legs' tiles (FP32 32x24, FP64 32x16, z chunks of 24; and the FP32
-# library default 32x8), from rocprofv3
+# Sod legs' tiles (FP32 32x24, FP64 32x16, z chunks of 24), from rocprofv3
 # FETCH_SIZE and WRITE_SIZE counter runs over the 200-step workload (mean per
 # dispatch) scaled by a 1 GiB device copy (FETCH_SIZE x 2.000, WRITE_SIZE x
 # 1.000); the commands and per-counter values are kept with the profile
 # summaries.
-TRAFFIC = {("f32", 256, (32, 24, 24)): 1257390379, ("f32", 256, (32, 8, 24)): 1464121217,
-           ("f64", 256, (32, 16, 24)): 2203313824}
+TRAFFIC = {("f32", 256, (32, 24, 24)): 1275653237, ("f64", 256, (32, 16, 24)): 2203313824}
 
 # Wave-level VALU instructions per launch of the same kernels (rocprofv3
 # --pmc SQ_INSTS_VALU, mean over the dispatches of the same 200-step runs).
-VALU_INSTS = {("f32", 256, (32, 24, 24)): 196339627, ("f32", 256, (32, 8, 24)): 240896000,
-              ("f64", 256, (32, 16, 24)): 298922000}
+VALU_INSTS = {("f32", 256, (32, 24, 24)): 200359171, ("f64", 256, (32, 16, 24)): 298922000}
 
 # VALU issue peak of MI355X: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz lane
 # slots per second (a wave64 instruction holds a SIMD for 4 cycles; FP32

@@ -42,13 +42,13 @@ def test_more_than_one_gpu_is_refused(capsys):
 
 def test_roofline_fractions_follow_the_counted_bytes_and_instructions():
     # 40 B per FP32 cell-update of the 256^3 tube over a 0.5 ms launch
-    r = bench.roofline(256, 40, 0.5, "f32", (32, 8, 24))
+    r = bench.roofline(256, 40, 0.5, "f32", (32, 24, 24))
     assert r["bound"] == "hbm" and r["unit"] == "GB/s" and r["peak"] == bench.HBM_PEAK_GBS
     assert r["algorithmic_bytes"] == 40 * 256 ** 3
     assert abs(r["achieved"] - 40 * 256 ** 3 / 0.5e-3 / 1e9) < 0.1
     assert abs(r["frac"] - r["achieved"] / bench.HBM_PEAK_GBS) < 1e-4
-    assert r["traffic"] == bench.TRAFFIC[("f32", 256, (32, 8, 24))]
-    insts = bench.VALU_INSTS[("f32", 256, (32, 8, 24))]
+    assert r["traffic"] == bench.TRAFFIC[("f32", 256, (32, 24, 24))]
+    insts = bench.VALU_INSTS[("f32", 256, (32, 24, 24))]
     v = r["valu_issue"]
     assert v["insts_per_launch"] == insts and v["peak"] == 256 * 4 * 16 * 2.4e9
     assert abs(v["frac"] - insts * 64 / 0.5e-3 / v["peak"]) < 1e-4
