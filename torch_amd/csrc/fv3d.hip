@@ -154,13 +154,14 @@ struct Tile3 {
     static constexpr int FLUX = NFY * BX * BY;
     static constexpr int BUF = PLANE;
     static constexpr int YS = NYS * BY * BX;
-    // FP32 tiles without transported fractions stage their global reads
-    // through LDS a plane ahead: each thread's own cell (5 fields) and the
-    // ring cells, double-buffered by plane parity, the ring padded to whole
-    // waves.
-    static constexpr bool STAGED = sizeof(real) == sizeof(float) && NQ_ == 0;
+    // FP32 tiles stage their global reads through LDS a plane ahead: each
+    // thread's own cell (5 fields, then lambda or gamma and the fractions:
+    // OWN rows) and the ring cells (5 fields), double-buffered by plane
+    // parity, the ring padded to whole waves.
+    static constexpr bool STAGED = sizeof(real) == sizeof(float);
+    static constexpr int OWN = NQ_ == 0 ? 5 : 6 + NQ_;
     static constexpr int RING = (2 * PH + 2 * BX + 63) / 64 * 64;
-    static constexpr int DMA = STAGED ? 10 * (BX * BY + RING) : 1;
+    static constexpr int DMA = STAGED ? 2 * (OWN * BX * BY + 5 * RING) : 1;
 };
 
 using fvl::Coef;
@@ -237,21 +238,21 @@ __device__ __forceinline__ void step3d_tile(const Step& s, const G3& g, int tbx,
     load5(U, Ue, N, g, x, y, z0 - 1, Ac);
     // FP64 tiles load their ring cell one plane ahead into registers (after
     // the slopes, so the load overlaps the plane's arithmetic): 2-6% faster
-    // there. FP32 tiles have no registers to spare; those without fractions
-    // instead copy each thread's next-but-one cell and the next ring cell
-    // from global memory straight into LDS (global_load_lds, no registers),
-    // issued right after a plane's first barrier: the barrier after it waits
-    // for the copy (a pending LDS write), so its latency hides behind the
-    // slopes and the predictor, and the next plane starts from LDS instead
-    // of waiting on global loads at its first barrier (256^3 Sod step 7-10%
-    // faster, same results bitwise). Tiles carrying fractions keep register
-    // loads: there the staging's addressing spills and the reactive batch
-    // ran 6% slower.
+    // there. FP32 tiles have no registers to spare and instead copy each
+    // thread's next-but-one cell (with its lambda or gamma and fractions)
+    // and the next ring cell from global memory straight into LDS
+    // (global_load_lds, no registers), issued right after a plane's first
+    // barrier: the barrier after it waits for the copy (a pending LDS
+    // write), so its latency hides behind the slopes and the predictor, and
+    // the next plane starts from LDS instead of waiting on global loads at
+    // its first barrier (256^3 Sod step 7-10% faster, reactive batch 7%,
+    // same results bitwise). Staging the state alone while lambda and the
+    // fractions stayed register loads spilled and was 6% slower there.
     constexpr bool kRingAhead = sizeof(real) == sizeof(double);
     constexpr bool kDma = T::STAGED;
-    constexpr int RING = T::RING;
-    real* sown = sdma;                  // [2][5][NT]: own cell, by plane parity
-    real* sring = sdma + 10 * BX * BY;  // [2][5][RING]: ring cell, by plane parity
+    constexpr int RING = T::RING, OWN = T::OWN;
+    real* sown = sdma;                        // [2][OWN][NT]: own cell, by plane parity
+    real* sring = sdma + 2 * OWN * BX * BY;  // [2][5][RING]: ring cell, by plane parity
     const int wbase = tid & ~63;        // first thread of this wave: its LDS rows start there
     const int ox = ghost_src(x, nx, g.bcx), oy = ghost_src(y, ny, g.bcy);
     const int qx = ghost_src(bx0 + rx, nx, g.bcx), qy = ghost_src(by0 + ry, ny, g.bcy);
@@ -259,10 +260,16 @@ __device__ __forceinline__ void step3d_tile(const Step& s, const G3& g, int tbx,
     // (each lane's value lands at its wave's row base + 4 x lane).
     auto stage = [&](int kk, int kr) {
         const long io = ((long)ghost_src(kk, nz, g.bcz) * ny + oy) * nx + ox;
-        real* d = sown + (kk & 1) * 5 * BX * BY + wbase;
+        real* d = sown + (kk & 1) * OWN * BX * BY + wbase;
 #pragma unroll
         for (int c = 0; c < 4; ++c) __builtin_amdgcn_global_load_lds(U + c * N + io, d + c * BX * BY, 4, 0, 0);
         __builtin_amdgcn_global_load_lds(Ue + 4 * N + io, d + 4 * BX * BY, 4, 0, 0);
+        if constexpr (OWN > 5) {
+            if (lamp) __builtin_amdgcn_global_load_lds(lamp + io, d + 5 * BX * BY, 4, 0, 0);
+            for (int j = 0; j < NQ; ++j)
+                if (j < nq)
+                    __builtin_amdgcn_global_load_lds((const real*)s.q[j] + io, d + (6 + j) * BX * BY, 4, 0, 0);
+        }
         if (tid < nring) {
             const long ir = ((long)ghost_src(kr, nz, g.bcz) * ny + qy) * nx + qx;
             real* e = sring + (kr & 1) * 5 * RING + wbase;
@@ -288,17 +295,23 @@ __device__ __forceinline__ void step3d_tile(const Step& s, const G3& g, int tbx,
         if constexpr (kDma) {
             // Plane k+1, staged in LDS during the previous plane (or before
             // the loop); the same sign flips as load5.
-            const real* so = sown + ((k + 1) & 1) * 5 * BX * BY + tid;
+            const real* so = sown + ((k + 1) & 1) * OWN * BX * BY + tid;
 #pragma unroll
             for (int c = 0; c < 5; ++c) An[c] = so[c * BX * BY];
             if (ghost_flip(x, nx, g.bcx)) An[1] = -An[1];
             if (ghost_flip(y, ny, g.bcy)) An[2] = -An[2];
             if (ghost_flip(k + 1, nz, g.bcz)) An[3] = -An[3];
+            if constexpr (OWN > 5) {
+                if (lamp) gn = gamma_from(so[5 * BX * BY], direct);
+                for (int j = 0; j < NQ; ++j) if (j < nq) qn[j] = so[(6 + j) * BX * BY];
+            } else {
+                if (lamp) gn = gamma_from(load1(lamp, g, x, y, k + 1), direct);
+            }
         } else {
             load5(U, Ue, N, g, x, y, k + 1, An);
+            if (lamp) gn = gamma_from(load1(lamp, g, x, y, k + 1), direct);
+            for (int j = 0; j < NQ; ++j) if (j < nq) qn[j] = load1((const real*)s.q[j], g, x, y, k + 1);
         }
-        if (lamp) gn = gamma_from(load1(lamp, g, x, y, k + 1), direct);
-        for (int j = 0; j < NQ; ++j) if (j < nq) qn[j] = load1((const real*)s.q[j], g, x, y, k + 1);
 
         // No barrier needed before rewriting sA: its last readers finished
         // before the previous plane's second barrier.
