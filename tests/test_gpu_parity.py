@@ -163,6 +163,30 @@ def test_session3d_reactive_retry_golden(tam, model):
     check_session(f"ignition3d_{model}", sim, dts)
 
 
+@pytest.mark.parametrize("model", ["arrhenius", "znd"])
+def test_fp32_reactive_session_with_walls_tracks_fp64(tam, model):
+    """The FP32 form's reactive 3D tiles (state, lambda and fractions staged
+    in LDS a plane ahead, reflective sign flips applied on reading) against
+    the FP64 method (register loads) on the ignition vectors' initial state,
+    with reflective walls in x and z: the same steps and dt halvings, dt
+    within 1e-6 relative, every field within 1e-5 relative L1, lambda within
+    1e-5 absolute. A flip applied to the wrong cell or component breaks the
+    wall-normal momentum at O(1)."""
+    d = golden("session")
+    opts = dict(t_end=1e-5, reactive=True, kinetic_model=model, lambda_init=d[f"ignition3d_{model}_lam0"],
+                boundaries=["Reflective", "Periodic", "Reflective"])
+    s64, dt64 = run_device(d["ignition3d_prim"], [1.0] * 3, 4, precision="fp64", **opts)
+    s32, dt32 = run_device(d["ignition3d_prim"], [1.0] * 3, 4, precision="fp32", **opts)
+    assert s32.steps == s64.steps and np.allclose(dt32, dt64, rtol=1e-6, atol=0)
+    for c in range(5):
+        ref = np.abs(s64.state[c]).sum()
+        if ref > 0:
+            assert np.abs(s32.state[c] - s64.state[c]).sum() / ref <= 1e-5, c
+    assert np.max(np.abs(s32.lam - s64.lam)) <= 1e-5
+    s32.close()
+    s64.close()
+
+
 def test_session2d_quadrant_and_rde_golden(tam):
     d = golden("session")
     n = 24
