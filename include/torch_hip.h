@@ -61,7 +61,8 @@ typedef struct torch_config {
     int wavespeed;             /* 1D: 0 "Pressure Based", 1 "Enthalpy Based" */
     double cfl;
     int precision;             /* enum torch_precision */
-    int block_x, block_y, zchunk; /* 3D tile; 0 selects the tuned default */
+    int block_x, block_y, zchunk; /* 3D tile (64x4, 64x8, 32x8, 32x16; 32x24 in
+                                     FP32 only); 0 selects the tuned default */
     double spacing[3];         /* cell sizes x, y, z; 0 derives lengths / n */
     int gamma_mode;            /* 0 scalar gamma; 1 gamma(lambda) per cell (reactive);
                                   2 per-cell gamma array uploaded in place of lambda */
@@ -318,6 +319,9 @@ int torch_batch_download(torch_batch* batch, int id, double* U, double* lam, dou
 /* Autotuner: times the 3D second-order work list over the batch's current
  * instances for every supported tile shape and z chunk, keeps the fastest. */
 int torch_batch_tune(torch_batch* batch, int reps, int* block_x, int* block_y, int* zchunk, double* best_ms);
+/* Fix the tile instead of tuning it: a shape listed at torch_config.block_x
+ * (32x24 only for an FP32 batch; TORCH_INVALID otherwise) and any z chunk
+ * >= 1. */
 int torch_batch_set_tiles(torch_batch* batch, int block_x, int block_y, int zchunk);
 /* Measurement hook: average milliseconds of `reps` launches of the 3D
  * second-order work list on the current states (HIP events on the batch's
