@@ -246,6 +246,11 @@ def test_fp32_batch_autotune_and_wide_tile():
     from torch_amd.torchBatch import Batch
     with pytest.raises(ValueError):
         Batch(precision="fp64").set_tiles(32, 24, 24)
+    from torch_amd import Simulation
+    p = np.ones((5, 6, 6, 6))
+    p[1:4] = 0.0
+    with pytest.raises(ValueError, match="tile"):
+        Simulation(p, [1.0, 1.0, 1.0], t_end=1.0, precision="fp64", tile=(32, 24, 24)).step()
     sims = _sessions("fp32", seed=14, specs=SPECS[7:])
     batch = Batch(precision="fp32")
     ids = [batch.add(s) for s in sims]

@@ -45,6 +45,14 @@ int check_config(const torch_config& c) {
     if (c.precision != TORCH_FP64 && c.precision != TORCH_FP32) return fail(TORCH_INVALID, "unknown precision");
     if (c.gamma_mode < 0 || c.gamma_mode > 2 || (c.reactive && c.gamma_mode != 1))
         return fail(TORCH_INVALID, "gamma_mode must be 1 for reactive states, 0 or 2 otherwise");
+    // A chosen 3D tile must be one the step kernels are built for (the
+    // 768-thread 32x24 one exists in the FP32 form only); 0 picks the default.
+    if (c.dim == 3 && c.block_x != 0 && c.block_y != 0) {
+        const bool ok = (c.block_x == 64 && (c.block_y == 4 || c.block_y == 8)) ||
+                        (c.block_x == 32 && (c.block_y == 8 || c.block_y == 16 ||
+                                             (c.block_y == 24 && c.precision == TORCH_FP32)));
+        if (!ok) return fail(TORCH_INVALID, "unsupported tile shape");
+    }
     return TORCH_OK;
 }
 
