@@ -319,6 +319,12 @@ int torch_batch_download(torch_batch* batch, int id, double* U, double* lam, dou
 /* Autotuner: times the 3D second-order work list over the batch's current
  * instances for every supported tile shape and z chunk, keeps the fastest. */
 int torch_batch_tune(torch_batch* batch, int reps, int* block_x, int* block_y, int* zchunk, double* best_ms);
+/* Measurement hook, no counterpart in the reference: advance the batch by
+ * `steps` replays of its step graph, captured with a HIP event pair around
+ * its one 3D work list (TORCH_INVALID if it has none or several), one replay
+ * at a time, and return that kernel's mean duration in ms (`ms`). The
+ * replays are ordinary steps; only the host waits between them. */
+int torch_batch_time_step_kernel(torch_batch* batch, int steps, double* ms);
 /* Fix the tile instead of tuning it: a shape listed at torch_config.block_x
  * (32x24 only for an FP32 batch; TORCH_INVALID otherwise) and any z chunk
  * >= 1. */

@@ -265,6 +265,31 @@ def test_fp32_batch_autotune_and_wide_tile():
     batch.close()
 
 
+def test_step_kernel_timing_takes_ordinary_steps():
+    """The step graph captured with an event pair around the one 3D work
+    list advances the instance exactly as the ordinary graph does (bitwise
+    against the session alone), later runs replay the ordinary graph, and a
+    batch with several 3D work lists is refused."""
+    from torch_amd.torchBatch import Batch
+    sims = _sessions("fp32", seed=15, specs=SPECS[7:8])
+    batch = Batch(precision="fp32")
+    i = batch.add(sims[0])
+    batch.run(2)
+    ms = batch.time_step_kernel(3)
+    assert ms > 0
+    batch.run(1)
+    ref = _alone(sims[0], 6)
+    assert batch.info(i)["steps"] == ref.steps == 6
+    assert np.array_equal(batch.state(i), ref.state)
+    batch.close()
+    batch = Batch(precision="fp32")
+    for s in _sessions("fp32", seed=16, specs=SPECS[7:9]):  # Superbee and VanLeer: two 3D lists
+        batch.add(s)
+    with pytest.raises(ValueError, match="one 3D work list"):
+        batch.time_step_kernel(2)
+    batch.close()
+
+
 def test_batch_commit_into_session():
     from torch_amd.torchBatch import Batch
     sims = _sessions("fp64", seed=13, specs=SPECS[4:6])

@@ -32,7 +32,7 @@ EXPORTS = (
     "torch_batch_footprint", "torch_batch_create", "torch_batch_destroy", "torch_batch_add",
     "torch_batch_remove", "torch_batch_seed", "torch_batch_run", "torch_batch_info", "torch_batch_download",
     "torch_batch_tune", "torch_batch_set_tiles", "torch_batch_profile", "torch_batch_graph_counts",
-    "torch_batch_time_pass",
+    "torch_batch_time_pass", "torch_batch_time_step_kernel",
     "torch_state_set_metric", "torch_state_metric_dt", "torch_state_metric_rhs", "torch_state_metric_advance",
     "torch_state_metric_step", "torch_rusanov_faces", "torch_state_wave_maxima",
     "torch_amr_create", "torch_amr_destroy", "torch_amr_set_mesh", "torch_amr_upload", "torch_amr_download",
@@ -165,6 +165,7 @@ def load():
                 "torch_batch_graph_counts": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_long),
                                                             ctypes.POINTER(ctypes.c_long)]),
                 "torch_batch_time_pass": (ctypes.c_int, [vp, ctypes.c_int, _dp]),
+                "torch_batch_time_step_kernel": (ctypes.c_int, [vp, ctypes.c_int, _dp]),
                 "torch_state_set_metric": (ctypes.c_int, [vp, ctypes.c_int, _dp, ctypes.c_long,
                                                           ctypes.POINTER(Face)]),
                 "torch_state_metric_dt": (ctypes.c_int, [vp, _dp]),
@@ -586,6 +587,13 @@ class DeviceBatch:
         """Average ms of the 3D second-order work list (nothing advances)."""
         ms = ctypes.c_double()
         check(load().torch_batch_time_pass(self.handle, int(reps), ctypes.byref(ms)))
+        return ms.value
+
+    def time_step_kernel(self, steps=10):
+        """Advance `steps` ordinary steps, timing the one 3D work list of each
+        with HIP events inside the step graph; its mean ms per launch."""
+        ms = ctypes.c_double()
+        check(load().torch_batch_time_step_kernel(self.handle, int(steps), ctypes.byref(ms)))
         return ms.value
 
     def profile(self, enable=True):

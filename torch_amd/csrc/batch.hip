@@ -112,6 +112,11 @@ struct torch_batch {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double prof_ms = 0;
     long prof_steps = 0;
+    // torch_batch_time_step_kernel: while it runs, the captured step records
+    // kev[0] and kev[1] around the batch's one 3D work list (external event
+    // nodes: a plain record during capture only orders streams).
+    bool kev_capture = false;
+    hipEvent_t kev[2] = {nullptr, nullptr};
 };
 
 namespace {
@@ -207,8 +212,12 @@ int grow(void** p, size_t* cap, size_t bytes) {
 hipError_t enqueue_3d(torch_batch* b, hipStream_t st) {
     for (int L = 0; L < 5; ++L) {
         const Table& t = b->hyd[0][L];
-        hipError_t e = b->ops->batch3d(b->items + b->lim[0][L], b->tables + t.offset, t.count, t.blocks, b->bx, b->by,
-                                       L, 0, st);
+        const bool timed = b->kev_capture && t.blocks > 0;
+        hipError_t e = timed ? hipEventRecordWithFlags(b->kev[0], st, hipEventRecordExternal) : hipSuccess;
+        if (e == hipSuccess)
+            e = b->ops->batch3d(b->items + b->lim[0][L], b->tables + t.offset, t.count, t.blocks, b->bx, b->by, L, 0,
+                                st);
+        if (e == hipSuccess && timed) e = hipEventRecordWithFlags(b->kev[1], st, hipEventRecordExternal);
         if (e != hipSuccess) return e;
     }
     // Arrhenius instances transport one fraction, ZND ones two.
@@ -267,8 +276,10 @@ hipError_t enqueue_branched(torch_batch* b, hipStream_t st) {
         if (t.blocks <= 0) continue;
         hipStream_t s = f.open(st);
         open3.push_back(s);
+        if (b->kev_capture && (e = hipEventRecordWithFlags(b->kev[0], s, hipEventRecordExternal))) return e;
         if ((e = o->batch3d(b->items + b->lim[0][L], T + t.offset, t.count, t.blocks, b->bx, b->by, L, 0, s)))
             return e;
+        if (b->kev_capture && (e = hipEventRecordWithFlags(b->kev[1], s, hipEventRecordExternal))) return e;
     }
     // 2D and 1D pipelines: their lists in parallel, then the segment's
     // first-order (and 1D CFL) passes after the join.
@@ -678,6 +689,8 @@ int torch_batch_destroy(torch_batch* b) {
     (void)hipHostFree(b->host_ctl);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
+    for (hipEvent_t ev : b->kev)
+        if (ev) (void)hipEventDestroy(ev);
     for (hipEvent_t ev : b->mark)
         if (ev) (void)hipEventDestroy(ev);
     for (hipStream_t s : b->side)
@@ -951,6 +964,44 @@ int torch_batch_set_tiles(torch_batch* b, int block_x, int block_y, int zchunk) 
     b->tiles_set = true;
     b->dirty = true;
     drop_graph(b);
+    return TORCH_OK;
+}
+
+int torch_batch_time_step_kernel(torch_batch* b, int steps, double* ms) {
+    if (!b || steps <= 0 || !ms) return fail(TORCH_INVALID, "invalid argument");
+    std::lock_guard<std::mutex> lock(b->ctx->mu);
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    int rc;
+    if (b->dirty && (rc = rebuild(b))) return rc;
+    int lists = 0;
+    for (int L = 0; L < 5; ++L) lists += b->hyd[0][L].blocks > 0;
+    if (lists != 1) return fail(TORCH_INVALID, "step kernel timing needs exactly one 3D work list");
+    if (!b->kev[0]) {
+        HIP_TRY(hipEventCreate(&b->kev[0]));
+        HIP_TRY(hipEventCreate(&b->kev[1]));
+    }
+    hipStream_t st = b->ctx->stream;
+    HIP_TRY(hipStreamSynchronize(st));
+    // A step graph with the event pair around the 3D work list, replayed one
+    // step at a time so each pair is read before the next replay records it.
+    b->kev_capture = true;
+    rc = capture(b);
+    b->kev_capture = false;
+    double total = 0;
+    for (int k = 0; !rc && k < steps; ++k) {
+        hipError_t e = hipGraphLaunch(b->graph, st);
+        if (e == hipSuccess) e = hipEventSynchronize(b->kev[1]);
+        float t = 0;
+        if (e == hipSuccess) e = hipEventElapsedTime(&t, b->kev[0], b->kev[1]);
+        if (e != hipSuccess) rc = fail(TORCH_HIP_ERROR, std::string("step kernel timing: ") + hipGetErrorString(e));
+        total += t;
+    }
+    if (!rc) rc = read_ctl(b);
+    // Later runs replay the ordinary step graph (no event nodes).
+    const int rc2 = capture(b);
+    if (rc) return rc;
+    if (rc2) return rc2;
+    *ms = total / steps;
     return TORCH_OK;
 }
 

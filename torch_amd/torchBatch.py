@@ -120,6 +120,11 @@ class Batch:
         """Average ms of one 3D second-order work-list launch (nothing advances)."""
         return self._batch.time_pass(reps)
 
+    def time_step_kernel(self, steps=10):
+        """Advance `steps` steps and return the mean ms of the batch's one 3D
+        work-list launch within them (HIP events inside the step graph)."""
+        return self._batch.time_step_kernel(steps)
+
     # -- results -------------------------------------------------------------
     def info(self, ident):
         """Progress of an instance. Its boundary and source integrals are the
